@@ -1,0 +1,599 @@
+// Device-resident CGLS for GaussNewtonInversion::solve_cg
+// (src/Inversion/GaussNewtonInversion.cpp:1417-1698) and the Gauss-Newton
+// helpers around it (:856-861 barrier P, :877-911 model update, :913-916
+// misfit, :962-1022 IRLS weights).
+//
+// One CG iteration = 8 launches on the context stream, none of which waits on
+// the host:
+//   p_update  -> gemv_n(G, Pp) -> t_rows (+ t^2 partials) -> alpha (1 block)
+//   -> s_update (s += a t, dx += a p, w = Wd s_d) -> gemv_t(G, w)
+//   -> r_update (r = A^T s, + r^2 partials) -> scalar_end (1 block)
+// The scalars (r2, t2, alpha, beta, err, iterations) live in device memory;
+// when err < tol the scalar kernel sets `done` and every later kernel of the
+// queued iterations returns at once.  All reductions have a fixed shape, so a
+// solve is bit-for-bit repeatable.
+//
+// Operator association follows the reference's Eigen expressions: a term such
+// as sqrt(l) * P * Ws^T * WL_s^T * v is summed as ((sqrt(l)*P_i)*Ws_ji)*WL_j*v_j
+// over the row of the transposed operator.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+
+namespace gra {
+
+namespace {
+
+constexpr int kB = 256;
+
+struct Csr {
+    const int64_t *rp;
+    const int32_t *ci;
+    const double *v;
+};
+
+// Everything the row kernels need, passed by value (kernel argument space).
+struct CgDev {
+    int64_t nd, nm, ld, nrow;
+    int nseg;                 // 4, or 7 with the cross-gradient term
+    int quirks;
+    double sl;                // sqrt(lambda)
+    const double *Wd, *P, *dobs, *mk, *mref;
+    const double *WL[7];      // s, z, x, y, tz, tx, ty (null = identity)
+    const double *WLt_rows[3];// right-diagonals used for t rows 4..6
+    Csr A[7];                 // Ws, Wz, Wx, Wy, Tz, Tx, Ty (T scaled by a*S)
+    Csr AT[7];                // transposes
+    double *s, *t, *r, *p, *Pp, *dx, *w;
+    const double *part_n;
+    int splits;
+    const double *part_t;
+    int chunks;
+    double *red;
+    double *red2;
+    CgScalars *sc;
+};
+
+__device__ inline double dgv(const double *d, int64_t i) { return d ? d[i] : 1.0; }
+
+// Deterministic block sum (256 threads): xor butterfly per wave, then the
+// four wave sums in order.  Result valid in thread 0.
+__device__ inline double block_sum(double v)
+{
+    __shared__ double ws[kB / 64];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) ws[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        s = ws[0];
+        for (int q = 1; q < kB / 64; ++q) s += ws[q];
+    }
+    return s;
+}
+
+// Sum of n partials by one block, fixed order.
+__device__ inline double sum_partials(const double *red, int64_t n)
+{
+    double v = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += kB) v += red[k];
+    return block_sum(v);
+}
+
+// out = sum_k ((pre*L_i)*A_ik) [*R_k] * x_k over row i of A.
+__device__ inline double row_dot(const Csr &A, int64_t i, double pre, const double *L,
+                                 const double *R, const double *x)
+{
+    double li = pre * dgv(L, i);
+    double acc = 0.0;
+    for (int64_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+        int64_t j = A.ci[k];
+        double coef = li * A.v[k];
+        if (R) coef = coef * R[j];
+        acc += coef * x[j];
+    }
+    return acc;
+}
+
+// Segment q's left-multiplier for t rows (quirks: GN.cpp:1573, :1582-1584).
+__device__ inline void t_row_factors(const CgDev &d, int q, double &pre, const double *&L)
+{
+    if (q < 4) {
+        pre = d.sl;
+        L = (q == 0 && d.quirks) ? nullptr : d.WL[q];
+    } else {
+        pre = 1.0;
+        L = d.WLt_rows[q - 4];
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_init_scalars(CgScalars *sc)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        memset(sc, 0, sizeof(CgScalars));
+    }
+}
+
+// b and s = -b (GN.cpp:1459-1472); w = Wd * s_d for the first G^T product.
+__global__ void __launch_bounds__(kB) k_b_init(CgDev d)
+{
+    int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (g >= d.nrow) return;
+    double b;
+    if (g < d.nd) {
+        double y = d.part_n[g];
+        for (int q = 1; q < d.splits; ++q) y += d.part_n[(int64_t)q * d.nd + g];
+        b = dgv(d.Wd, g) * (d.dobs[g] - y);
+        d.w[g] = dgv(d.Wd, g) * (-b);
+    } else {
+        int64_t h = g - d.nd;
+        int q = (int)(h / d.nm);
+        int64_t i = h - (int64_t)q * d.nm;
+        if (q == 0) {
+            // -sqrt(l) * WL_s * Ws * (mk - mref)
+            double li = -d.sl * dgv(d.WL[0], i);
+            double acc = 0.0;
+            for (int64_t k = d.A[0].rp[i]; k < d.A[0].rp[i + 1]; ++k) {
+                int64_t j = d.A[0].ci[k];
+                acc += (li * d.A[0].v[k]) * (d.mk[j] - d.mref[j]);
+            }
+            b = acc;
+        } else if (q < 4) {
+            b = row_dot(d.A[q], i, -d.sl, d.WL[q], nullptr, d.mk);
+        } else {
+            b = row_dot(d.A[q], i, -1.0, d.WL[q], nullptr, d.mk);
+        }
+    }
+    d.s[g] = -b;
+}
+
+// r_j = P_j (G^T w)_j + sum_q (pre P_j) * AT_q row j * WL_q * seg_q   (A^T v).
+// mode 0: loop residual (vec8 uses WL_ty); mode 1: initial residual plus
+// A^T b, whose vec8 uses WL_y under quirks (GN.cpp:1508 vs :1544).
+template <int MODE>
+__global__ void __launch_bounds__(kB) k_r_update(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double r2 = 0.0, atb2 = 0.0;
+    if (j < d.nm) {
+        double gsum = d.part_t[j];
+        for (int c = 1; c < d.chunks; ++c) gsum += d.part_t[(int64_t)c * d.ld + j];
+        double Pj = d.P[j];
+        double r = Pj * gsum;
+        double atb = 0.0 + Pj * (-gsum);
+        const double *seg = d.s + d.nd;
+        for (int q = 0; q < d.nseg; ++q) {
+            double pre = q < 4 ? d.sl : 1.0;
+            const double *R = d.WL[q];
+            if (MODE == 1 && q == 6 && d.quirks) R = d.WL[3];   // WL_y (GN.cpp:1508)
+            double v = row_dot(d.AT[q], j, pre, d.P, R, seg + (int64_t)q * d.nm);
+            r = r + v;
+            if (MODE == 1) {
+                double vb = (q == 6 && d.quirks)
+                                ? -row_dot(d.AT[q], j, pre, d.P, d.WL[6], seg + (int64_t)q * d.nm)
+                                : -v;
+                atb += vb;
+            }
+        }
+        d.r[j] = r;
+        r2 = r * r;
+        atb2 = atb * atb;
+    }
+    double s1 = block_sum(r2);
+    if (threadIdx.x == 0) d.red[blockIdx.x] = s1;
+    if (MODE == 1) {
+        double s2 = block_sum(atb2);
+        if (threadIdx.x == 0) d.red2[blockIdx.x] = s2;
+    }
+}
+
+// After the initial residual: r2, ATb2.
+__global__ void __launch_bounds__(kB) k_scalar_init(CgDev d, int64_t nblk)
+{
+    double r2 = sum_partials(d.red, nblk);
+    double atb2 = sum_partials(d.red2, nblk);
+    if (threadIdx.x == 0) {
+        d.sc->r2 = r2;
+        d.sc->ATb2 = atb2;
+        d.sc->beta = 0.0;
+        d.sc->done = 0;
+        d.sc->iterations = 0;
+    }
+}
+
+// p = -r + beta p; Pp = P p   (GN.cpp:1558-1560)
+__global__ void __launch_bounds__(kB) k_p_update(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (j >= d.nm) return;
+    double beta = d.sc->beta;
+    double p = -d.r[j] + beta * d.p[j];
+    d.p[j] = p;
+    d.Pp[j] = d.P[j] * p;
+}
+
+// t = A Pp (GN.cpp:1562-1587) with t^2 block partials.
+__global__ void __launch_bounds__(kB) k_t_rows(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double t2 = 0.0;
+    if (g < d.nrow) {
+        double t;
+        if (g < d.nd) {
+            double y = d.part_n[g];
+            for (int q = 1; q < d.splits; ++q) y += d.part_n[(int64_t)q * d.nd + g];
+            t = dgv(d.Wd, g) * y;
+        } else {
+            int64_t h = g - d.nd;
+            int q = (int)(h / d.nm);
+            int64_t i = h - (int64_t)q * d.nm;
+            double pre;
+            const double *L;
+            t_row_factors(d, q, pre, L);
+            t = row_dot(d.A[q], i, pre, L, nullptr, d.Pp);
+        }
+        d.t[g] = t;
+        t2 = t * t;
+    }
+    double s = block_sum(t2);
+    if (threadIdx.x == 0) d.red[blockIdx.x] = s;
+}
+
+// alpha = r2 / t2   (GN.cpp:1588-1603)
+__global__ void __launch_bounds__(kB) k_alpha(CgDev d, int64_t nblk)
+{
+    if (d.sc->done) return;
+    double t2 = sum_partials(d.red, nblk);
+    if (threadIdx.x == 0) {
+        double a = d.sc->r2 / t2;
+        d.sc->t2 = t2;
+        d.sc->alpha = a;
+        if (!(t2 != 0.0) || !isfinite(a)) d.sc->breakdown = 1;
+    }
+}
+
+// s += alpha t; dx += alpha p; w = Wd s_d   (GN.cpp:1604-1629, 1634)
+__global__ void __launch_bounds__(kB) k_s_update(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double a = d.sc->alpha;
+    if (g < d.nrow) {
+        double s = d.s[g] + a * d.t[g];
+        d.s[g] = s;
+        if (g < d.nd) d.w[g] = dgv(d.Wd, g) * s;
+    }
+    if (g < d.nm) d.dx[g] = d.dx[g] + a * d.p[g];
+}
+
+// beta = r2/r2_last; err = r2/ATb2; stop test   (GN.cpp:1674-1691)
+__global__ void __launch_bounds__(kB) k_scalar_end(CgDev d, int64_t nblk, int iter, double tol)
+{
+    if (d.sc->done) return;
+    double r2 = sum_partials(d.red, nblk);
+    if (threadIdx.x == 0) {
+        CgScalars *sc = d.sc;
+        double last = sc->r2;
+        sc->r2_prev = last;
+        sc->r2 = r2;
+        sc->beta = r2 / last;
+        sc->err = r2 / sc->ATb2;
+        sc->iterations = iter + 1;
+        if (sc->err < tol) sc->done = 1;
+    }
+}
+
+// T*_scaled = (a*S_i) * T0_ij, and for the transpose (a*S_j) * T0_ji.
+__global__ void k_scale_T(const int64_t *rp, const int32_t *ci, const double *v0, int64_t n,
+                          double a, const double *S, int by_col, double *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+        double si = S ? (by_col ? S[ci[k]] : S[i]) : 1.0;
+        out[k] = (a * si) * v0[k];
+    }
+}
+
+__global__ void k_init_count(int *c) { *c = 0; }
+
+__global__ void k_fill(double *p, int64_t n, double v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- Gauss-Newton helpers ----
+
+__global__ void k_barrier_P(const double *m, const double *mmin, const double *mmax, int64_t n,
+                            double *P)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    P[k] = (mmax[k] - m[k]) * (m[k] - mmin[k]) / (mmax[k] - mmin[k]);
+}
+
+// Regular cells in parallel; cells hitting the reference's overflow /
+// underflow fallbacks are flagged and resolved in cell order afterwards.
+__global__ void k_barrier_main(const double *m, const double *dx, const double *mmin,
+                               const double *mmax, int64_t n, double *mnew, int *flags, int *count)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    double e = exp(dx[id]);
+    double t2 = mmin[id] * (mmax[id] - m[id]) + mmax[id] * (m[id] - mmin[id]) * e;
+    double t3 = (mmax[id] - m[id]) + (m[id] - mmin[id]) * e;
+    int f = 0;
+    if (isinf(e) || isinf(t2)) f = 1;
+    else if (fabs(t3) < 1e-10) f = 2;
+    else mnew[id] = t2 / t3;
+    flags[id] = f;
+    if (f) atomicAdd(count, 1);
+}
+
+__global__ void k_barrier_copy(const double *mnew, int64_t n, double *m_trial, const int *count)
+{
+    if (*count) return;
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) m_trial[id] = mnew[id];
+}
+
+__global__ void k_barrier_seq(const double *mnew, const double *mmin, const double *mmax,
+                              int64_t n, const int *flags, double *m_trial, const int *count)
+{
+    if (*count == 0 || threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int64_t id = 0; id < n; ++id) {
+        int f = flags[id];
+        if (f == 0) {
+            m_trial[id] = mnew[id];
+            continue;
+        }
+        double v = f == 1 ? (mmax[id] - 1e-6 + m_trial[id]) / 2.0
+                          : (mmin[id] + 1e-6 + m_trial[id]) / 2.0;
+        m_trial[id] = v;
+        int64_t nb = id == 0 ? (n > 1 ? 1 : 0) : id - 1;
+        m_trial[id] = 0.5 * (m_trial[id] + m_trial[nb]);
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+k_misfit_rows(const double *part, int splits, int64_t nd, const double *Wd, const double *dobs,
+              double *red)
+{
+    int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double v2 = 0.0;
+    if (i < nd) {
+        double y = part[i];
+        for (int q = 1; q < splits; ++q) y += part[(int64_t)q * nd + i];
+        double v = dgv(Wd, i) * (y - dobs[i]);
+        v2 = v * v;
+    }
+    double s = block_sum(v2);
+    if (threadIdx.x == 0) red[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kB) k_misfit_final(const double *red, int64_t nblk, int64_t nd,
+                                                     double *out)
+{
+    double s = sum_partials(red, nblk);
+    if (threadIdx.x == 0) *out = sqrt(s / (double)nd);
+}
+
+__global__ void k_irls(const int64_t *rp, const int32_t *ci, const double *v, int64_t n,
+                       const double *m, double e, double eps2, double *WL)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) x += v[k] * m[ci[k]];
+    WL[i] = pow(x * x + eps2, e);
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+
+gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const double *d_mref,
+                      const double *d_dobs, const CgDiag &dg, double *d_dx, double *err,
+                      int *iterations)
+{
+    hipStream_t st = c.stream;
+    const int64_t nd = c.Nd, nm = c.Nm, ld = c.ld;
+    const bool crg = a.use_cross_gradient != 0;
+    if (crg && !c.have_crg_ops)
+        return set_error(c, GRA_BAD_ARG, "cross-gradient requested but no T operators uploaded");
+    const int nseg = crg ? 7 : 4;
+    const int64_t nrow = nd + nseg * nm;
+    const int splits = gemv_n_splits(nd, nm);
+    const int chunks = gemv_t_chunks(nd, nm);
+    hipError_t e;
+
+    // work vectors
+    size_t need_rows = sizeof(double) * (size_t)nrow;
+    if ((e = c.vec_row[0].reserve(need_rows)) != hipSuccess) return hip_fail(c, e, "alloc s");
+    if ((e = c.vec_row[1].reserve(need_rows)) != hipSuccess) return hip_fail(c, e, "alloc t");
+    size_t vnm = sizeof(double) * (size_t)ld;
+    for (int q = 0; q < 3; ++q)
+        if ((e = c.vec_nm[q].reserve(vnm)) != hipSuccess) return hip_fail(c, e, "alloc vec");
+    if ((e = c.vec_nd[0].reserve(sizeof(double) * (size_t)nd)) != hipSuccess)
+        return hip_fail(c, e, "alloc w");
+    if ((e = c.part_n.reserve(sizeof(double) * (size_t)splits * nd)) != hipSuccess)
+        return hip_fail(c, e, "alloc part_n");
+    if ((e = c.part_t.reserve(sizeof(double) * (size_t)chunks * ld)) != hipSuccess)
+        return hip_fail(c, e, "alloc part_t");
+    int64_t nblk_row = nblocks(nrow), nblk_nm = nblocks(nm);
+    int64_t nred = nblk_row > nblk_nm ? nblk_row : nblk_nm;
+    if ((e = c.red.reserve(sizeof(double) * 2 * (size_t)nred)) != hipSuccess)
+        return hip_fail(c, e, "alloc red");
+    if ((e = c.scal.reserve(sizeof(CgScalars))) != hipSuccess) return hip_fail(c, e, "alloc sc");
+
+    // scaled cross-gradient operators
+    double *Tv[3] = {nullptr, nullptr, nullptr}, *TvT[3] = {nullptr, nullptr, nullptr};
+    DevCsr *Tops[3] = {&c.Tz0, &c.Tx0, &c.Ty0};
+    if (crg) {
+        int64_t tot = 0;
+        for (int q = 0; q < 3; ++q) tot += 2 * Tops[q]->nnz;
+        if ((e = c.vec_nm[3].reserve(sizeof(double) * (size_t)(tot > 0 ? tot : 1))) != hipSuccess)
+            return hip_fail(c, e, "alloc T");
+        double *base = c.vec_nm[3].as<double>();
+        for (int q = 0; q < 3; ++q) {
+            Tv[q] = base;
+            base += Tops[q]->nnz;
+            TvT[q] = base;
+            base += Tops[q]->nnz;
+            unsigned nb = nblocks(nm);
+            hipLaunchKernelGGL(k_scale_T, dim3(nb), dim3(kB), 0, st, Tops[q]->rp.as<int64_t>(),
+                               Tops[q]->ci.as<int32_t>(), Tops[q]->v.as<double>(), nm, a.a_crg,
+                               dg.S, 0, Tv[q]);
+            hipLaunchKernelGGL(k_scale_T, dim3(nb), dim3(kB), 0, st, Tops[q]->rpT.as<int64_t>(),
+                               Tops[q]->ciT.as<int32_t>(), Tops[q]->vT.as<double>(), nm, a.a_crg,
+                               dg.S, 1, TvT[q]);
+        }
+    }
+
+    CgDev d;
+    memset(&d, 0, sizeof(d));
+    d.nd = nd; d.nm = nm; d.ld = ld; d.nrow = nrow; d.nseg = nseg;
+    d.quirks = a.quirk_mode ? 1 : 0;
+    d.sl = std::sqrt(a.lambda);
+    d.Wd = dg.Wd; d.P = dg.P; d.dobs = d_dobs; d.mk = d_mk; d.mref = d_mref;
+    d.WL[0] = dg.WLs; d.WL[1] = dg.WLz; d.WL[2] = dg.WLx; d.WL[3] = dg.WLy;
+    d.WL[4] = dg.WLtz; d.WL[5] = dg.WLtx; d.WL[6] = dg.WLty;
+    if (d.quirks) { d.WLt_rows[0] = dg.WLz; d.WLt_rows[1] = dg.WLx; d.WLt_rows[2] = dg.WLy; }
+    else { d.WLt_rows[0] = dg.WLtz; d.WLt_rows[1] = dg.WLtx; d.WLt_rows[2] = dg.WLty; }
+    DevCsr *W[4] = {&c.Ws, &c.Wz, &c.Wx, &c.Wy};
+    for (int q = 0; q < 4; ++q) {
+        d.A[q] = Csr{W[q]->rp.as<int64_t>(), W[q]->ci.as<int32_t>(), W[q]->v.as<double>()};
+        d.AT[q] = Csr{W[q]->rpT.as<int64_t>(), W[q]->ciT.as<int32_t>(), W[q]->vT.as<double>()};
+    }
+    if (crg) {
+        for (int q = 0; q < 3; ++q) {
+            d.A[4 + q] = Csr{Tops[q]->rp.as<int64_t>(), Tops[q]->ci.as<int32_t>(), Tv[q]};
+            d.AT[4 + q] = Csr{Tops[q]->rpT.as<int64_t>(), Tops[q]->ciT.as<int32_t>(), TvT[q]};
+        }
+    }
+    d.s = c.vec_row[0].as<double>();
+    d.t = c.vec_row[1].as<double>();
+    d.r = c.vec_nm[0].as<double>();
+    d.p = c.vec_nm[1].as<double>();
+    d.Pp = c.vec_nm[2].as<double>();
+    d.dx = d_dx;
+    d.w = c.vec_nd[0].as<double>();
+    d.part_n = c.part_n.as<double>();
+    d.splits = splits;
+    d.part_t = c.part_t.as<double>();
+    d.chunks = chunks;
+    d.red = c.red.as<double>();
+    d.red2 = d.red + nred;
+    d.sc = c.scal.as<CgScalars>();
+
+    // zero p, Pp (incl. padding up to ld) and dx
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(ld)), dim3(kB), 0, st, d.p, ld, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(ld)), dim3(kB), 0, st, d.Pp, ld, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(nm)), dim3(kB), 0, st, d.dx, nm, 0.0);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(kB), 0, st, d.sc);
+
+    // b, s, first residual and A^T b
+    {
+        TimedRegion tr(c, kTimeGemvN);
+        if ((e = launch_gemv_n(c.G.p, c.store, nd, nm, ld, d_mk, c.part_n.as<double>(), splits,
+                               nullptr, st)))
+            return hip_fail(c, e, "gemv_n");
+    }
+    hipLaunchKernelGGL(k_b_init, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
+    {
+        TimedRegion tr(c, kTimeGemvT);
+        if ((e = launch_gemv_t(c.G.p, c.store, nd, nm, ld, d.w, c.part_t.as<double>(), chunks,
+                               nullptr, st)))
+            return hip_fail(c, e, "gemv_t");
+    }
+    hipLaunchKernelGGL(k_r_update<1>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+    hipLaunchKernelGGL(k_scalar_init, dim3(1), dim3(kB), 0, st, d, nblk_nm);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg init");
+
+    const int *done = &d.sc->done;
+    for (int it = 0; it < a.maxit; ++it) {
+        TimedRegion tr_it(c, kTimeCgIter);
+        hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+        {
+            TimedRegion tr(c, kTimeGemvN);
+            e = launch_gemv_n(c.G.p, c.store, nd, nm, ld, d.Pp, c.part_n.as<double>(), splits,
+                              done, st);
+            if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
+        }
+        hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
+        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, d, nblk_row);
+        hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
+        {
+            TimedRegion tr(c, kTimeGemvT);
+            e = launch_gemv_t(c.G.p, c.store, nd, nm, ld, d.w, c.part_t.as<double>(), chunks,
+                              done, st);
+            if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
+        }
+        hipLaunchKernelGGL(k_r_update<0>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+        hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_nm, it, a.tol);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
+    }
+
+    CgScalars hs;
+    if ((e = hipMemcpyAsync(&hs, d.sc, sizeof(hs), hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return hip_fail(c, e, "copy scalars");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(c, e, "cg sync");
+    if (a.maxit > 0 && err) *err = hs.err;
+    if (iterations) *iterations = a.maxit > 0 ? (hs.done ? hs.iterations : a.maxit) : 0;
+    if (hs.breakdown)
+        return set_error(c, GRA_NUMERIC_BREAKDOWN, "CG breakdown: t2 == 0 or non-finite alpha");
+    return GRA_OK;
+}
+
+hipError_t launch_barrier_P(const double *m, const double *mmin, const double *mmax, int64_t n,
+                            double *P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_barrier_P, dim3(nblocks(n)), dim3(kB), 0, st, m, mmin, mmax, n, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_barrier_update(const double *m, const double *dx, const double *mmin,
+                                 const double *mmax, int64_t n, double *m_trial, double *scratch,
+                                 int *flags, hipStream_t st)
+{
+    // flags[0] is the fallback counter, flags[1..n] the per-cell codes.
+    int *count = flags;
+    hipLaunchKernelGGL(k_init_count, dim3(1), dim3(1), 0, st, count);
+    hipLaunchKernelGGL(k_barrier_main, dim3(nblocks(n)), dim3(kB), 0, st, m, dx, mmin, mmax, n,
+                       scratch, flags + 1, count);
+    hipLaunchKernelGGL(k_barrier_copy, dim3(nblocks(n)), dim3(kB), 0, st, scratch, n, m_trial,
+                       count);
+    hipLaunchKernelGGL(k_barrier_seq, dim3(1), dim3(64), 0, st, scratch, mmin, mmax, n, flags + 1,
+                       m_trial, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_misfit(const double *part, int splits, int64_t Nd, const double *Wd,
+                         const double *dobs, double *red, double *out, hipStream_t st)
+{
+    int64_t nb = nblocks(Nd);
+    hipLaunchKernelGGL(k_misfit_rows, dim3((unsigned)nb), dim3(kB), 0, st, part, splits, Nd, Wd,
+                       dobs, red);
+    hipLaunchKernelGGL(k_misfit_final, dim3(1), dim3(kB), 0, st, red, nb, Nd, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_irls(const int64_t *rp, const int32_t *ci, const double *v, int64_t n,
+                       const double *m, int p, double eps2, double *WL, hipStream_t st)
+{
+    double ex = -(2 - p) / 4.0;
+    hipLaunchKernelGGL(k_irls, dim3(nblocks(n)), dim3(kB), 0, st, rp, ci, v, n, m, ex, eps2, WL);
+    return hipGetLastError();
+}
+
+}  // namespace gra

@@ -1,0 +1,700 @@
+// C-ABI entry points of libgraeng (declared in include/graeng.h).
+//
+// Each entry point validates its arguments on the host, takes the context
+// lock, enqueues device work on the context's own stream and, for calls with
+// host outputs, synchronises that stream before returning (the reference's
+// Fwd / solve_cg calls are synchronous).  Errors come back as gra_status with
+// a message in gra_last_error(); nothing throws across the boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace gra;
+
+namespace gra {
+
+gra_status set_error(Ctx &c, gra_status s, const std::string &msg)
+{
+    c.last_error = msg;
+    return s;
+}
+
+gra_status hip_fail(Ctx &c, hipError_t e, const char *what)
+{
+    c.last_error = std::string(what) + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory) return GRA_OOM;
+    return GRA_HIP_ERROR;
+}
+
+TimedRegion::TimedRegion(Ctx &ctx, int klass) : c(ctx), cls(klass)
+{
+    if (!c.timing) return;
+    while (c.ev_used + 2 > c.ev_pool.size()) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) return;
+        c.ev_pool.push_back(ev);
+    }
+    a = c.ev_pool[c.ev_used++];
+    b = c.ev_pool[c.ev_used++];
+    (void)hipEventRecord(a, c.stream);
+}
+
+TimedRegion::~TimedRegion()
+{
+    if (!a) return;
+    (void)hipEventRecord(b, c.stream);
+    c.ev_marks.push_back({cls, {a, b}});
+}
+
+namespace {
+
+// Host CSR -> device CSR plus its transpose (columns ascending per row).
+void transpose_csr(const gra_csr *h, std::vector<int64_t> &rp, std::vector<int32_t> &ci,
+                   std::vector<double> &v)
+{
+    int64_t n = h->n, nnz = h->nnz;
+    rp.assign((size_t)n + 1, 0);
+    ci.resize((size_t)nnz);
+    v.resize((size_t)nnz);
+    for (int64_t k = 0; k < nnz; ++k) rp[(size_t)h->col_idx[k] + 1]++;
+    for (int64_t i = 0; i < n; ++i) rp[(size_t)i + 1] += rp[(size_t)i];
+    std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t k = h->row_ptr[i]; k < h->row_ptr[i + 1]; ++k) {
+            int64_t j = h->col_idx[k];
+            int64_t q = pos[(size_t)j]++;
+            ci[(size_t)q] = (int32_t)i;
+            v[(size_t)q] = h->val[k];
+        }
+    }
+}
+
+bool csr_valid(const gra_csr *h, int64_t n)
+{
+    if (!h || h->n != n || h->nnz < 0 || !h->row_ptr) return false;
+    if (h->nnz > 0 && (!h->col_idx || !h->val)) return false;
+    if (h->row_ptr[0] != 0 || h->row_ptr[n] != h->nnz) return false;
+    for (int64_t i = 0; i < n; ++i) {
+        if (h->row_ptr[i + 1] < h->row_ptr[i]) return false;
+        for (int64_t k = h->row_ptr[i]; k < h->row_ptr[i + 1]; ++k)
+            if (h->col_idx[k] < 0 || h->col_idx[k] >= n) return false;
+    }
+    return true;
+}
+
+template <typename T>
+hipError_t put(DevBuf &b, const T *h, size_t n, hipStream_t st)
+{
+    hipError_t e = b.reserve(sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) return e;
+    if (n) e = hipMemcpyAsync(b.p, h, sizeof(T) * n, hipMemcpyHostToDevice, st);
+    return e;
+}
+
+// Uploads an Nm-vector into a buffer of ld entries with zero padding.
+hipError_t put_padded(DevBuf &b, const double *h, int64_t n, int64_t ld, hipStream_t st)
+{
+    hipError_t e = b.reserve(sizeof(double) * (size_t)ld);
+    if (e != hipSuccess) return e;
+    if (ld > n) {
+        e = hipMemsetAsync(b.as<double>() + n, 0, sizeof(double) * (size_t)(ld - n), st);
+        if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(b.p, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, st);
+}
+
+bool prisms_valid(const double *pr, int64_t n)
+{
+    for (int64_t j = 0; j < n; ++j) {
+        const double *p = pr + 6 * j;
+        for (int q = 0; q < 6; ++q)
+            if (!std::isfinite(p[q])) return false;
+        if (!(p[0] < p[1] && p[2] < p[3] && p[4] < p[5])) return false;
+    }
+    return true;
+}
+
+bool points_valid(const double *pt, int64_t n)
+{
+    for (int64_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(pt[i])) return false;
+    return true;
+}
+
+int ncomp(uint32_t flag) { return __builtin_popcount(flag & 0x3FEu); }
+
+size_t dtype_size(gra_dtype d) { return d == GRA_F64 ? 8 : (d == GRA_F32 ? 4 : 2); }
+
+}  // namespace
+
+gra_status upload_csr(Ctx &c, DevCsr &d, const gra_csr *h)
+{
+    std::vector<int64_t> rpT;
+    std::vector<int32_t> ciT;
+    std::vector<double> vT;
+    transpose_csr(h, rpT, ciT, vT);
+    hipError_t e;
+    if ((e = put(d.rp, h->row_ptr, (size_t)h->n + 1, c.stream)) != hipSuccess ||
+        (e = put(d.ci, h->col_idx, (size_t)h->nnz, c.stream)) != hipSuccess ||
+        (e = put(d.v, h->val, (size_t)h->nnz, c.stream)) != hipSuccess ||
+        (e = put(d.rpT, rpT.data(), rpT.size(), c.stream)) != hipSuccess ||
+        (e = put(d.ciT, ciT.data(), ciT.size(), c.stream)) != hipSuccess ||
+        (e = put(d.vT, vT.data(), vT.size(), c.stream)) != hipSuccess)
+        return hip_fail(c, e, "upload CSR");
+    // host vectors die at return: finish the copies first
+    if ((e = hipStreamSynchronize(c.stream)) != hipSuccess) return hip_fail(c, e, "upload CSR");
+    d.n = h->n;
+    d.nnz = h->nnz;
+    d.present = true;
+    return GRA_OK;
+}
+
+}  // namespace gra
+
+#define CTX_GUARD(ctx)                                  \
+    if (!(ctx)) return GRA_BAD_ARG;                     \
+    Ctx &c = (ctx)->c;                                  \
+    std::lock_guard<std::mutex> lock_(c.mu);            \
+    c.last_error.clear();                               \
+    if (hipSetDevice(c.device) != hipSuccess)           \
+        return set_error(c, GRA_HIP_ERROR, "hipSetDevice failed");
+
+#define HIPCHK(expr, what)                                  \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(c, e_, what); \
+    } while (0)
+
+extern "C" {
+
+const char *gra_status_string(int s)
+{
+    switch (s) {
+    case GRA_OK: return "ok";
+    case GRA_BAD_ARG: return "bad argument";
+    case GRA_OOM: return "out of memory";
+    case GRA_HIP_ERROR: return "GPU runtime error";
+    case GRA_NUMERIC_BREAKDOWN: return "numeric breakdown";
+    case GRA_DEGENERATE_GEOMETRY: return "degenerate geometry";
+    case GRA_NO_DEVICE: return "no GPU device";
+    default: return "unknown status";
+    }
+}
+
+const char *gra_last_error(const gra_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : ""; }
+
+gra_status gra_ctx_create(int device, gra_ctx **out)
+{
+    if (!out) return GRA_BAD_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GRA_NO_DEVICE;
+    if (device < 0 || device >= n) return GRA_BAD_ARG;
+    if (hipSetDevice(device) != hipSuccess) return GRA_HIP_ERROR;
+    gra_ctx *ctx = new (std::nothrow) gra_ctx();
+    if (!ctx) return GRA_OOM;
+    ctx->c.device = device;
+    if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return GRA_HIP_ERROR;
+    }
+    if (ctx->c.status.reserve(sizeof(int) * 4) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return GRA_OOM;
+    }
+    *out = ctx;
+    return GRA_OK;
+}
+
+void gra_ctx_destroy(gra_ctx *ctx)
+{
+    if (!ctx) return;
+    {
+        Ctx &c = ctx->c;
+        (void)hipSetDevice(c.device);
+        (void)hipStreamSynchronize(c.stream);
+        for (hipEvent_t ev : c.ev_pool) (void)hipEventDestroy(ev);
+        c.ev_pool.clear();
+        (void)hipStreamDestroy(c.stream);
+    }
+    delete ctx;
+}
+
+gra_status gra_sync(gra_ctx *ctx)
+{
+    CTX_GUARD(ctx);
+    HIPCHK(hipStreamSynchronize(c.stream), "sync");
+    return GRA_OK;
+}
+
+gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const double *pts,
+                        int64_t N_obs, uint32_t field_flag, gra_dtype store)
+{
+    CTX_GUARD(ctx);
+    if (!prisms || !pts || Nm <= 0 || N_obs <= 0)
+        return set_error(c, GRA_BAD_ARG, "empty mesh or point set");
+    if ((field_flag & 1u) || (field_flag & ~0x3FFu) || !(field_flag & 0x3FEu))
+        return set_error(c, GRA_BAD_ARG, "field_flag must select components among bits 1..9");
+    if (store != GRA_F64 && store != GRA_F32 && store != GRA_BF16)
+        return set_error(c, GRA_BAD_ARG, "unknown storage type");
+    if (Nm > INT32_MAX) return set_error(c, GRA_BAD_ARG, "Nm exceeds int32 column indices");
+    if (!prisms_valid(prisms, Nm))
+        return set_error(c, GRA_BAD_ARG, "prism bounds must be finite with x0<x1, y0<y1, z0<z1");
+    if (!points_valid(pts, N_obs)) return set_error(c, GRA_BAD_ARG, "non-finite point");
+    const int nc = ncomp(field_flag);
+    const int64_t Nd = (int64_t)nc * N_obs;
+    const int64_t ld = (Nm + 63) / 64 * 64;
+    c.G.release();
+    c.Nd = c.Nm = c.N_obs = c.ld = 0;
+    c.gn_ready = false;
+    HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
+    if (ld > Nm) HIPCHK(hipMemsetAsync(c.G.p, 0, c.G.bytes, c.stream), "clear G padding");
+    DevBuf dpr, dpt;
+    HIPCHK(put(dpr, prisms, (size_t)(6 * Nm), c.stream), "upload prisms");
+    HIPCHK(put(dpt, pts, (size_t)(3 * N_obs), c.stream), "upload points");
+    int *bad = c.status.as<int>();
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), c.stream), "clear status");
+    {
+        TimedRegion tr(c, kTimeAssemble);
+        HIPCHK(launch_assemble(dpr.as<double>(), Nm, dpt.as<double>(), N_obs, field_flag, store,
+                               c.G.p, ld, bad, c.stream),
+               "assemble kernel");
+    }
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream), "status");
+    HIPCHK(hipStreamSynchronize(c.stream), "assemble");
+    c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    if (hbad)
+        return set_error(c, GRA_DEGENERATE_GEOMETRY,
+                         "an observation point lies on a prism edge (s0*s1 <= 0, Integral.cpp:77)");
+    return GRA_OK;
+}
+
+gra_status gra_get_dims(const gra_ctx *ctx, int64_t *Nd, int64_t *Nm, int64_t *N_obs,
+                        uint32_t *field_flag, int *store)
+{
+    if (!ctx) return GRA_BAD_ARG;
+    const Ctx &c = ctx->c;
+    if (Nd) *Nd = c.Nd;
+    if (Nm) *Nm = c.Nm;
+    if (N_obs) *N_obs = c.N_obs;
+    if (field_flag) *field_flag = c.flag;
+    if (store) *store = (int)c.store;
+    return GRA_OK;
+}
+
+gra_status gra_copy_G(gra_ctx *ctx, double *out, int col_major)
+{
+    CTX_GUARD(ctx);
+    if (!out) return set_error(c, GRA_BAD_ARG, "null output");
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    DevBuf tmp;
+    HIPCHK(tmp.reserve(sizeof(double) * (size_t)c.Nd * (size_t)c.Nm), "allocate copy");
+    HIPCHK(launch_expand_G(c.G.p, c.store, c.Nd, c.Nm, c.ld, tmp.as<double>(), col_major, c.stream),
+           "expand G");
+    HIPCHK(hipMemcpyAsync(out, tmp.p, sizeof(double) * (size_t)c.Nd * (size_t)c.Nm,
+                          hipMemcpyDeviceToHost, c.stream),
+           "copy G");
+    HIPCHK(hipStreamSynchronize(c.stream), "copy G");
+    return GRA_OK;
+}
+
+static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *product)
+{
+    CTX_GUARD(ctx);
+    if (!vec || !product) return set_error(c, GRA_BAD_ARG, "null vector");
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    const int64_t nin = trans ? c.Nd : c.Nm, nout = trans ? c.Nm : c.Nd;
+    DevBuf &x = c.vec_nd[1], &y = c.vec_nd[2];
+    if (trans) HIPCHK(put(x, vec, (size_t)nin, c.stream), "upload vec");
+    else HIPCHK(put_padded(x, vec, c.Nm, c.ld, c.stream), "upload vec");
+    HIPCHK(y.reserve(sizeof(double) * (size_t)nout), "alloc out");
+    if (!trans) {
+        int splits = gemv_n_splits(c.Nd, c.Nm);
+        HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)splits * c.Nd), "alloc part");
+        {
+            TimedRegion tr(c, kTimeGemvN);
+            HIPCHK(launch_gemv_n(c.G.p, c.store, c.Nd, c.Nm, c.ld, x.as<double>(),
+                                 c.part_n.as<double>(), splits, nullptr, c.stream),
+                   "gemv_n");
+        }
+        HIPCHK(launch_reduce_parts(c.part_n.as<double>(), splits, c.Nd, c.Nd, nullptr,
+                                   y.as<double>(), c.stream),
+               "reduce");
+    } else {
+        int chunks = gemv_t_chunks(c.Nd, c.Nm);
+        HIPCHK(c.part_t.reserve(sizeof(double) * (size_t)chunks * c.ld), "alloc part");
+        {
+            TimedRegion tr(c, kTimeGemvT);
+            HIPCHK(launch_gemv_t(c.G.p, c.store, c.Nd, c.Nm, c.ld, x.as<double>(),
+                                 c.part_t.as<double>(), chunks, nullptr, c.stream),
+                   "gemv_t");
+        }
+        HIPCHK(launch_reduce_parts(c.part_t.as<double>(), chunks, c.ld, c.Nm, nullptr,
+                                   y.as<double>(), c.stream),
+               "reduce");
+    }
+    HIPCHK(hipMemcpyAsync(product, y.p, sizeof(double) * (size_t)nout, hipMemcpyDeviceToHost,
+                          c.stream),
+           "download");
+    HIPCHK(hipStreamSynchronize(c.stream), "gemv");
+    return GRA_OK;
+}
+
+gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product)
+{
+    return gemv_host(ctx, 0, vec, product);
+}
+
+gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product)
+{
+    return gemv_host(ctx, 1, vec, product);
+}
+
+gra_status gra_forward(gra_ctx *ctx, const double *prisms, const double *rho, int64_t Nm,
+                       const double *pts, int64_t N_obs, uint32_t field_flag, double *out)
+{
+    CTX_GUARD(ctx);
+    if (!prisms || !rho || !pts || !out || Nm <= 0 || N_obs <= 0)
+        return set_error(c, GRA_BAD_ARG, "empty input");
+    if ((field_flag & 1u) || (field_flag & ~0x3FFu) || !(field_flag & 0x3FEu))
+        return set_error(c, GRA_BAD_ARG, "field_flag must select components among bits 1..9");
+    if (!prisms_valid(prisms, Nm))
+        return set_error(c, GRA_BAD_ARG, "prism bounds must be finite with x0<x1, y0<y1, z0<z1");
+    if (!points_valid(pts, N_obs)) return set_error(c, GRA_BAD_ARG, "non-finite point");
+    const int nc = ncomp(field_flag);
+    DevBuf dpr, drho, dpt, dout;
+    HIPCHK(put(dpr, prisms, (size_t)(6 * Nm), c.stream), "upload prisms");
+    HIPCHK(put(drho, rho, (size_t)Nm, c.stream), "upload rho");
+    HIPCHK(put(dpt, pts, (size_t)(3 * N_obs), c.stream), "upload points");
+    HIPCHK(dout.reserve(sizeof(double) * (size_t)N_obs * nc), "alloc out");
+    int *bad = c.status.as<int>();
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), c.stream), "clear status");
+    HIPCHK(launch_forward(dpr.as<double>(), drho.as<double>(), Nm, dpt.as<double>(), N_obs,
+                          field_flag, dout.as<double>(), bad, c.stream),
+           "forward kernel");
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(double) * (size_t)N_obs * nc, hipMemcpyDeviceToHost,
+                          c.stream),
+           "download");
+    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream), "status");
+    HIPCHK(hipStreamSynchronize(c.stream), "forward");
+    if (hbad)
+        return set_error(c, GRA_DEGENERATE_GEOMETRY,
+                         "an observation point lies on a prism edge (s0*s1 <= 0, Integral.cpp:77)");
+    return GRA_OK;
+}
+
+gra_status gra_set_regularization(gra_ctx *ctx, const gra_csr *Ws, const gra_csr *Wx,
+                                  const gra_csr *Wy, const gra_csr *Wz, const gra_csr *Tx0,
+                                  const gra_csr *Ty0, const gra_csr *Tz0)
+{
+    CTX_GUARD(ctx);
+    if (c.Nm == 0) return set_error(c, GRA_BAD_ARG, "assemble G first (Nm unknown)");
+    const int64_t n = c.Nm;
+    if (!csr_valid(Ws, n) || !csr_valid(Wx, n) || !csr_valid(Wy, n) || !csr_valid(Wz, n))
+        return set_error(c, GRA_BAD_ARG, "W operators must be valid Nm x Nm CSR matrices");
+    const bool crg = Tx0 || Ty0 || Tz0;
+    if (crg && (!csr_valid(Tx0, n) || !csr_valid(Ty0, n) || !csr_valid(Tz0, n)))
+        return set_error(c, GRA_BAD_ARG, "T operators must be valid Nm x Nm CSR matrices");
+    gra_status s;
+    if ((s = upload_csr(c, c.Ws, Ws)) || (s = upload_csr(c, c.Wx, Wx)) ||
+        (s = upload_csr(c, c.Wy, Wy)) || (s = upload_csr(c, c.Wz, Wz)))
+        return s;
+    c.have_crg_ops = false;
+    if (crg) {
+        if ((s = upload_csr(c, c.Tx0, Tx0)) || (s = upload_csr(c, c.Ty0, Ty0)) ||
+            (s = upload_csr(c, c.Tz0, Tz0)))
+            return s;
+        c.have_crg_ops = true;
+    }
+    return GRA_OK;
+}
+
+gra_status gra_solve_cg(gra_ctx *ctx, const gra_cg_args *a, double *dx, double *err,
+                        int *iterations)
+{
+    CTX_GUARD(ctx);
+    if (!a || !dx || !a->mk || !a->mref || !a->Wd || !a->dobs || !a->P)
+        return set_error(c, GRA_BAD_ARG, "missing solve_cg argument");
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    if (!c.Ws.present || c.Ws.n != c.Nm)
+        return set_error(c, GRA_BAD_ARG, "regularisation operators not uploaded for this Nm");
+    if (a->maxit < 0 || !(a->lambda >= 0)) return set_error(c, GRA_BAD_ARG, "maxit/lambda");
+    const int64_t nm = c.Nm, nd = c.Nd, ld = c.ld;
+    // input arena: mk(ld) mref P dx WL*7 S (ld each) | Wd dobs (nd each)
+    DevBuf &in = c.host_io;
+    const size_t nvec = 12;
+    HIPCHK(in.reserve(sizeof(double) * ((size_t)ld * nvec + 2 * (size_t)nd)), "alloc inputs");
+    double *base = in.as<double>();
+    double *d_mk = base, *d_mref = base + ld, *d_P = base + 2 * ld, *d_dx = base + 3 * ld;
+    double *d_WL = base + 4 * ld, *d_S = base + 11 * ld;
+    double *d_Wd = base + 12 * ld, *d_dobs = d_Wd + nd;
+    HIPCHK(hipMemsetAsync(d_mk, 0, sizeof(double) * (size_t)ld, c.stream), "clear");
+    HIPCHK(hipMemcpyAsync(d_mk, a->mk, sizeof(double) * nm, hipMemcpyHostToDevice, c.stream), "mk");
+    HIPCHK(hipMemcpyAsync(d_mref, a->mref, sizeof(double) * nm, hipMemcpyHostToDevice, c.stream),
+           "mref");
+    HIPCHK(hipMemcpyAsync(d_P, a->P, sizeof(double) * nm, hipMemcpyHostToDevice, c.stream), "P");
+    HIPCHK(hipMemcpyAsync(d_Wd, a->Wd, sizeof(double) * nd, hipMemcpyHostToDevice, c.stream), "Wd");
+    HIPCHK(hipMemcpyAsync(d_dobs, a->dobs, sizeof(double) * nd, hipMemcpyHostToDevice, c.stream),
+           "dobs");
+    const double *hWL[7] = {a->WL_s, a->WL_z, a->WL_x, a->WL_y, a->WL_tz, a->WL_tx, a->WL_ty};
+    const double *dWL[7];
+    for (int q = 0; q < 7; ++q) {
+        if (hWL[q]) {
+            HIPCHK(hipMemcpyAsync(d_WL + q * ld, hWL[q], sizeof(double) * nm,
+                                  hipMemcpyHostToDevice, c.stream),
+                   "WL");
+            dWL[q] = d_WL + q * ld;
+        } else {
+            dWL[q] = nullptr;
+        }
+    }
+    const double *dS = nullptr;
+    if (a->S_crg) {
+        HIPCHK(hipMemcpyAsync(d_S, a->S_crg, sizeof(double) * nm, hipMemcpyHostToDevice, c.stream),
+               "S");
+        dS = d_S;
+    }
+    CgDiag dg{d_Wd, d_P, dWL[0], dWL[2], dWL[3], dWL[1], dWL[5], dWL[6], dWL[4], dS};
+    gra_status s = cgls_solve(c, *a, d_mk, d_mref, d_dobs, dg, d_dx, err, iterations);
+    if (s != GRA_OK && s != GRA_NUMERIC_BREAKDOWN) return s;
+    HIPCHK(hipMemcpyAsync(dx, d_dx, sizeof(double) * nm, hipMemcpyDeviceToHost, c.stream), "dx");
+    HIPCHK(hipStreamSynchronize(c.stream), "solve_cg");
+    return s;
+}
+
+gra_status gra_barrier_P(gra_ctx *ctx, const double *m, const double *m_min, const double *m_max,
+                         double *P)
+{
+    CTX_GUARD(ctx);
+    if (!m || !m_min || !m_max || !P || c.Nm == 0) return set_error(c, GRA_BAD_ARG, "args");
+    const int64_t n = c.Nm;
+    DevBuf &b = c.vec_nm[8];
+    HIPCHK(b.reserve(sizeof(double) * 4 * (size_t)n), "alloc");
+    double *dm = b.as<double>(), *dlo = dm + n, *dhi = dm + 2 * n, *dP = dm + 3 * n;
+    HIPCHK(hipMemcpyAsync(dm, m, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "m");
+    HIPCHK(hipMemcpyAsync(dlo, m_min, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "min");
+    HIPCHK(hipMemcpyAsync(dhi, m_max, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "max");
+    HIPCHK(launch_barrier_P(dm, dlo, dhi, n, dP, c.stream), "barrier_P");
+    HIPCHK(hipMemcpyAsync(P, dP, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream), "P");
+    HIPCHK(hipStreamSynchronize(c.stream), "barrier_P");
+    return GRA_OK;
+}
+
+gra_status gra_barrier_update(gra_ctx *ctx, const double *m, const double *dx, const double *m_min,
+                              const double *m_max, double *m_trial)
+{
+    CTX_GUARD(ctx);
+    if (!m || !dx || !m_min || !m_max || !m_trial || c.Nm == 0)
+        return set_error(c, GRA_BAD_ARG, "args");
+    const int64_t n = c.Nm;
+    DevBuf &b = c.vec_nm[9];
+    HIPCHK(b.reserve(sizeof(double) * 6 * (size_t)n + sizeof(int) * (size_t)(n + 1)), "alloc");
+    double *dm = b.as<double>(), *ddx = dm + n, *dlo = dm + 2 * n, *dhi = dm + 3 * n,
+           *dmt = dm + 4 * n, *scr = dm + 5 * n;
+    int *flags = reinterpret_cast<int *>(dm + 6 * n);
+    HIPCHK(hipMemcpyAsync(dm, m, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "m");
+    HIPCHK(hipMemcpyAsync(ddx, dx, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "dx");
+    HIPCHK(hipMemcpyAsync(dlo, m_min, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "min");
+    HIPCHK(hipMemcpyAsync(dhi, m_max, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "max");
+    HIPCHK(hipMemcpyAsync(dmt, m_trial, sizeof(double) * n, hipMemcpyHostToDevice, c.stream), "mt");
+    HIPCHK(launch_barrier_update(dm, ddx, dlo, dhi, n, dmt, scr, flags, c.stream), "update");
+    HIPCHK(hipMemcpyAsync(m_trial, dmt, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream), "mt");
+    HIPCHK(hipStreamSynchronize(c.stream), "barrier_update");
+    return GRA_OK;
+}
+
+gra_status gra_misfit(gra_ctx *ctx, const double *m, const double *Wd, const double *dobs,
+                      double *misfit)
+{
+    CTX_GUARD(ctx);
+    if (!m || !Wd || !dobs || !misfit) return set_error(c, GRA_BAD_ARG, "args");
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    const int64_t nd = c.Nd;
+    DevBuf &x = c.vec_nm[10], &v = c.vec_nd[3];
+    HIPCHK(put_padded(x, m, c.Nm, c.ld, c.stream), "m");
+    HIPCHK(v.reserve(sizeof(double) * 2 * (size_t)nd), "alloc");
+    HIPCHK(hipMemcpyAsync(v.p, Wd, sizeof(double) * nd, hipMemcpyHostToDevice, c.stream), "Wd");
+    HIPCHK(hipMemcpyAsync(v.as<double>() + nd, dobs, sizeof(double) * nd, hipMemcpyHostToDevice,
+                          c.stream),
+           "dobs");
+    int splits = gemv_n_splits(c.Nd, c.Nm);
+    HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)splits * nd), "alloc");
+    HIPCHK(c.red.reserve(sizeof(double) * (size_t)((nd + 255) / 256 + 1)), "alloc");
+    HIPCHK(c.vec_nm[11].reserve(sizeof(double)), "alloc");
+    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, c.Nm, c.ld, x.as<double>(), c.part_n.as<double>(),
+                         splits, nullptr, c.stream),
+           "gemv_n");
+    HIPCHK(launch_misfit(c.part_n.as<double>(), splits, nd, v.as<double>(), v.as<double>() + nd,
+                         c.red.as<double>(), c.vec_nm[11].as<double>(), c.stream),
+           "misfit");
+    HIPCHK(hipMemcpyAsync(misfit, c.vec_nm[11].p, sizeof(double), hipMemcpyDeviceToHost, c.stream),
+           "misfit");
+    HIPCHK(hipStreamSynchronize(c.stream), "misfit");
+    return GRA_OK;
+}
+
+gra_status gra_irls_weights(gra_ctx *ctx, const gra_csr *W, const double *m, int p, double eps2,
+                            double *WL)
+{
+    CTX_GUARD(ctx);
+    if (!W || !m || !WL || W->n <= 0 || !csr_valid(W, W->n))
+        return set_error(c, GRA_BAD_ARG, "args");
+    const int64_t n = W->n;
+    DevBuf rp, ci, v, dm, dout;
+    HIPCHK(put(rp, W->row_ptr, (size_t)n + 1, c.stream), "rp");
+    HIPCHK(put(ci, W->col_idx, (size_t)W->nnz, c.stream), "ci");
+    HIPCHK(put(v, W->val, (size_t)W->nnz, c.stream), "v");
+    HIPCHK(put(dm, m, (size_t)n, c.stream), "m");
+    HIPCHK(dout.reserve(sizeof(double) * (size_t)n), "alloc");
+    HIPCHK(launch_irls(rp.as<int64_t>(), ci.as<int32_t>(), v.as<double>(), n, dm.as<double>(), p,
+                       eps2, dout.as<double>(), c.stream),
+           "irls");
+    HIPCHK(hipMemcpyAsync(WL, dout.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c.stream),
+           "WL");
+    HIPCHK(hipStreamSynchronize(c.stream), "irls");
+    return GRA_OK;
+}
+
+gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
+{
+    CTX_GUARD(ctx);
+    if (!s || !s->dobs || !s->Wd || !s->m_min || !s->m_max || !s->mref || !s->m_ini)
+        return set_error(c, GRA_BAD_ARG, "missing setup array");
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    const int64_t nd = c.Nd, nm = c.Nm, ld = c.ld;
+    HIPCHK(put(c.gn_dobs, s->dobs, (size_t)nd, c.stream), "dobs");
+    HIPCHK(put(c.gn_Wd, s->Wd, (size_t)nd, c.stream), "Wd");
+    HIPCHK(put(c.gn_mmin, s->m_min, (size_t)nm, c.stream), "m_min");
+    HIPCHK(put(c.gn_mmax, s->m_max, (size_t)nm, c.stream), "m_max");
+    HIPCHK(put(c.gn_mref, s->mref, (size_t)nm, c.stream), "mref");
+    // The reference's start model rule (GaussNewtonInversion.cpp:769-780):
+    // values at or beyond a bound move to the middle of the interval.
+    std::vector<double> m0(s->m_ini, s->m_ini + nm);
+    for (int64_t i = 0; i < nm; ++i) {
+        double lo = s->m_min[i], hi = s->m_max[i];
+        if (m0[i] > hi || std::fabs(m0[i] - hi) < 1e-7) m0[i] = 0.5 * (hi + lo);
+        if (m0[i] < lo || std::fabs(m0[i] - lo) < 1e-7) m0[i] = 0.5 * (hi + lo);
+    }
+    HIPCHK(put_padded(c.gn_m, m0.data(), nm, ld, c.stream), "m");
+    HIPCHK(c.gn_mtrial.reserve(sizeof(double) * (size_t)ld), "alloc");
+    HIPCHK(c.gn_P.reserve(sizeof(double) * (size_t)nm), "alloc");
+    HIPCHK(c.gn_dx.reserve(sizeof(double) * (size_t)nm), "alloc");
+    HIPCHK(c.gn_one.reserve(sizeof(double) * 2 * (size_t)nm + sizeof(int) * (size_t)(nm + 1) + 64),
+           "alloc");
+    HIPCHK(hipStreamSynchronize(c.stream), "gn setup");
+    c.gn_ready = true;
+    return GRA_OK;
+}
+
+gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit, double *misfit,
+                       double *cg_err, int *cg_iterations)
+{
+    CTX_GUARD(ctx);
+    if (!c.gn_ready) return set_error(c, GRA_BAD_ARG, "gra_gn_setup_problem not called");
+    if (!c.Ws.present || c.Ws.n != c.Nm)
+        return set_error(c, GRA_BAD_ARG, "regularisation operators not uploaded for this Nm");
+    if (cg_maxit < 0 || !(lambda >= 0)) return set_error(c, GRA_BAD_ARG, "maxit/lambda");
+    const int64_t nm = c.Nm, nd = c.Nd, ld = c.ld;
+    double *m = c.gn_m.as<double>(), *mt = c.gn_mtrial.as<double>();
+    double *scr = c.gn_one.as<double>();
+    double *dmis = scr + nm;
+    int *flags = reinterpret_cast<int *>(scr + nm + 1);
+    // P from the current model; m_trial starts from m (GN.cpp:825, :856-861)
+    HIPCHK(launch_barrier_P(m, c.gn_mmin.as<double>(), c.gn_mmax.as<double>(), nm,
+                            c.gn_P.as<double>(), c.stream),
+           "barrier_P");
+    HIPCHK(hipMemcpyAsync(mt, m, sizeof(double) * (size_t)ld, hipMemcpyDeviceToDevice, c.stream),
+           "m_trial");
+    gra_cg_args a;
+    memset(&a, 0, sizeof(a));
+    a.tol = cg_tol;
+    a.maxit = cg_maxit;
+    a.lambda = lambda;
+    a.quirk_mode = 1;
+    CgDiag dg{c.gn_Wd.as<double>(), c.gn_P.as<double>(), nullptr, nullptr, nullptr, nullptr,
+              nullptr, nullptr, nullptr, nullptr};
+    double err = NAN;
+    int its = 0;
+    gra_status s = cgls_solve(c, a, m, c.gn_mref.as<double>(), c.gn_dobs.as<double>(), dg,
+                              c.gn_dx.as<double>(), &err, &its);
+    if (s != GRA_OK) return s;
+    HIPCHK(launch_barrier_update(m, c.gn_dx.as<double>(), c.gn_mmin.as<double>(),
+                                 c.gn_mmax.as<double>(), nm, mt, scr, flags, c.stream),
+           "barrier_update");
+    int splits = gemv_n_splits(nd, nm);
+    HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)splits * nd), "alloc");
+    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, nm, ld, mt, c.part_n.as<double>(), splits, nullptr,
+                         c.stream),
+           "gemv_n");
+    HIPCHK(launch_misfit(c.part_n.as<double>(), splits, nd, c.gn_Wd.as<double>(),
+                         c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
+           "misfit");
+    // accept the trial model
+    HIPCHK(hipMemcpyAsync(m, mt, sizeof(double) * (size_t)nm, hipMemcpyDeviceToDevice, c.stream),
+           "accept");
+    double hm = NAN;
+    HIPCHK(hipMemcpyAsync(&hm, dmis, sizeof(double), hipMemcpyDeviceToHost, c.stream), "misfit");
+    HIPCHK(hipStreamSynchronize(c.stream), "gn step");
+    if (misfit) *misfit = hm;
+    if (cg_err) *cg_err = err;
+    if (cg_iterations) *cg_iterations = its;
+    return GRA_OK;
+}
+
+gra_status gra_gn_get_model(gra_ctx *ctx, double *m)
+{
+    CTX_GUARD(ctx);
+    if (!c.gn_ready || !m) return set_error(c, GRA_BAD_ARG, "no resident model");
+    HIPCHK(hipMemcpyAsync(m, c.gn_m.p, sizeof(double) * (size_t)c.Nm, hipMemcpyDeviceToHost,
+                          c.stream),
+           "model");
+    HIPCHK(hipStreamSynchronize(c.stream), "model");
+    return GRA_OK;
+}
+
+gra_status gra_timing_enable(gra_ctx *ctx, int on)
+{
+    CTX_GUARD(ctx);
+    c.timing = on != 0;
+    return GRA_OK;
+}
+
+gra_status gra_timing_reset(gra_ctx *ctx)
+{
+    CTX_GUARD(ctx);
+    HIPCHK(hipStreamSynchronize(c.stream), "sync");
+    c.ev_marks.clear();
+    c.ev_used = 0;
+    return GRA_OK;
+}
+
+gra_status gra_timing_get(gra_ctx *ctx, int which, double *avg_ms, int64_t *count)
+{
+    CTX_GUARD(ctx);
+    if (which < 0 || which >= kTimeClasses) return set_error(c, GRA_BAD_ARG, "timing class");
+    HIPCHK(hipStreamSynchronize(c.stream), "sync");
+    double tot = 0.0;
+    int64_t n = 0;
+    for (auto &mk : c.ev_marks) {
+        if (mk.first != which) continue;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, mk.second.first, mk.second.second), "elapsed");
+        tot += ms;
+        ++n;
+    }
+    if (avg_ms) *avg_ms = n ? tot / (double)n : 0.0;
+    if (count) *count = n;
+    return GRA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "graeng.h"
+
+namespace gra {
+
+// ------------------------------------------------------------ device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows to at least `n` bytes; contents are not preserved.
+    hipError_t reserve(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, n ? n : 256);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct DevCsr {
+    int64_t n = 0, nnz = 0;
+    DevBuf rp, ci, v;          // A
+    DevBuf rpT, ciT, vT;       // A^T (columns ascending in each row)
+    bool present = false;
+};
+
+// Device-resident scalar state of one CGLS solve.  Every CG kernel reads
+// `done` first and returns when it is set, so a captured or pre-queued
+// sequence of iterations drains as no-ops after convergence.
+struct CgScalars {
+    double r2;        // ||r||^2 of the current residual
+    double r2_prev;
+    double t2;
+    double alpha;
+    double beta;
+    double ATb2;
+    double err;
+    int done;
+    int iterations;
+    int breakdown;
+    int pad;
+};
+
+// Kernel timing classes (gra_timing_get).
+enum TimingClass { kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeClasses = 4 };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string last_error;
+
+    // G (row-major, leading dimension ld >= Nm, padding columns are zero)
+    DevBuf G;
+    int64_t Nd = 0, Nm = 0, N_obs = 0, ld = 0;
+    uint32_t flag = 0;
+    gra_dtype store = GRA_F64;
+
+    // regularisation operators
+    DevCsr Ws, Wx, Wy, Wz, Tx0, Ty0, Tz0;
+    bool have_crg_ops = false;
+
+    // scratch
+    DevBuf part_n, part_t;     // GEMV partial sums
+    DevBuf vec_nd[4], vec_nm[12], vec_row[3];
+    DevBuf red;                // block partial sums for reductions
+    DevBuf scal;               // CgScalars
+    DevBuf status;             // int flags written by kernels
+    DevBuf host_io;            // unused placeholder
+
+    // resident Gauss-Newton state
+    bool gn_ready = false;
+    DevBuf gn_dobs, gn_Wd, gn_mmin, gn_mmax, gn_mref, gn_m, gn_mtrial, gn_P, gn_dx, gn_one;
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_marks;
+};
+
+// Records a timed region on the context stream when timing is enabled.
+struct TimedRegion {
+    Ctx &c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedRegion(Ctx &ctx, int klass);
+    ~TimedRegion();
+};
+
+// ------------------------------------------------------------ launchers
+// All launchers enqueue on `st` and return the launch error.
+
+// G assembly (prism_kernels.hip): G[(k*N_obs+i)*ld + j] in `store` type.
+hipError_t launch_assemble(const double *d_prisms, int64_t Nm, const double *d_pts, int64_t N_obs,
+                           uint32_t flag, gra_dtype store, void *G, int64_t ld, int *d_bad,
+                           hipStream_t st);
+// Matrix-free forward: out[i*nc+k].
+hipError_t launch_forward(const double *d_prisms, const double *d_rho, int64_t Nm,
+                          const double *d_pts, int64_t N_obs, uint32_t flag, double *d_out,
+                          int *d_bad, hipStream_t st);
+
+// GEMV (linalg_kernels.hip).  gemv_n writes part[s*Nd + i] for s < splits.
+int gemv_n_splits(int64_t Nd, int64_t Nm);
+hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
+                         const double *x, double *part, int splits, const int *done,
+                         hipStream_t st);
+// gemv_t writes part[c*ld + j] for c < chunks.
+int gemv_t_chunks(int64_t Nd, int64_t Nm);
+hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
+                         const double *w, double *part, int chunks, const int *done,
+                         hipStream_t st);
+// y[i] = scale_i * sum_s part[s*stride + i]  (scale may be null => 1)
+hipError_t launch_reduce_parts(const double *part, int nparts, int64_t stride, int64_t n,
+                               const double *scale, double *y, hipStream_t st);
+
+// Copy G to a float64 host-layout device buffer (row- or column-major).
+hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
+                           double *out, int col_major, hipStream_t st);
+
+// CGLS (cgls.hip)
+struct CgDiag {
+    const double *Wd, *P;
+    const double *WLs, *WLx, *WLy, *WLz, *WLtx, *WLty, *WLtz;  // may be null (identity)
+    const double *S;                                            // may be null
+};
+gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const double *d_mref,
+                      const double *d_dobs, const CgDiag &dg, double *d_dx, double *err,
+                      int *iterations);
+hipError_t launch_barrier_P(const double *m, const double *mmin, const double *mmax, int64_t n,
+                            double *P, hipStream_t st);
+hipError_t launch_barrier_update(const double *m, const double *dx, const double *mmin,
+                                 const double *mmax, int64_t n, double *m_trial, double *scratch,
+                                 int *flags, hipStream_t st);
+hipError_t launch_misfit(const double *part, int splits, int64_t Nd, const double *Wd,
+                         const double *dobs, double *red, double *out, hipStream_t st);
+hipError_t launch_irls(const int64_t *rp, const int32_t *ci, const double *v, int64_t n,
+                       const double *m, int p, double eps2, double *WL, hipStream_t st);
+
+// Host helpers
+gra_status set_error(Ctx &c, gra_status s, const std::string &msg);
+gra_status hip_fail(Ctx &c, hipError_t e, const char *what);
+gra_status upload_csr(Ctx &c, DevCsr &d, const gra_csr *h);
+
+}  // namespace gra
+
+struct gra_ctx {
+    gra::Ctx c;
+};

@@ -1,0 +1,330 @@
+// Per-(point, prism) field evaluation in FP64, face/edge formulation.
+//
+// This is the reference's closed form (GravFormula.cpp:75-272 over
+// Integral::Surface_R_1 / getBeta / Surface_rR_3 / Surface_R_3 / Line_R_1,
+// Integral.cpp:59-85, 540-633, 764-823), specialised to axis-aligned prisms.
+//
+// Why the specialisation is exact.  For a RectPrism every edge vector has one
+// non-zero coordinate, so the reference's unit tangents t, unit face normals n
+// and in-plane edge normals m (unitCross / getUnit, Point.cpp:158-194) are
+// signed unit axis vectors *exactly* (sqrt(fl(d*d)) == |d| in binary IEEE
+// arithmetic, and products of +-1 with +-0 are exact).  A dot product with such
+// a vector therefore returns one signed, rounded coordinate difference, e.g.
+// (f - a).t == +-fl(f_x - a_x), bit for bit.  Every quantity below is formed
+// from the six differences D = fl(bound - a) and eight vertex distances
+// R = sqrt((dx^2 + dy^2) + dz^2) in the reference's operation order, so with
+// the same libm the results equal the reference's bit for bit (checked on the
+// host against the oracle); on the GPU only atan/log differ (device libm).
+//
+// Per face the edge table lists, in the reference's vertex order: the
+// coordinate differences giving s0, s1 (along the edge, signed by the
+// traversal direction), mv (along the outward in-plane edge normal m), the
+// vertex distances R0, R1 and, for the 1/R^3 term, the perpendicular distance
+// L0 of Point::projection_line (Point.cpp:232-237) including its rounding.
+#pragma once
+
+#ifndef GRA_HD
+#define GRA_HD __host__ __device__
+#endif
+
+#include <math.h>
+#include <stdint.h>
+
+namespace gra {
+
+constexpr double kTol = 1.000e-12;            // gs.h:19
+constexpr double kG0 = 6.673e-11;             // gs.h:20
+constexpr double kSI2Eotvos = 1000000000.0;   // gs.h:24
+constexpr double kSI2mGal = 100000.0;         // gs.h:25
+
+// Field bits of gs.h:46-57.
+enum : uint32_t {
+    kGz = 1u << 1, kGx = 1u << 2, kGy = 1u << 3, kTzz = 1u << 4, kTxz = 1u << 5,
+    kTyz = 1u << 6, kTxx = 1u << 7, kTxy = 1u << 8, kTyy = 1u << 9,
+    kAllFields = 0x3FEu
+};
+
+// One edge of a face as seen from the observation point.
+struct Edge {
+    double s0, s1;   // t.(v0 - a), t.(v1 - a)
+    double mv;       // m.(v0 - a) == m.(v1 - a)
+    double R0, R1;   // |v0 - a|, |v1 - a|
+    int ax;          // axis of the edge (0 x, 1 y, 2 z) -- for L0
+    double c_next;   // coordinate of v1 along ax (projection base point)
+    double pa, pb;   // fl(v1 - a) on the two axes other than ax, in xyz order
+    int max_;        // axis of m (for the vector integral)
+    double msign;    // sign of m along max_
+};
+
+struct Geo {
+    double a[3];               // observation point
+    double D[3][2];            // D[ax][lo/hi] = fl(bound - a[ax])
+    double R[2][2][2];         // R[ix][iy][iz]
+};
+
+GRA_HD inline void geo_init(Geo &g, const double *pt, const double *pr)
+{
+    g.a[0] = pt[0]; g.a[1] = pt[1]; g.a[2] = pt[2];
+    for (int ax = 0; ax < 3; ++ax) {
+        g.D[ax][0] = pr[2 * ax] - g.a[ax];
+        g.D[ax][1] = pr[2 * ax + 1] - g.a[ax];
+    }
+    double sx[2] = {g.D[0][0] * g.D[0][0], g.D[0][1] * g.D[0][1]};
+    double sy[2] = {g.D[1][0] * g.D[1][0], g.D[1][1] * g.D[1][1]};
+    double sz[2] = {g.D[2][0] * g.D[2][0], g.D[2][1] * g.D[2][1]};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k) g.R[i][j][k] = sqrt((sx[i] + sy[j]) + sz[k]);
+}
+
+// Face descriptor: 4 vertices as (ix,iy,iz) bound indices in the reference's
+// order, the face's normal axis and its outward sign.
+struct FaceDef {
+    int8_t v[4][3];
+    int8_t nax;
+    int8_t nsign;
+};
+
+// GravFormula.cpp vertex orders: p0=(0,0,0) p1=(1,0,0) p2=(1,1,0) p3=(0,1,0)
+// p4=(0,0,1) p5=(1,0,1) p6=(1,1,1) p7=(0,1,1).
+//   z faces {p3,p2,p1,p0} / {p4,p5,p6,p7}   (GravFormula.cpp:133-134)
+//   x faces {p0,p4,p7,p3} / {p1,p2,p6,p5}   (GravFormula.cpp:87-88)
+//   y faces {p0,p1,p5,p4} / {p2,p3,p7,p6}   (GravFormula.cpp:110-111)
+enum FaceId { kZ0 = 0, kZ1, kX0, kX1, kY0, kY1 };
+
+GRA_HD inline FaceDef face_def(int f)
+{
+    switch (f) {
+    case kZ0: return FaceDef{{{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0}}, 2, -1};
+    case kZ1: return FaceDef{{{0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}}, 2, +1};
+    case kX0: return FaceDef{{{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}}, 0, -1};
+    case kX1: return FaceDef{{{1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}}, 0, +1};
+    case kY0: return FaceDef{{{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}}, 1, -1};
+    default:  return FaceDef{{{1, 1, 0}, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}}, 1, +1};
+    }
+}
+
+// Builds edge j of face `fd` (v0 = vertex j, v1 = vertex j+1).
+GRA_HD inline Edge make_edge(const Geo &g, const FaceDef &fd, int j, const double *pr)
+{
+    const int8_t *u = fd.v[j];
+    const int8_t *w = fd.v[(j + 1) & 3];
+    int ax = (u[0] != w[0]) ? 0 : ((u[1] != w[1]) ? 1 : 2);
+    double dir = (w[ax] > u[ax]) ? 1.0 : -1.0;           // t = dir * e_ax
+    Edge e;
+    e.ax = ax;
+    e.s0 = dir * g.D[ax][u[ax]];
+    e.s1 = dir * g.D[ax][w[ax]];
+    e.R0 = g.R[u[0]][u[1]][u[2]];
+    e.R1 = g.R[w[0]][w[1]][w[2]];
+    // In-plane outward normal m = t x n lies on the remaining axis; the edge
+    // sits on the bound of that axis it shares with both vertices.
+    int max_ = 3 - ax - fd.nax;
+    int side = u[max_];
+    e.max_ = max_;
+    e.msign = side ? 1.0 : -1.0;
+    e.mv = e.msign * g.D[max_][side];
+    e.c_next = pr[2 * ax + w[ax]];
+    int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    e.pa = g.D[o1][w[o1]];
+    e.pb = g.D[o2][w[o2]];
+    return e;
+}
+
+// Perpendicular distance of Point::projection_line(v1, v0) from a, with the
+// rounding of prj = v1 + ((a - v1).tl) tl along the edge axis.
+GRA_HD inline double edge_L0(const Geo &g, const Edge &e)
+{
+    double d = g.a[e.ax] - e.c_next;
+    double prj = e.c_next + d;
+    double r = prj - g.a[e.ax];
+    double q[3];
+    int o1 = (e.ax == 0) ? 1 : 0, o2 = (e.ax == 2) ? 1 : 2;
+    q[e.ax] = r;
+    q[o1] = e.pa;
+    q[o2] = e.pb;
+    return sqrt((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]);
+}
+
+// h = (a - f0).n for the face (n = nsign * e_nax).
+GRA_HD inline double face_h(const Geo &g, const FaceDef &fd)
+{
+    int side = fd.v[0][fd.nax];
+    return -(fd.nsign * g.D[fd.nax][side]);
+}
+
+// log((s1 + R1)/(s0 + R0)) of an edge (shared by Surface_R_1 and Line_R_1).
+GRA_HD inline double edge_log(const Edge &e) { return log((e.s1 + e.R1) / (e.s0 + e.R0)); }
+
+// Integral::Surface_R_1 (Integral.cpp:593-633) including getBeta (:540-570).
+GRA_HD inline double surface_r_1(const Edge *E, const double *lg, double h)
+{
+    double ah = fabs(h);
+    double beta = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        double am = fabs(E[j].mv);
+        if (am < kTol) {
+            beta += 0.0;
+        } else {
+            double tt = (E[j].mv > 0) ? 1. : -1.;
+            beta = beta + tt * (atan(E[j].s1 / am) - atan(E[j].s0 / am));
+        }
+    }
+    double res = 0.0;
+    for (int j = 0; j < 4; ++j) {
+        double bj = 0.0;
+        if (!(fabs(E[j].mv) < kTol)) {
+            bj = ah * (atan((ah * E[j].s1) / (E[j].mv * E[j].R1)) -
+                       atan((ah * E[j].s0) / (E[j].mv * E[j].R0))) +
+                 E[j].mv * lg[j];
+        }
+        res += bj;
+    }
+    return res - beta * ah;
+}
+
+// Integral::Surface_R_3 (Integral.cpp:764-793), |h| >= TOL.
+GRA_HD inline double surface_r_3(const Geo &g, const Edge *E, double h)
+{
+    double ah = fabs(h);
+    double res = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        double L0 = edge_L0(g, E[i]);
+        double LL = L0 * L0;
+        double m0 = E[i].mv;
+        res += atan((m0 * E[i].s1) / (LL + ah * E[i].R1)) -
+               atan((m0 * E[i].s0) / (LL + ah * E[i].R0));
+    }
+    return res / ah;
+}
+
+// Integral::Line_R_1 (Integral.cpp:59-85) given the shared log; sets *bad for
+// the asserted case s0*s1 <= 0 (observation point on the edge itself).
+GRA_HD inline double line_r_1(const Edge &e, double lg, int *bad)
+{
+    if (fabs(e.s0 + e.R0) < kTol) {
+        if (!(e.s0 * e.s1 > 0)) *bad = 1;
+        return fabs(log(e.s1 / e.s0));
+    }
+    return lg;
+}
+
+// Integral::Surface_rR_3 (Integral.cpp:795-823): vector integral of r/R^3.
+// The edge terms land on the axis of each edge's normal m; the 1/R^3 term on
+// the face-normal axis.  A non-finite edge term poisons every component, as
+// the reference's 0 * inf products do.
+GRA_HD inline void surface_rr_3(const Geo &g, const Edge *E, const double *lg, double h,
+                                int nax, double nsign, double *out, int *bad)
+{
+    double r[3] = {0.0, 0.0, 0.0};
+    bool poison = false;
+    for (int i = 0; i < 4; ++i) {
+        double L = line_r_1(E[i], lg[i], bad);
+        if (!isfinite(L)) poison = true;
+        r[E[i].max_] = r[E[i].max_] - E[i].msign * L;
+    }
+    if (!(fabs(h) < kTol)) {
+        double sr3 = surface_r_3(g, E, h);
+        r[nax] = r[nax] - (nsign * h) * sr3;
+    }
+    if (poison) {
+        double nan = NAN;
+        r[0] = r[1] = r[2] = nan;
+    }
+    out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+}
+
+// Only the face-normal component of Surface_rR_3 (what Tzz/Txx/Tyy use from
+// a face normal to their own axis): the edge terms do not reach it.
+GRA_HD inline double surface_rr_3_normal(const Geo &g, const Edge *E, double h, double nsign,
+                                         int *bad)
+{
+    bool poison = false;
+    for (int i = 0; i < 4; ++i) {
+        if (fabs(E[i].s0 + E[i].R0) < kTol) {
+            if (!(E[i].s0 * E[i].s1 > 0)) *bad = 1;
+            if (!isfinite(log(E[i].s1 / E[i].s0))) poison = true;
+        }
+    }
+    double r = 0.0;
+    if (!(fabs(h) < kTol)) r = r - (nsign * h) * surface_r_3(g, E, h);
+    return poison ? (double)NAN : r;
+}
+
+// Evaluates the requested components (field bits 1..9 in `flag`) of one
+// (point, prism) pair with density rho into out[bit].  Returns 1 when the
+// reference would have asserted (degenerate geometry), else 0.
+GRA_HD inline int prism_fields(const double *pt, const double *pr, double rho, uint32_t flag,
+                               double *out)
+{
+    Geo g;
+    geo_init(g, pt, pr);
+    int bad = 0;
+    const double cg = -kSI2mGal * kG0 * rho;
+    const double ct = -kSI2Eotvos * kG0 * rho;
+
+    // z faces: gz (Surface_R_1), Tzz (normal component of Surface_rR_3)
+    if (flag & (kGz | kTzz)) {
+        double vR1[2] = {0, 0}, vT[2] = {0, 0};
+        for (int s = 0; s < 2; ++s) {
+            FaceDef fd = face_def(kZ0 + s);
+            Edge E[4];
+            for (int j = 0; j < 4; ++j) E[j] = make_edge(g, fd, j, pr);
+            double h = face_h(g, fd);
+            if (flag & kGz) {
+                double lg[4];
+                for (int j = 0; j < 4; ++j) lg[j] = fabs(E[j].mv) < kTol ? 0.0 : edge_log(E[j]);
+                vR1[s] = surface_r_1(E, lg, h);
+            }
+            if (flag & kTzz) vT[s] = surface_rr_3_normal(g, E, h, (double)fd.nsign, &bad);
+        }
+        if (flag & kGz) out[1] = cg * (-vR1[0] + vR1[1]);
+        if (flag & kTzz) out[4] = ct * (-vT[0] + vT[1]);
+    }
+    // x faces: gx, Txx (normal comp.), Txy, Txz (edge comps.)
+    if (flag & (kGx | kTxx | kTxy | kTxz)) {
+        double vR1[2] = {0, 0}, vT[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        for (int s = 0; s < 2; ++s) {
+            FaceDef fd = face_def(kX0 + s);
+            Edge E[4];
+            double lg[4];
+            for (int j = 0; j < 4; ++j) E[j] = make_edge(g, fd, j, pr);
+            for (int j = 0; j < 4; ++j) lg[j] = edge_log(E[j]);
+            double h = face_h(g, fd);
+            if (flag & kGx) vR1[s] = surface_r_1(E, lg, h);
+            if (flag & (kTxy | kTxz)) {
+                surface_rr_3(g, E, lg, h, fd.nax, (double)fd.nsign, vT[s], &bad);
+            } else if (flag & kTxx) {
+                vT[s][0] = surface_rr_3_normal(g, E, h, (double)fd.nsign, &bad);
+            }
+        }
+        if (flag & kGx) out[2] = cg * (-vR1[0] + vR1[1]);
+        if (flag & kTxz) out[5] = ct * (-vT[0][2] + vT[1][2]);
+        if (flag & kTxx) out[7] = ct * (-vT[0][0] + vT[1][0]);
+        if (flag & kTxy) out[8] = ct * (-vT[0][1] + vT[1][1]);
+    }
+    // y faces: gy, Tyy (normal comp.), Tyz (edge comp.)
+    if (flag & (kGy | kTyy | kTyz)) {
+        double vR1[2] = {0, 0}, vT[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        for (int s = 0; s < 2; ++s) {
+            FaceDef fd = face_def(kY0 + s);
+            Edge E[4];
+            double lg[4];
+            for (int j = 0; j < 4; ++j) E[j] = make_edge(g, fd, j, pr);
+            for (int j = 0; j < 4; ++j) lg[j] = edge_log(E[j]);
+            double h = face_h(g, fd);
+            if (flag & kGy) vR1[s] = surface_r_1(E, lg, h);
+            if (flag & kTyz) {
+                surface_rr_3(g, E, lg, h, fd.nax, (double)fd.nsign, vT[s], &bad);
+            } else if (flag & kTyy) {
+                vT[s][1] = surface_rr_3_normal(g, E, h, (double)fd.nsign, &bad);
+            }
+        }
+        if (flag & kGy) out[3] = cg * (-vR1[0] + vR1[1]);
+        if (flag & kTyz) out[6] = ct * (-vT[0][2] + vT[1][2]);
+        if (flag & kTyy) out[9] = ct * (-vT[0][1] + vT[1][1]);
+    }
+    return bad;
+}
+
+}  // namespace gra

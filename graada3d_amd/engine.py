@@ -1,0 +1,290 @@
+"""Host-side mirror of the reference's forward / inversion interfaces.
+
+``Fwd`` mirrors ``class Fwd`` (src/Forward_modelling/Fwd.h:14-105):
+``compute_G``, ``G_vec_mul``, ``GT_vec_mul``, ``get_G``, ``compute_gobs``.
+``GaussNewtonInversion`` adds ``solve_cg`` with the argument list of
+``GaussNewtonInversion::solve_cg`` (src/Inversion/GaussNewtonInversion.h:70-93)
+and the barrier / misfit / IRLS helpers of ``invert_with_own_CG``.
+``forward`` is the matrix-free loop of the ``compute`` tool
+(src/compute.cpp:246-260).
+
+Every numerical call goes through libgraeng's C ABI onto the GPU; there is no
+CPU fallback.  Errors raise ``GraError`` (the reference asserts or aborts).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import GraError, GraCsr, GraCgArgs, GraGnSetup, c_dp, c_i32p, c_i64p
+
+# Field bits (gs.h:46-57)
+GZ, GX, GY, TZZ, TXZ, TYZ, TXX, TXY, TYY = (1 << 1, 1 << 2, 1 << 3, 1 << 4, 1 << 5, 1 << 6,
+                                            1 << 7, 1 << 8, 1 << 9)
+FIELD_NAMES = {1: "gz", 2: "gx", 3: "gy", 4: "Tzz", 5: "Txz", 6: "Tyz", 7: "Txx", 8: "Txy",
+               9: "Tyy"}
+NAME_TO_BIT = {v: k for k, v in FIELD_NAMES.items()}
+
+
+def flag_from_names(names) -> int:
+    """'gz/Tzz' or ['gz','Tzz'] -> field flag (aliases as compute.cpp:168-219)."""
+    if isinstance(names, str):
+        names = names.split("/")
+    alias = {"gxx": "Txx", "gxy": "Txy", "gyx": "Txy", "Tyx": "Txy", "gxz": "Txz", "gzx": "Txz",
+             "Tzx": "Txz", "gyy": "Tyy", "gyz": "Tyz", "gzy": "Tyz", "Tzy": "Tyz", "gzz": "Tzz"}
+    f = 0
+    for n in names:
+        n = alias.get(n, n)
+        if n not in NAME_TO_BIT:
+            raise ValueError(f"invalid component name {n!r}")
+        f |= 1 << NAME_TO_BIT[n]
+    return f
+
+
+def component_bits(flag: int):
+    return [b for b in range(1, 10) if flag & (1 << b)]
+
+
+def _f64(a, n=None, name="array"):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"{name}: expected {n} values, got {a.size}")
+    return a
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_dp)
+
+
+class Context:
+    """One engine context (device buffers + its own GPU stream)."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        st = self.L.gra_ctx_create(int(device), ctypes.byref(h))
+        if st != 0:
+            raise GraError(st, "gra_ctx_create", "(is a GPU visible?)")
+        self.h = h
+
+    def check(self, st: int, where: str):
+        if st != 0:
+            msg = self.L.gra_last_error(self.h).decode(errors="replace")
+            raise GraError(st, where, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gra_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(self.L.gra_sync(self.h), "gra_sync")
+
+    # timing (GPU events on the context stream)
+    def timing(self, on: bool):
+        self.check(self.L.gra_timing_enable(self.h, 1 if on else 0), "timing")
+
+    def timing_reset(self):
+        self.check(self.L.gra_timing_reset(self.h), "timing_reset")
+
+    def timing_get(self, which: int):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self.check(self.L.gra_timing_get(self.h, which, ctypes.byref(ms), ctypes.byref(n)),
+                   "timing_get")
+        return ms.value, n.value
+
+
+TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER = 0, 1, 2, 3
+
+
+def csr_struct(m, keep: list) -> GraCsr:
+    """scipy sparse (square) -> GraCsr (arrays appended to `keep`)."""
+    m = sp.csr_matrix(m)
+    m.sort_indices()
+    rp = np.ascontiguousarray(m.indptr, dtype=np.int64)
+    ci = np.ascontiguousarray(m.indices, dtype=np.int32)
+    v = np.ascontiguousarray(m.data, dtype=np.float64)
+    keep += [rp, ci, v]
+    return GraCsr(m.shape[0], v.size, rp.ctypes.data_as(c_i64p), ci.ctypes.data_as(c_i32p),
+                  v.ctypes.data_as(c_dp))
+
+
+class Fwd:
+    """Dense forward operator G on the GPU (mirror of the reference's Fwd)."""
+
+    def __init__(self, prisms, points, field_flag: int = GZ, ctx: Context | None = None):
+        self.ctx = ctx or Context()
+        self.prisms = _f64(prisms).reshape(-1, 6)
+        self.points = _f64(points).reshape(-1, 3)
+        self.field_flag = int(field_flag)
+        self.Nm = self.prisms.shape[0]
+        self.N_obs = self.points.shape[0]
+        self.Nd = len(component_bits(self.field_flag)) * self.N_obs
+        self._assembled = False
+
+    def compute_G(self, store: str = "f64"):
+        """Fwd::compute_G (Fwd.cpp:47-79); G stays in device memory."""
+        c = self.ctx
+        st = c.L.gra_assemble(c.h, _p(self.prisms), self.Nm, _p(self.points), self.N_obs,
+                              self.field_flag, _lib.DTYPES[store])
+        c.check(st, "gra_assemble")
+        self._assembled = True
+
+    def _need_G(self):
+        if not self._assembled:
+            raise GraError(1, "Fwd", "compute_G() has not been called")
+
+    def G_vec_mul(self, vec):
+        self._need_G()
+        v = _f64(vec, self.Nm, "vec")
+        out = np.empty(self.Nd)
+        self.ctx.check(self.ctx.L.gra_G_vec_mul(self.ctx.h, _p(v), _p(out)), "G_vec_mul")
+        return out
+
+    def GT_vec_mul(self, vec):
+        self._need_G()
+        v = _f64(vec, self.Nd, "vec")
+        out = np.empty(self.Nm)
+        self.ctx.check(self.ctx.L.gra_GT_vec_mul(self.ctx.h, _p(v), _p(out)), "GT_vec_mul")
+        return out
+
+    def compute_gobs(self, rho):
+        return self.G_vec_mul(rho)
+
+    def get_G(self, col_major: bool = False):
+        """Host copy of G as float64 (Nd x Nm)."""
+        self._need_G()
+        out = np.empty((self.Nd, self.Nm)) if not col_major else np.empty((self.Nm, self.Nd))
+        self.ctx.check(self.ctx.L.gra_copy_G(self.ctx.h, _p(out), 1 if col_major else 0),
+                       "copy_G")
+        return out if not col_major else out.T
+
+
+class GaussNewtonInversion(Fwd):
+    """solve_cg and the Gauss-Newton helpers on the GPU-resident G."""
+
+    def __init__(self, prisms, points, field_flag: int = GZ, ctx: Context | None = None):
+        super().__init__(prisms, points, field_flag, ctx)
+        self.use_cross_gradient_constraint = False
+        self.a_crg = 1.0
+        self.S_crg = None
+        self._ops_key = None
+
+    def set_regularization(self, Ws, Wx, Wy, Wz, Tx=None, Ty=None, Tz=None):
+        keep = []
+        W = [csr_struct(m, keep) for m in (Ws, Wx, Wy, Wz)]
+        if Tx is not None:
+            T = [ctypes.byref(csr_struct(m, keep)) for m in (Tx, Ty, Tz)]
+        else:
+            T = [None, None, None]
+        st = self.ctx.L.gra_set_regularization(self.ctx.h, *[ctypes.byref(w) for w in W], *T)
+        self.ctx.check(st, "set_regularization")
+
+    def solve_cg(self, tol, maxit, lam, mk, mref, Wd, dobs, P, WL_s=None, WL_x=None, WL_y=None,
+                 WL_z=None, WL_tx=None, WL_ty=None, WL_tz=None, quirks: bool = True):
+        """GaussNewtonInversion::solve_cg; diagonal operators as vectors (None =
+        identity).  Regularisation operators come from set_regularization.
+        Returns (dx, err, iterations)."""
+        self._need_G()
+        nm, nd = self.Nm, self.Nd
+        arr = dict(mk=_f64(mk, nm, "mk"), mref=_f64(mref, nm, "mref"), Wd=_f64(Wd, nd, "Wd"),
+                   dobs=_f64(dobs, nd, "dobs"), P=_f64(P, nm, "P"))
+        wl = {k: (None if v is None else _f64(v, nm, k)) for k, v in
+              dict(WL_s=WL_s, WL_x=WL_x, WL_y=WL_y, WL_z=WL_z, WL_tx=WL_tx, WL_ty=WL_ty,
+                   WL_tz=WL_tz).items()}
+        S = None if self.S_crg is None else _f64(self.S_crg, nm, "S_crg")
+        a = GraCgArgs()
+        a.tol, a.maxit, a.lambda_ = float(tol), int(maxit), float(lam)
+        for k, v in arr.items():
+            setattr(a, k, _p(v))
+        for k, v in wl.items():
+            setattr(a, k, _p(v))
+        a.use_cross_gradient = 1 if self.use_cross_gradient_constraint else 0
+        a.a_crg = float(self.a_crg)
+        a.S_crg = _p(S)
+        a.quirk_mode = 1 if quirks else 0
+        dx = np.empty(nm)
+        err = ctypes.c_double(np.nan)
+        its = ctypes.c_int(0)
+        st = self.ctx.L.gra_solve_cg(self.ctx.h, ctypes.byref(a), _p(dx), ctypes.byref(err),
+                                     ctypes.byref(its))
+        self.ctx.check(st, "solve_cg")
+        return dx, err.value, its.value
+
+    def barrier_P(self, m, m_min, m_max):
+        nm = self.Nm
+        out = np.empty(nm)
+        st = self.ctx.L.gra_barrier_P(self.ctx.h, _p(_f64(m, nm)), _p(_f64(m_min, nm)),
+                                      _p(_f64(m_max, nm)), _p(out))
+        self.ctx.check(st, "barrier_P")
+        return out
+
+    def barrier_update(self, m, dx, m_min, m_max, m_trial):
+        nm = self.Nm
+        mt = _f64(m_trial, nm).copy()
+        st = self.ctx.L.gra_barrier_update(self.ctx.h, _p(_f64(m, nm)), _p(_f64(dx, nm)),
+                                           _p(_f64(m_min, nm)), _p(_f64(m_max, nm)), _p(mt))
+        self.ctx.check(st, "barrier_update")
+        return mt
+
+    def misfit(self, m, Wd, dobs):
+        out = ctypes.c_double()
+        st = self.ctx.L.gra_misfit(self.ctx.h, _p(_f64(m, self.Nm)), _p(_f64(Wd, self.Nd)),
+                                   _p(_f64(dobs, self.Nd)), ctypes.byref(out))
+        self.ctx.check(st, "misfit")
+        return out.value
+
+    def irls_weights(self, W, m, p: int, eps2: float):
+        keep = []
+        c = csr_struct(W, keep)
+        out = np.empty(W.shape[0])
+        st = self.ctx.L.gra_irls_weights(self.ctx.h, ctypes.byref(c), _p(_f64(m, W.shape[0])),
+                                         int(p), float(eps2), _p(out))
+        self.ctx.check(st, "irls_weights")
+        return out
+
+    # resident Gauss-Newton loop (model and CG state stay on the GPU)
+    def gn_setup(self, dobs, Wd, m_min, m_max, mref, m_ini):
+        nm, nd = self.Nm, self.Nd
+        self._gn_keep = [_f64(dobs, nd), _f64(Wd, nd), _f64(m_min, nm), _f64(m_max, nm),
+                         _f64(mref, nm), _f64(m_ini, nm)]
+        s = GraGnSetup(*[_p(a) for a in self._gn_keep])
+        self.ctx.check(self.ctx.L.gra_gn_setup_problem(self.ctx.h, ctypes.byref(s)), "gn_setup")
+
+    def gn_step(self, lam, cg_tol, cg_maxit):
+        mis = ctypes.c_double()
+        err = ctypes.c_double()
+        its = ctypes.c_int()
+        st = self.ctx.L.gra_gn_step(self.ctx.h, float(lam), float(cg_tol), int(cg_maxit),
+                                    ctypes.byref(mis), ctypes.byref(err), ctypes.byref(its))
+        self.ctx.check(st, "gn_step")
+        return mis.value, err.value, its.value
+
+    def gn_model(self):
+        m = np.empty(self.Nm)
+        self.ctx.check(self.ctx.L.gra_gn_get_model(self.ctx.h, _p(m)), "gn_model")
+        return m
+
+
+def forward(prisms, rho, points, field_flag: int, ctx: Context | None = None):
+    """compute.cpp:246-260 matrix-free forward: (N_obs, n_comp) array."""
+    ctx = ctx or Context()
+    prisms = _f64(prisms).reshape(-1, 6)
+    points = _f64(points).reshape(-1, 3)
+    rho = _f64(rho, prisms.shape[0], "rho")
+    nc = len(component_bits(field_flag))
+    out = np.empty((points.shape[0], nc))
+    st = ctx.L.gra_forward(ctx.h, _p(prisms), _p(rho), prisms.shape[0], _p(points),
+                           points.shape[0], int(field_flag), _p(out))
+    ctx.check(st, "forward")
+    return out

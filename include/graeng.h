@@ -1,0 +1,211 @@
+/*
+ * graeng — MI355X-native engine for GraAda3D's prism-kernel / CG hot path.
+ *
+ * Plain C ABI: plain pointers and sizes, no C++ or GPU-runtime types.  Every
+ * entry point returns a gra_status; no exception crosses the boundary.  All
+ * host pointers are borrowed for the duration of the call only.  Calls on one
+ * context are serialised by the context's own lock; separate contexts may be
+ * used from separate threads concurrently.
+ *
+ * Units and conventions follow the reference (src/Forward_modelling/gs.h):
+ * lengths in m, z positive DOWN, density in kg/m^3, G0 = 6.673e-11,
+ * g-components in mGal (1e5 SI), gradient components in Eotvos (1e9 SI).
+ *
+ * Prisms are float64 (Nm x 6) rows {x0, x1, y0, y1, z0, z1} with x0<x1,
+ * y0<y1, z0<z1 (RectPrism, src/Forward_modelling/RectPrism.h:41-43); points
+ * are float64 (N_obs x 3) rows {x, y, z}.
+ */
+#ifndef GRAENG_H
+#define GRAENG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gra_status {
+    GRA_OK = 0,
+    GRA_BAD_ARG = 1,               /* invalid argument / state                     */
+    GRA_OOM = 2,                   /* device or host allocation failed            */
+    GRA_HIP_ERROR = 3,             /* GPU runtime error (message: gra_last_error)  */
+    GRA_NUMERIC_BREAKDOWN = 4,     /* CG step with t2 == 0 or a non-finite scalar  */
+    GRA_DEGENERATE_GEOMETRY = 5,   /* point on an edge extension (Integral.cpp:77) */
+    GRA_NO_DEVICE = 6              /* no GPU visible                               */
+} gra_status;
+
+typedef enum gra_dtype { GRA_F64 = 0, GRA_F32 = 1, GRA_BF16 = 2 } gra_dtype;
+
+/* Field bits, identical to GS::ComputationOptions (gs.h:46-57).  Bit 0 (the
+ * potential V) is not part of this engine and is rejected. */
+#define GRA_GZ  0x0002u
+#define GRA_GX  0x0004u
+#define GRA_GY  0x0008u
+#define GRA_TZZ 0x0010u
+#define GRA_TXZ 0x0020u
+#define GRA_TYZ 0x0040u
+#define GRA_TXX 0x0080u
+#define GRA_TXY 0x0100u
+#define GRA_TYY 0x0200u
+
+typedef struct gra_ctx gra_ctx;
+
+/* Square CSR matrix (n x n): row_ptr[n+1] (int64), col_idx[nnz] (int32),
+ * val[nnz].  The reference's SMatrix is Eigen::SparseMatrix<double,RowMajor>
+ * (gs.h:29), i.e. the same storage. */
+typedef struct gra_csr {
+    int64_t n;
+    int64_t nnz;
+    const int64_t *row_ptr;
+    const int32_t *col_idx;
+    const double *val;
+} gra_csr;
+
+/* ---------------------------------------------------------------- context */
+const char *gra_status_string(int status);
+/* Message of the last failing call on this context ("" if none). */
+const char *gra_last_error(const gra_ctx *ctx);
+/* Creates a context on GPU `device` with its own (default-priority) stream. */
+gra_status gra_ctx_create(int device, gra_ctx **out);
+void gra_ctx_destroy(gra_ctx *ctx);
+/* Waits for all work queued on the context's stream. */
+gra_status gra_sync(gra_ctx *ctx);
+
+/* ---------------------------------------------------------- forward model */
+/*
+ * Replaces Fwd::compute_G (src/Forward_modelling/Fwd.cpp:47-79).
+ * Assembles the dense sensitivity matrix G on the device (it stays there):
+ * row k*N_obs + i, column j holds component k (ascending field bit) of the
+ * unit-density field of prism j at point i, evaluated in FP64 with the
+ * reference's face/edge formulation (GravFormula.cpp:75-272,
+ * Integral.cpp:59-85, 540-633, 764-823) and stored in `store` precision.
+ * Any previous G of the context is released first.
+ */
+gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const double *pts,
+                        int64_t N_obs, uint32_t field_flag, gra_dtype store);
+
+/* Shape of the context's G: Nd = n_comp * N_obs rows, Nm columns. */
+gra_status gra_get_dims(const gra_ctx *ctx, int64_t *Nd, int64_t *Nm, int64_t *N_obs,
+                        uint32_t *field_flag, int *store);
+
+/*
+ * Replaces Fwd::get_G (Fwd.h:46): copies G to host as float64, column-major
+ * (Eigen::MatrixXd layout, leading dimension Nd) when col_major != 0, else
+ * row-major.  `out` holds Nd*Nm doubles.
+ */
+gra_status gra_copy_G(gra_ctx *ctx, double *out, int col_major);
+
+/* Replaces Fwd::G_vec_mul (Fwd.cpp:114-147): product[Nd] = G * vec[Nm]. */
+gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product);
+/* Replaces Fwd::GT_vec_mul (Fwd.cpp:80-113): product[Nm] = G^T * vec[Nd]. */
+gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product);
+
+/*
+ * Replaces the matrix-free forward loop of compute.cpp:246-260:
+ * out[i*n_comp + k] = sum_j field_k(point i, prism j, rho_j), no G stored.
+ */
+gra_status gra_forward(gra_ctx *ctx, const double *prisms, const double *rho, int64_t Nm,
+                       const double *pts, int64_t N_obs, uint32_t field_flag, double *out);
+
+/* ------------------------------------------------------------- inversion */
+/*
+ * Uploads the regularisation operators used by gra_solve_cg: Ws, Wx, Wy, Wz
+ * (W* = a*S*V*D*Z, GaussNewtonInversion.cpp:711-718) and, for the
+ * cross-gradient term, Tx0, Ty0, Tz0 (InversionBase::set_Tmatrix,
+ * InversionBase.cpp:468-495); pass NULL for the T's when the cross-gradient
+ * constraint is off.  All are Nm x Nm with Nm equal to G's column count.
+ */
+gra_status gra_set_regularization(gra_ctx *ctx, const gra_csr *Ws, const gra_csr *Wx,
+                                  const gra_csr *Wy, const gra_csr *Wz, const gra_csr *Tx0,
+                                  const gra_csr *Ty0, const gra_csr *Tz0);
+
+/* Diagonal operators are passed as vectors; NULL WL_* / S_crg = identity. */
+typedef struct gra_cg_args {
+    double tol;
+    int maxit;
+    double lambda;
+    const double *mk;      /* [Nm] current model                 */
+    const double *mref;    /* [Nm] reference model (m0)          */
+    const double *Wd;      /* [Nd] data weights (diagonal)       */
+    const double *dobs;    /* [Nd] observed data                 */
+    const double *P;       /* [Nm] barrier Jacobian (diagonal)   */
+    const double *WL_s, *WL_x, *WL_y, *WL_z, *WL_tx, *WL_ty, *WL_tz;  /* [Nm] */
+    int use_cross_gradient;
+    double a_crg;
+    const double *S_crg;   /* [Nm] */
+    /* 1: reproduce the reference operator exactly, including its asymmetric
+     *    weighting (t_s omits WL_s, GN.cpp:1573; cross-gradient rows of t use
+     *    WL_z/WL_x/WL_y, :1582-1584; initial residual's Ty term uses WL_y,
+     *    :1508).  0: consistent operator (A^T of the same A).            */
+    int quirk_mode;
+} gra_cg_args;
+
+/*
+ * Replaces GaussNewtonInversion::solve_cg
+ * (src/Inversion/GaussNewtonInversion.h:70-93, .cpp:1417-1698): CGLS on
+ * [Wd G P; sqrt(l) WL_s Ws P; sqrt(l) WL_z Wz P; sqrt(l) WL_x Wx P;
+ *  sqrt(l) WL_y Wy P; (WL_t* a S T* P)] dx ~= b, in FP64 on the device with
+ * the reference's two passes over G per iteration.  Writes dx[Nm], *err
+ * (= ||r||^2/||A^T b||^2, untouched when maxit == 0) and *iterations.
+ */
+gra_status gra_solve_cg(gra_ctx *ctx, const gra_cg_args *args, double *dx, double *err,
+                        int *iterations);
+
+/* P_ii = (M-m)(m-m_min)/(M-m_min)  (GaussNewtonInversion.cpp:856-861). */
+gra_status gra_barrier_P(gra_ctx *ctx, const double *m, const double *m_min,
+                         const double *m_max, double *P);
+
+/* Log-barrier model update with the reference's overflow fallbacks
+ * (GaussNewtonInversion.cpp:877-911); m_trial is read and written. */
+gra_status gra_barrier_update(gra_ctx *ctx, const double *m, const double *dx,
+                              const double *m_min, const double *m_max, double *m_trial);
+
+/* misfit = sqrt(||Wd (G m - dobs)||^2 / Nd)  (GaussNewtonInversion.cpp:913-916). */
+gra_status gra_misfit(gra_ctx *ctx, const double *m, const double *Wd, const double *dobs,
+                      double *misfit);
+
+/* IRLS weights WL_ii = ((W m)_i^2 + eps2)^(-(2-p)/4) for a CSR W
+ * (GaussNewtonInversion.cpp:962-1022); p is the integer Lp exponent. */
+gra_status gra_irls_weights(gra_ctx *ctx, const gra_csr *W, const double *m, int p,
+                            double eps2, double *WL);
+
+/* ----------------------------------------- resident Gauss-Newton driver */
+/*
+ * A Gauss-Newton loop whose model, weights and CG state stay in device
+ * memory (the per-outer-iteration body of invert_with_own_CG,
+ * GaussNewtonInversion.cpp:815-1048, with the lambda schedule chosen by the
+ * caller and every trial accepted).  gra_gn_setup uploads the problem (the
+ * regularisation operators come from gra_set_regularization);
+ * gra_gn_step runs: P from m; dx = solve_cg(...); m_trial = barrier update;
+ * misfit(m_trial); m = m_trial.  WL_* stay identity (L2).
+ */
+typedef struct gra_gn_setup {
+    const double *dobs;    /* [Nd] */
+    const double *Wd;      /* [Nd] */
+    const double *m_min;   /* [Nm] */
+    const double *m_max;   /* [Nm] */
+    const double *mref;    /* [Nm] */
+    const double *m_ini;   /* [Nm] */
+} gra_gn_setup;
+
+gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *setup);
+gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
+                       double *misfit, double *cg_err, int *cg_iterations);
+gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
+
+/* ---------------------------------------------------------- measurement */
+/*
+ * Average device time (ms, from GPU events on the context's stream) of the
+ * launches of kernel class `which` since the last gra_timing_reset:
+ * 0 = GEMV G*x, 1 = GEMV G^T*x, 2 = G assembly, 3 = one full CG iteration.
+ * Timing events are recorded only while enabled.
+ */
+gra_status gra_timing_enable(gra_ctx *ctx, int on);
+gra_status gra_timing_reset(gra_ctx *ctx);
+gra_status gra_timing_get(gra_ctx *ctx, int which, double *avg_ms, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAENG_H */

@@ -1,0 +1,16 @@
+// Test support only: compiles the engine's per-(point, prism) math
+// (graada3d_amd/csrc/prism_math.hpp) for the host with g++, so tests can
+// compare it bit for bit with the CPU oracle under the same libm.  On the GPU
+// the same header is compiled for gfx950; only atan/log come from a
+// different library there.
+#define GRA_HD
+#include <cstdint>
+
+#include "prism_math.hpp"
+
+extern "C" int hc_prism_field(const double *pt, const double *pr, double rho, uint32_t flag,
+                              double *out10)
+{
+    for (int k = 0; k < 10; ++k) out10[k] = 0.0;
+    return gra::prism_fields(pt, pr, rho, flag, out10);
+}

@@ -1,0 +1,185 @@
+"""The CGLS oracle (restating GaussNewtonInversion::solve_cg, .cpp:1417-1698)
+and the Gauss-Newton helpers, pinned by:
+
+* convergence to the least-squares solution of the stacked system
+  [Wd G P; sqrt(l) WL_s Ws P; ...] (numpy lstsq) when the operator is
+  consistent (quirks off), with and without the cross-gradient rows;
+* an independent dense-matrix restatement of the reference's iteration with
+  its operator asymmetries on (quirks on), written from the reference's
+  expressions as whole-matrix products;
+* quirks on == quirks off when the asymmetric weights are the identity;
+* closed forms for the barrier update (m=0, bounds +-1 gives tanh(dx/2)),
+  including the reference's sequential overflow fallback;
+* numpy for misfit and IRLS weights.
+The reference's solve_cg cannot be compiled here (Fwd.h includes GSL headers
+the image lacks), so no reference-run vector pins the unconverged iterates
+with quirks on: that part is pinned by the dense restatement only.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+
+
+def problem(seed, nd=40, nm=24, crg=False, wl=False):
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((nd, nm))
+    mk = rng.standard_normal(nm)
+    mref = 0.1 * rng.standard_normal(nm)
+    Wd = rng.uniform(0.5, 2.0, nd)
+    dobs = rng.standard_normal(nd)
+    P = rng.uniform(0.2, 1.5, nm)
+
+    def rs(density=0.15):
+        m = sp.random(nm, nm, density=density, random_state=rng.integers(1 << 30), format="csr")
+        return (m + sp.identity(nm)).tocsr()
+
+    Ws, Wx, Wy, Wz = rs(), rs(), rs(), rs()
+    WL = {}
+    if wl:
+        for k in ("s", "x", "y", "z", "tx", "ty", "tz"):
+            WL[k] = rng.uniform(0.5, 2.0, nm)
+    c = None
+    if crg:
+        c = dict(a=0.7, S=rng.uniform(0.5, 1.5, nm), Tx=rs(0.2), Ty=rs(0.2), Tz=rs(0.2))
+    return dict(G=G, mk=mk, mref=mref, Wd=Wd, dobs=dobs, P=P, Ws=Ws, Wx=Wx, Wy=Wy, Wz=Wz,
+                WL=WL, crg=c)
+
+
+def stacked(p, lam, quirks=False):
+    """Dense A (t operator, P applied) and b of the stacked least-squares system."""
+    nm = p["mk"].size
+    I = np.ones(nm)
+    WL = {k: p["WL"].get(k, I) for k in ("s", "x", "y", "z", "tx", "ty", "tz")}
+    sl = np.sqrt(lam)
+    D = np.diag
+    P = D(p["P"])
+    blocks = [D(p["Wd"]) @ p["G"],
+              sl * (D(I if quirks else WL["s"]) @ p["Ws"].toarray()),
+              sl * D(WL["z"]) @ p["Wz"].toarray(), sl * D(WL["x"]) @ p["Wx"].toarray(),
+              sl * D(WL["y"]) @ p["Wy"].toarray()]
+    b = [p["Wd"] * (p["dobs"] - p["G"] @ p["mk"]),
+         -sl * WL["s"] * (p["Ws"] @ (p["mk"] - p["mref"])),
+         -sl * WL["z"] * (p["Wz"] @ p["mk"]), -sl * WL["x"] * (p["Wx"] @ p["mk"]),
+         -sl * WL["y"] * (p["Wy"] @ p["mk"])]
+    rblocks = [D(p["Wd"]) @ p["G"], sl * D(WL["s"]) @ p["Ws"].toarray(), blocks[2], blocks[3],
+               blocks[4]]
+    if p["crg"] is not None:
+        c = p["crg"]
+        T = {k: D(c["a"] * c["S"]) @ c[k].toarray() for k in ("Tx", "Ty", "Tz")}
+        tw = (WL["z"], WL["x"], WL["y"]) if quirks else (WL["tz"], WL["tx"], WL["ty"])
+        blocks += [D(tw[0]) @ T["Tz"], D(tw[1]) @ T["Tx"], D(tw[2]) @ T["Ty"]]
+        b += [-WL["tz"] * (T["Tz"] @ p["mk"]), -WL["tx"] * (T["Tx"] @ p["mk"]),
+              -WL["ty"] * (T["Ty"] @ p["mk"])]
+        rblocks += [D(WL["tz"]) @ T["Tz"], D(WL["tx"]) @ T["Tx"], D(WL["ty"]) @ T["Ty"]]
+        rblocks_init = rblocks[:-1] + [D(WL["y"] if quirks else WL["ty"]) @ T["Ty"]]
+    else:
+        rblocks_init = rblocks
+    return [B @ P for B in blocks], np.concatenate(b), [B @ P for B in rblocks], \
+        [B @ P for B in rblocks_init]
+
+
+def dense_cgls(p, lam, maxit, tol, quirks):
+    """The reference's iteration, restated with dense matrices."""
+    A_t, b, A_r, A_r0 = stacked(p, lam, quirks)
+    At = np.vstack(A_t)
+    Ar = np.vstack(A_r)
+    Ar0 = np.vstack(A_r0)
+    s = -b
+    r = Ar0.T @ s
+    ATb = Ar.T @ b
+    ATb2 = ATb @ ATb
+    p_ = np.zeros_like(r)
+    dx = np.zeros_like(r)
+    beta = 0.0
+    err, its = np.nan, maxit
+    for i in range(maxit):
+        p_ = -r + beta * p_
+        # t = A (P p): the P is folded into the blocks
+        t = At @ p_
+        alpha = (r @ r) / (t @ t)
+        s = s + alpha * t
+        dx = dx + alpha * p_
+        rl = r @ r
+        r = Ar.T @ s
+        beta = (r @ r) / rl
+        err = (r @ r) / ATb2
+        if err < tol:
+            its = i + 1
+            break
+    return dx, err, its
+
+
+def run_oracle(p, lam, maxit, tol, quirks):
+    return oracle.solve_cg(p["G"], tol, maxit, lam, p["mk"], p["mref"], p["Wd"], p["dobs"],
+                           p["P"], p["Ws"], p["Wx"], p["Wy"], p["Wz"], WL=p["WL"], crg=p["crg"],
+                           quirks=quirks)
+
+
+@pytest.mark.parametrize("crg", [False, True])
+def test_converges_to_least_squares(crg):
+    p = problem(1, crg=crg, wl=True)
+    lam = 0.3
+    dx, err, its = run_oracle(p, lam, 400, 1e-28, quirks=False)
+    A_t, b, _, _ = stacked(p, lam, quirks=False)
+    sol = np.linalg.lstsq(np.vstack(A_t), b, rcond=None)[0]
+    assert np.linalg.norm(dx - sol) < 1e-8 * np.linalg.norm(sol)
+
+
+@pytest.mark.parametrize("crg,wl", [(False, False), (False, True), (True, True)])
+def test_matches_dense_restatement_with_quirks(crg, wl):
+    p = problem(2, crg=crg, wl=wl)
+    lam = 2.5
+    dx, err, its = run_oracle(p, lam, 15, 1e-30, quirks=True)
+    dx2, err2, its2 = dense_cgls(p, lam, 15, 1e-30, quirks=True)
+    assert its == its2 == 15
+    assert np.linalg.norm(dx - dx2) < 1e-10 * np.linalg.norm(dx2)
+    assert abs(err - err2) < 1e-8 * abs(err2)
+
+
+def test_quirks_inert_when_weights_are_identity():
+    p = problem(3, crg=True, wl=False)
+    a = run_oracle(p, 1.0, 12, 1e-30, quirks=True)
+    b = run_oracle(p, 1.0, 12, 1e-30, quirks=False)
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+
+
+def test_stops_at_tolerance_and_counts_iterations():
+    p = problem(4)
+    dx, err, its = run_oracle(p, 1.0, 200, 1e-6, quirks=True)
+    assert its < 200 and err < 1e-6
+    dx0, err0, its0 = run_oracle(p, 1.0, 0, 1e-6, quirks=True)
+    assert its0 == 0 and np.isnan(err0) and np.all(dx0 == 0)
+
+
+def test_barrier_update_closed_form_and_fallback():
+    m = np.zeros(3)
+    lo, hi = -np.ones(3), np.ones(3)
+    dx = np.array([0.1, 1000.0, -0.2])          # cell 1 overflows exp
+    mt = oracle.barrier_update(m, dx, lo, hi, np.array([0.5, 0.25, 0.125]))
+    t0 = np.tanh(0.05)
+    assert abs(mt[0] - t0) < 1e-16
+    assert abs(mt[1] - 0.5 * ((1.0 - 1e-6 + 0.25) / 2.0 + t0)) < 1e-16
+    assert abs(mt[2] - np.tanh(-0.1)) < 1e-16
+    # first cell overflowing averages with the *previous* trial value of cell 1
+    mt = oracle.barrier_update(m, np.array([1000.0, 0.1, 0.0]), lo, hi, np.array([0.5, 0.3, 0.0]))
+    assert abs(mt[0] - 0.5 * ((1.0 - 1e-6 + 0.5) / 2.0 + 0.3)) < 1e-16
+
+
+def test_barrier_P_misfit_irls_against_numpy():
+    rng = np.random.default_rng(8)
+    m = rng.uniform(-1, 1, 50)
+    lo, hi = np.full(50, -2.0), np.full(50, 3.0)
+    P = oracle.barrier_P(m, lo, hi)
+    assert np.allclose(P, (hi - m) * (m - lo) / (hi - lo), rtol=1e-15)
+    G = rng.standard_normal((30, 50))
+    Wd = rng.uniform(1, 2, 30)
+    d = rng.standard_normal(30)
+    ref = np.sqrt(np.sum((Wd * (G @ m - d)) ** 2) / 30)
+    assert abs(oracle.misfit(G, m, Wd, d) - ref) < 1e-13 * ref
+    W = sp.random(50, 50, density=0.1, random_state=1, format="csr") + sp.identity(50)
+    for pexp in (0, 1, 2):
+        WL = oracle.irls_weights(W.tocsr(), m, pexp, 1e-3)
+        x = W @ m
+        assert np.allclose(WL, (x * x + 1e-3) ** (-(2 - pexp) / 4.0), rtol=1e-14)

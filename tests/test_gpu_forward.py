@@ -1,0 +1,136 @@
+"""GPU prism kernels (through the C ABI) against the oracle and the
+reference's golden vectors.
+
+Bar: the GPU evaluates the reference's FP64 face/edge formulation with the
+same operation order (bit-exact on the host, test_host_math.py); only the
+device atan/log differ from glibc's by an ulp, amplified by the face
+differences' cancellation.  Tolerances: per component block
+||x - x_ref||/||x_ref|| <= 1e-12 and max|x - x_ref|/max|x_ref| <= 1e-12 against
+the oracle; against the 16-digit golden files 1e-12 as well.  The required
+bound for the plain FP64 path is 1e-10.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from graada3d_amd import GraError, Fwd, forward
+from graada3d_amd import mesh as M
+from helpers import load, max_rel_to_scale, model_from_fixture, rel_norm
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _blocks(G, nc, n_obs):
+    return [G[k * n_obs:(k + 1) * n_obs] for k in range(nc)]
+
+
+def test_assemble_small_G_all_components_vs_reference(ctx):
+    fx = load("ref_G_small.npz")
+    f = Fwd(fx["prisms"], fx["points"], int(fx["flag"]), ctx=ctx)
+    f.compute_G("f64")
+    G = f.get_G()
+    n_obs = fx["points"].shape[0]
+    for k, (a, b) in enumerate(zip(_blocks(G, 9, n_obs), _blocks(fx["G"], 9, n_obs))):
+        assert rel_norm(a, b) < TOL, k
+        assert max_rel_to_scale(a, b) < TOL, k
+    # column-major copy is the same matrix
+    assert np.array_equal(f.get_G(col_major=True), G)
+
+
+def test_assemble_reference_pairs(ctx):
+    fx = load("ref_pairs.npz")
+    vals = fx["values"]
+    # one (point, prism) per call batch: points as rows, one prism at a time is
+    # slow, so evaluate every point against every prism and take the diagonal
+    n = 256
+    f = Fwd(fx["prisms"][:n], fx["points"][:n], 0x3FE, ctx=ctx)
+    f.compute_G()
+    G = f.get_G()
+    for k, bit in enumerate(range(1, 10)):
+        got = np.array([G[k * n + q, q] for q in range(n)])
+        ref = vals[:n, bit]
+        assert max_rel_to_scale(got, ref) < 1e-11, bit
+        assert rel_norm(got, ref) < 1e-11, bit
+
+
+@pytest.mark.parametrize("flag", [0x12, 0x3FE, 0x2, 0x10, 0x240])
+def test_assemble_vs_oracle_reference_grid(ctx, flag):
+    """40x40x20 mesh of the reference examples, points on the golden grid
+    (on face planes and edge extensions), all component sets that have their
+    own kernel instantiation plus a generic one."""
+    prisms = M.regular_mesh((0.0, 2000.0), 40, (0.0, 2000.0), 40, (0.0, 1000.0), 20)
+    pts = M.grid_points(41, 0.0, 2000.0, 41, 0.0, 2000.0, -0.1)[::97]
+    f = Fwd(prisms, pts, flag, ctx=ctx)
+    f.compute_G()
+    G = f.get_G()
+    Go, st = oracle.assemble(prisms, pts, flag)
+    assert st == 0
+    nc = bin(flag).count("1")
+    for k, (a, b) in enumerate(zip(_blocks(G, nc, len(pts)), _blocks(Go, nc, len(pts)))):
+        assert rel_norm(a, b) < TOL, k
+        assert max_rel_to_scale(a, b) < TOL, k
+
+
+def test_store_f32_bf16_are_rounded_f64(ctx):
+    prisms = M.regular_mesh((0.0, 500.0), 10, (0.0, 500.0), 10, (0.0, 250.0), 5)
+    pts = M.grid_points(9, 0.0, 500.0, 9, 0.0, 500.0, -3.0)
+    f = Fwd(prisms, pts, 0x12, ctx=ctx)
+    f.compute_G("f64")
+    G64 = f.get_G()
+    f.compute_G("f32")
+    G32 = f.get_G()
+    assert np.array_equal(G32, G64.astype(np.float32).astype(np.float64))
+    f.compute_G("bf16")
+    Gb = f.get_G()
+    f32 = G64.astype(np.float32)
+    u = f32.view(np.uint32).astype(np.uint64)
+    rounded = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16     # round-to-nearest-even
+    ref = rounded.astype(np.uint32).view(np.float32).astype(np.float64)
+    assert np.array_equal(Gb, ref)
+
+
+@pytest.mark.parametrize("name", ["model1", "model3", "model2"])
+def test_forward_vs_reference_golden(ctx, name):
+    fx = load(f"ref_forward_{name}.npz")
+    prisms, rho = model_from_fixture(fx)
+    flag = int(fx["flag"])
+    print(f"[gpu] forward {name}: {prisms.shape[0]} cells x {fx['points'].shape[0]} points",
+          flush=True)
+    out = forward(prisms, rho, fx["points"], flag, ctx=ctx)   # all cells, as compute does
+    bits = sorted(fx["bits"].tolist())
+    out = out[:, [bits.index(b) for b in fx["bits"]]]
+    gold = fx["values"]
+    for k in range(gold.shape[1]):
+        assert rel_norm(out[:, k], gold[:, k]) < TOL, k
+        assert max_rel_to_scale(out[:, k], gold[:, k]) < TOL, k
+
+
+def test_forward_equals_G_times_rho(ctx):
+    prisms = M.regular_mesh((0.0, 1000.0), 20, (0.0, 800.0), 16, (0.0, 500.0), 10)
+    rng = np.random.default_rng(4)
+    rho = rng.uniform(-500, 500, prisms.shape[0])
+    pts = M.grid_points(12, -100.0, 1100.0, 10, -100.0, 900.0, -20.0)
+    out = forward(prisms, rho, pts, 0x3FE, ctx=ctx)
+    f = Fwd(prisms, pts, 0x3FE, ctx=ctx)
+    f.compute_G()
+    d = f.G_vec_mul(rho).reshape(9, -1).T
+    assert rel_norm(out, d) < 1e-12
+
+
+def test_degenerate_and_bad_arguments_raise(ctx):
+    pr = np.array([[0.0, 50.0, 0.0, 50.0, 0.0, 50.0]])
+    with pytest.raises(GraError) as e:
+        forward(pr, np.ones(1), np.array([[0.0, 0.0, 20.0]]), 0x10, ctx=ctx)
+    assert e.value.status == 5
+    with pytest.raises(GraError) as e:
+        forward(np.array([[0.0, -1.0, 0.0, 1.0, 0.0, 1.0]]), np.ones(1),
+                np.array([[0.0, 0.0, -1.0]]), 0x2, ctx=ctx)
+    assert e.value.status == 1
+    with pytest.raises(GraError) as e:
+        forward(pr, np.ones(1), np.array([[0.0, 0.0, -1.0]]), 0x1, ctx=ctx)
+    assert e.value.status == 1
+    # the context stays usable after errors
+    v = forward(pr, np.ones(1), np.array([[25.0, 25.0, -10.0]]), 0x2, ctx=ctx)
+    o, _ = oracle.forward(pr, np.ones(1), np.array([[25.0, 25.0, -10.0]]), 0x2)
+    assert abs(v[0, 0] - o[0, 0]) < 1e-13 * abs(o[0, 0])

@@ -1,0 +1,169 @@
+"""The prism oracle against what the mathematics fixes, independently of
+the reference's formulation:
+
+* volume quadrature of the Newtonian kernels, g_i = G0 int d_i/R^3 dV and
+  T_ij = G0 int (3 d_i d_j - delta_ij R^2)/R^5 dV (d = x' - a), in mGal / E;
+* Laplace: Txx + Tyy + Tzz = 0 outside the body;
+* superposition: a prism equals the sum of its 8 octants;
+* axis relabelling: gx, gy, Txx, ... equal gz, Tzz, ... of the prism with
+  coordinates permuted;
+* T = grad g (central differences);
+* a point on a prism edge is the reference's asserted (degenerate) case.
+A dropped term, a flipped sign, a swapped face or a wrong unit factor fails
+at least one of these.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+G0 = 6.673e-11
+ALL = 0x3FE
+IDX = {"gz": 1, "gx": 2, "gy": 3, "Tzz": 4, "Txz": 5, "Tyz": 6, "Txx": 7, "Txy": 8, "Tyy": 9}
+
+
+def quadrature(pt, pr, n=24):
+    x, w = np.polynomial.legendre.leggauss(n)
+
+    def m(a, b):
+        return 0.5 * (b - a) * x + 0.5 * (a + b), 0.5 * (b - a) * w
+
+    X, WX = m(pr[0], pr[1])
+    Y, WY = m(pr[2], pr[3])
+    Z, WZ = m(pr[4], pr[5])
+    XX, YY, ZZ = np.meshgrid(X, Y, Z, indexing="ij")
+    W = WX[:, None, None] * WY[None, :, None] * WZ[None, None, :]
+    d = [XX - pt[0], YY - pt[1], ZZ - pt[2]]
+    R = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+    g = [1e5 * G0 * np.sum(W * d[i] / R ** 3) for i in range(3)]
+
+    def T(i, j):
+        return 1e9 * G0 * np.sum(W * (3 * d[i] * d[j] - (i == j) * R ** 2) / R ** 5)
+
+    out = np.zeros(10)
+    out[1], out[2], out[3] = g[2], g[0], g[1]
+    out[4], out[5], out[6] = T(2, 2), T(0, 2), T(1, 2)
+    out[7], out[8], out[9] = T(0, 0), T(0, 1), T(1, 1)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_matches_volume_quadrature(seed):
+    rng = np.random.default_rng(seed)
+    pr = np.sort(rng.uniform(0, 400, (3, 2)), axis=1).ravel()
+    c = 0.5 * (pr[0::2] + pr[1::2])
+    size = (pr[1::2] - pr[0::2]).max()
+    direction = rng.standard_normal(3)
+    pt = c + direction / np.linalg.norm(direction) * size * rng.uniform(1.5, 4.0)
+    v, st = oracle.prism_field(pt, pr, 1.0, ALL)
+    q = quadrature(pt, pr, 40)
+    assert st == 0
+    scale_g = np.abs(q[1:4]).max()
+    scale_t = np.abs(q[4:]).max()
+    assert np.abs(v[1:4] - q[1:4]).max() < 1e-9 * scale_g
+    assert np.abs(v[4:] - q[4:]).max() < 1e-9 * scale_t
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_laplace_outside(seed):
+    rng = np.random.default_rng(100 + seed)
+    for _ in range(40):
+        pr = np.sort(rng.uniform(0, 500, (3, 2)), axis=1).ravel()
+        pt = rng.uniform(-800, 1300, 3)
+        inside = np.all((pt >= pr[0::2]) & (pt <= pr[1::2]))
+        if inside:
+            continue
+        v, st = oracle.prism_field(pt, pr, 1.0, ALL)
+        if st:
+            continue
+        s = np.abs(v[[4, 7, 9]]).max()
+        # far-field cancellation of the face/edge form limits this to ~1e-10
+        assert abs(v[4] + v[7] + v[9]) < 1e-9 * s
+
+
+def test_superposition_of_octants():
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        pr = np.sort(rng.uniform(0, 600, (3, 2)), axis=1).ravel()
+        mid = 0.5 * (pr[0::2] + pr[1::2])
+        pt = rng.uniform(-500, 1100, 3)
+        pt[2] = rng.uniform(-300, -1)
+        whole, st = oracle.prism_field(pt, pr, 1.0, ALL)
+        assert st == 0
+        acc = np.zeros(10)
+        for ix in range(2):
+            for iy in range(2):
+                for iz in range(2):
+                    lo = [pr[0], mid[0]][ix], [pr[2], mid[1]][iy], [pr[4], mid[2]][iz]
+                    hi = [mid[0], pr[1]][ix], [mid[1], pr[3]][iy], [mid[2], pr[5]][iz]
+                    ch = np.array([lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]])
+                    v, s2 = oracle.prism_field(pt, ch, 1.0, ALL)
+                    assert s2 == 0
+                    acc += v
+        scale = np.abs(whole[1:4]).max(), np.abs(whole[4:]).max()
+        # bounded by the face/edge form's far-field cancellation (~1e-10)
+        assert np.abs(acc[1:4] - whole[1:4]).max() < 1e-9 * scale[0]
+        assert np.abs(acc[4:] - whole[4:]).max() < 1e-9 * scale[1]
+
+
+def _perm(v, p):
+    return np.array([v[2 * p[0]], v[2 * p[0] + 1], v[2 * p[1]], v[2 * p[1] + 1], v[2 * p[2]],
+                     v[2 * p[2] + 1]])
+
+
+def test_axis_relabelling():
+    """Swapping axes x<->z maps gx->gz, Txx->Tzz, Txy->Tyz, Txz->Txz; y<->z maps
+    gy->gz, Tyy->Tzz, Txy->Txz."""
+    rng = np.random.default_rng(9)
+    for _ in range(40):
+        pr = np.sort(rng.uniform(0, 500, (3, 2)), axis=1).ravel()
+        pt = rng.uniform(-400, 900, 3)
+        v, st = oracle.prism_field(pt, pr, 1.0, ALL)
+        if st:
+            continue
+        s = np.abs(v[1:4]).max(), np.abs(v[4:]).max()
+        # x <-> z
+        w, _ = oracle.prism_field(pt[[2, 1, 0]], _perm(pr, (2, 1, 0)), 1.0, ALL)
+        assert abs(w[IDX["gz"]] - v[IDX["gx"]]) < 1e-9 * s[0]
+        assert abs(w[IDX["Tzz"]] - v[IDX["Txx"]]) < 1e-9 * s[1]
+        assert abs(w[IDX["Tyz"]] - v[IDX["Txy"]]) < 1e-9 * s[1]
+        assert abs(w[IDX["Txz"]] - v[IDX["Txz"]]) < 1e-9 * s[1]
+        # y <-> z
+        w, _ = oracle.prism_field(pt[[0, 2, 1]], _perm(pr, (0, 2, 1)), 1.0, ALL)
+        assert abs(w[IDX["gz"]] - v[IDX["gy"]]) < 1e-9 * s[0]
+        assert abs(w[IDX["Tzz"]] - v[IDX["Tyy"]]) < 1e-9 * s[1]
+        assert abs(w[IDX["Txz"]] - v[IDX["Txy"]]) < 1e-9 * s[1]
+
+
+def test_gradient_tensor_is_gradient_of_g():
+    rng = np.random.default_rng(3)
+    h = 1e-2
+    for _ in range(20):
+        pr = np.sort(rng.uniform(0, 300, (3, 2)), axis=1).ravel()
+        pt = np.array([rng.uniform(-200, 500), rng.uniform(-200, 500), rng.uniform(-300, -20)])
+        v, _ = oracle.prism_field(pt, pr, 1.0, ALL)
+        for axis, (gbit, tbit) in enumerate([(2, 7), (3, 9), (1, 4)]):
+            e = np.zeros(3)
+            e[axis] = h
+            gp, _ = oracle.prism_field(pt + e, pr, 1.0, ALL)
+            gm, _ = oracle.prism_field(pt - e, pr, 1.0, ALL)
+            fd = (gp[gbit] - gm[gbit]) / (2 * h) * 1e4      # mGal/m -> Eotvos
+            assert abs(fd - v[tbit]) < 1e-5 * abs(v[tbit]) + 1e-12
+
+
+def test_density_scales_linearly():
+    pr = np.array([0.0, 50.0, 0.0, 50.0, 100.0, 150.0])
+    pt = np.array([10.0, 70.0, -5.0])
+    a, _ = oracle.prism_field(pt, pr, 1.0, ALL)
+    b, _ = oracle.prism_field(pt, pr, -250.0, ALL)
+    assert np.allclose(b[1:], -250.0 * a[1:], rtol=1e-15, atol=0)
+
+
+def test_point_on_edge_is_degenerate_and_face_plane_is_not():
+    pr = np.array([0.0, 50.0, 0.0, 50.0, 0.0, 50.0])
+    _, st = oracle.prism_field(np.array([0.0, 0.0, 20.0]), pr, 1.0, ALL)     # on the edge
+    assert st == 1
+    v, st = oracle.prism_field(np.array([0.0, 0.0, -0.1]), pr, 1.0, ALL)     # on edge's extension
+    assert st == 0 and np.all(np.isfinite(v[1:]))
+    v, st = oracle.prism_field(np.array([25.0, 25.0, 0.0]), pr, 1.0, 0x2)    # on the top face
+    assert st == 0 and np.isfinite(v[1])
