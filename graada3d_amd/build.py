@@ -76,6 +76,16 @@ def build_hostcheck() -> str:
     return out
 
 
+def build_adaptor_check() -> str:
+    """Test-support program: a C++ host using include/graeng.hpp."""
+    src = os.path.join(ROOT, "tests", "cpp", "adaptor_check.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "adaptor_check")
+    _run(["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-o", out, src, "-L", HERE, "-lgraeng",
+          "-Wl,-rpath,$ORIGIN/../../graada3d_amd"])
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_hostcheck())
+    print(build_adaptor_check())

@@ -44,7 +44,9 @@ assemble_kernel(const double *__restrict__ prisms, int64_t Nm, const double *__r
 #pragma unroll
     for (int q = 0; q < 6; ++q) pr[q] = prisms[6 * j + q];
     double out[10];
-    int st = prism_fields(pt, pr, 1.0, flag, out);
+    int st;
+    if constexpr (FLAG != 0u) st = prism_fields_t<FLAG>(pt, pr, 1.0, out);
+    else st = prism_fields(pt, pr, 1.0, flag, out);
     if (st) atomicOr(bad, 1);
     int k = 0;
 #pragma unroll
@@ -76,7 +78,8 @@ forward_kernel(const double *__restrict__ prisms, const double *__restrict__ rho
 #pragma unroll
         for (int q = 0; q < 6; ++q) pr[q] = prisms[6 * j + q];
         double f[10];
-        st |= prism_fields(pt, pr, rho[j], flag, f);
+        if constexpr (FLAG != 0u) st |= prism_fields_t<FLAG>(pt, pr, rho[j], f);
+        else st |= prism_fields(pt, pr, rho[j], flag, f);
 #pragma unroll
         for (int bit = 1; bit < 10; ++bit)
             if (flag & (1u << bit)) acc[bit] += f[bit];

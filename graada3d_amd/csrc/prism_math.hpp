@@ -90,27 +90,29 @@ struct FaceDef {
 //   z faces {p3,p2,p1,p0} / {p4,p5,p6,p7}   (GravFormula.cpp:133-134)
 //   x faces {p0,p4,p7,p3} / {p1,p2,p6,p5}   (GravFormula.cpp:87-88)
 //   y faces {p0,p1,p5,p4} / {p2,p3,p7,p6}   (GravFormula.cpp:110-111)
+// The table is constexpr and every use below has a compile-time face index
+// and unrolled edge loops, so all vertex / axis indices fold to constants
+// and the per-thread arrays stay in registers.
 enum FaceId { kZ0 = 0, kZ1, kX0, kX1, kY0, kY1 };
 
-GRA_HD inline FaceDef face_def(int f)
-{
-    switch (f) {
-    case kZ0: return FaceDef{{{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0}}, 2, -1};
-    case kZ1: return FaceDef{{{0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}}, 2, +1};
-    case kX0: return FaceDef{{{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}}, 0, -1};
-    case kX1: return FaceDef{{{1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}}, 0, +1};
-    case kY0: return FaceDef{{{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}}, 1, -1};
-    default:  return FaceDef{{{1, 1, 0}, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}}, 1, +1};
-    }
-}
+constexpr FaceDef kFaces[6] = {
+    {{{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0}}, 2, -1},
+    {{{0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}}, 2, +1},
+    {{{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}}, 0, -1},
+    {{{1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}}, 0, +1},
+    {{{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}}, 1, -1},
+    {{{1, 1, 0}, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}}, 1, +1},
+};
 
-// Builds edge j of face `fd` (v0 = vertex j, v1 = vertex j+1).
-GRA_HD inline Edge make_edge(const Geo &g, const FaceDef &fd, int j, const double *pr)
+// Builds edge J of face F (v0 = vertex J, v1 = vertex J+1).
+template <int F, int J>
+GRA_HD inline Edge make_edge(const Geo &g, const double *pr)
 {
-    const int8_t *u = fd.v[j];
-    const int8_t *w = fd.v[(j + 1) & 3];
-    int ax = (u[0] != w[0]) ? 0 : ((u[1] != w[1]) ? 1 : 2);
-    double dir = (w[ax] > u[ax]) ? 1.0 : -1.0;           // t = dir * e_ax
+    constexpr FaceDef fd = kFaces[F];
+    constexpr int8_t u[3] = {fd.v[J][0], fd.v[J][1], fd.v[J][2]};
+    constexpr int8_t w[3] = {fd.v[(J + 1) & 3][0], fd.v[(J + 1) & 3][1], fd.v[(J + 1) & 3][2]};
+    constexpr int ax = (u[0] != w[0]) ? 0 : ((u[1] != w[1]) ? 1 : 2);
+    constexpr double dir = (w[ax] > u[ax]) ? 1.0 : -1.0;   // t = dir * e_ax
     Edge e;
     e.ax = ax;
     e.s0 = dir * g.D[ax][u[ax]];
@@ -119,13 +121,13 @@ GRA_HD inline Edge make_edge(const Geo &g, const FaceDef &fd, int j, const doubl
     e.R1 = g.R[w[0]][w[1]][w[2]];
     // In-plane outward normal m = t x n lies on the remaining axis; the edge
     // sits on the bound of that axis it shares with both vertices.
-    int max_ = 3 - ax - fd.nax;
-    int side = u[max_];
+    constexpr int max_ = 3 - ax - fd.nax;
+    constexpr int side = u[max_];
     e.max_ = max_;
     e.msign = side ? 1.0 : -1.0;
     e.mv = e.msign * g.D[max_][side];
     e.c_next = pr[2 * ax + w[ax]];
-    int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    constexpr int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
     e.pa = g.D[o1][w[o1]];
     e.pb = g.D[o2][w[o2]];
     return e;
@@ -146,11 +148,19 @@ GRA_HD inline double edge_L0(const Geo &g, const Edge &e)
     return sqrt((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]);
 }
 
-// h = (a - f0).n for the face (n = nsign * e_nax).
-GRA_HD inline double face_h(const Geo &g, const FaceDef &fd)
+// h = (a - f0).n for face F (n = nsign * e_nax).
+template <int F> GRA_HD inline double face_h(const Geo &g)
 {
-    int side = fd.v[0][fd.nax];
-    return -(fd.nsign * g.D[fd.nax][side]);
+    constexpr FaceDef fd = kFaces[F];
+    return -(double(fd.nsign) * g.D[fd.nax][fd.v[0][fd.nax]]);
+}
+
+template <int F> GRA_HD inline void face_edges(const Geo &g, const double *pr, Edge *E)
+{
+    E[0] = make_edge<F, 0>(g, pr);
+    E[1] = make_edge<F, 1>(g, pr);
+    E[2] = make_edge<F, 2>(g, pr);
+    E[3] = make_edge<F, 3>(g, pr);
 }
 
 // log((s1 + R1)/(s0 + R0)) of an edge (shared by Surface_R_1 and Line_R_1).
@@ -251,9 +261,108 @@ GRA_HD inline double surface_rr_3_normal(const Geo &g, const Edge *E, double h, 
     return poison ? (double)NAN : r;
 }
 
-// Evaluates the requested components (field bits 1..9 in `flag`) of one
-// (point, prism) pair with density rho into out[bit].  Returns 1 when the
-// reference would have asserted (degenerate geometry), else 0.
+// One face normal to z: Surface_R_1 (gz) and the normal component of
+// Surface_rR_3 (Tzz).
+template <int F, uint32_t FLAG>
+GRA_HD inline void z_face(const Geo &g, const double *pr, double &vR1, double &vT, int *bad)
+{
+    Edge E[4];
+    face_edges<F>(g, pr, E);
+    const double h = face_h<F>(g);
+    if (FLAG & kGz) {
+        double lg[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lg[j] = fabs(E[j].mv) < kTol ? 0.0 : edge_log(E[j]);
+        vR1 = surface_r_1(E, lg, h);
+    }
+    if (FLAG & kTzz) vT = surface_rr_3_normal(g, E, h, double(kFaces[F].nsign), bad);
+}
+
+// One face normal to x or y: Surface_R_1 (gx / gy) and the full vector
+// Surface_rR_3 (or only its normal component when that is all FLAG needs).
+template <int F, bool R1, bool FULL, bool NORMAL>
+GRA_HD inline void side_face(const Geo &g, const double *pr, double &vR1, double *vT, int *bad)
+{
+    Edge E[4];
+    double lg[4];
+    face_edges<F>(g, pr, E);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) lg[j] = edge_log(E[j]);
+    const double h = face_h<F>(g);
+    constexpr int nax = kFaces[F].nax;
+    if (R1) vR1 = surface_r_1(E, lg, h);
+    if (FULL) {
+        surface_rr_3(g, E, lg, h, nax, double(kFaces[F].nsign), vT, bad);
+    } else if (NORMAL) {
+        vT[nax] = surface_rr_3_normal(g, E, h, double(kFaces[F].nsign), bad);
+    }
+}
+
+// Face groups.  Each evaluates the two faces normal to one axis for a
+// compile-time component subset and writes only those outputs; `bad` is set
+// only by the evaluations the subset needs, as in the reference.
+template <bool GZ, bool TZZ>
+GRA_HD inline void z_group(const Geo &g, const double *pr, double cg, double ct, double *out,
+                           int *bad)
+{
+    constexpr uint32_t F = (GZ ? kGz : 0u) | (TZZ ? kTzz : 0u);
+    double r0 = 0, r1 = 0, t0 = 0, t1 = 0;
+    z_face<kZ0, F>(g, pr, r0, t0, bad);
+    z_face<kZ1, F>(g, pr, r1, t1, bad);
+    if (GZ) out[1] = cg * (-r0 + r1);
+    if (TZZ) out[4] = ct * (-t0 + t1);
+}
+
+// x faces: R1 -> gx; FULL -> Txy and Txz (and Txx); NORMAL -> Txx only.
+template <bool R1, bool FULL, bool NORMAL>
+GRA_HD inline void x_group(const Geo &g, const double *pr, double cg, double ct, uint32_t flag,
+                           double *out, int *bad)
+{
+    double r0 = 0, r1 = 0, t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
+    side_face<kX0, R1, FULL, NORMAL>(g, pr, r0, t0, bad);
+    side_face<kX1, R1, FULL, NORMAL>(g, pr, r1, t1, bad);
+    if (R1) out[2] = cg * (-r0 + r1);
+    if (flag & kTxz) out[5] = ct * (-t0[2] + t1[2]);
+    if (flag & kTxx) out[7] = ct * (-t0[0] + t1[0]);
+    if (flag & kTxy) out[8] = ct * (-t0[1] + t1[1]);
+}
+
+// y faces: R1 -> gy; FULL -> Tyz (and Tyy); NORMAL -> Tyy only.
+template <bool R1, bool FULL, bool NORMAL>
+GRA_HD inline void y_group(const Geo &g, const double *pr, double cg, double ct, uint32_t flag,
+                           double *out, int *bad)
+{
+    double r0 = 0, r1 = 0, t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
+    side_face<kY0, R1, FULL, NORMAL>(g, pr, r0, t0, bad);
+    side_face<kY1, R1, FULL, NORMAL>(g, pr, r1, t1, bad);
+    if (R1) out[3] = cg * (-r0 + r1);
+    if (flag & kTyz) out[6] = ct * (-t0[2] + t1[2]);
+    if (flag & kTyy) out[9] = ct * (-t0[1] + t1[1]);
+}
+
+// Components of a compile-time FLAG (field bits 1..9) of one (point, prism)
+// pair with density rho into out[bit].  Returns 1 when the reference would
+// have asserted (degenerate geometry), else 0.
+template <uint32_t FLAG>
+GRA_HD inline int prism_fields_t(const double *pt, const double *pr, double rho, double *out)
+{
+    Geo g;
+    geo_init(g, pt, pr);
+    int bad = 0;
+    const double cg = -kSI2mGal * kG0 * rho;
+    const double ct = -kSI2Eotvos * kG0 * rho;
+    if (FLAG & (kGz | kTzz))
+        z_group<(FLAG & kGz) != 0, (FLAG & kTzz) != 0>(g, pr, cg, ct, out, &bad);
+    if (FLAG & (kGx | kTxx | kTxy | kTxz))
+        x_group<(FLAG & kGx) != 0, (FLAG & (kTxy | kTxz)) != 0, (FLAG & kTxx) != 0>(
+            g, pr, cg, ct, FLAG, out, &bad);
+    if (FLAG & (kGy | kTyy | kTyz))
+        y_group<(FLAG & kGy) != 0, (FLAG & kTyz) != 0, (FLAG & kTyy) != 0>(g, pr, cg, ct, FLAG,
+                                                                            out, &bad);
+    return bad;
+}
+
+// Runtime-flag entry: dispatches each face group to its instantiation.
 GRA_HD inline int prism_fields(const double *pt, const double *pr, double rho, uint32_t flag,
                                double *out)
 {
@@ -262,67 +371,38 @@ GRA_HD inline int prism_fields(const double *pt, const double *pr, double rho, u
     int bad = 0;
     const double cg = -kSI2mGal * kG0 * rho;
     const double ct = -kSI2Eotvos * kG0 * rho;
-
-    // z faces: gz (Surface_R_1), Tzz (normal component of Surface_rR_3)
-    if (flag & (kGz | kTzz)) {
-        double vR1[2] = {0, 0}, vT[2] = {0, 0};
-        for (int s = 0; s < 2; ++s) {
-            FaceDef fd = face_def(kZ0 + s);
-            Edge E[4];
-            for (int j = 0; j < 4; ++j) E[j] = make_edge(g, fd, j, pr);
-            double h = face_h(g, fd);
-            if (flag & kGz) {
-                double lg[4];
-                for (int j = 0; j < 4; ++j) lg[j] = fabs(E[j].mv) < kTol ? 0.0 : edge_log(E[j]);
-                vR1[s] = surface_r_1(E, lg, h);
-            }
-            if (flag & kTzz) vT[s] = surface_rr_3_normal(g, E, h, (double)fd.nsign, &bad);
-        }
-        if (flag & kGz) out[1] = cg * (-vR1[0] + vR1[1]);
-        if (flag & kTzz) out[4] = ct * (-vT[0] + vT[1]);
+    switch (flag & (kGz | kTzz)) {
+    case kGz: z_group<true, false>(g, pr, cg, ct, out, &bad); break;
+    case kTzz: z_group<false, true>(g, pr, cg, ct, out, &bad); break;
+    case kGz | kTzz: z_group<true, true>(g, pr, cg, ct, out, &bad); break;
+    default: break;
     }
-    // x faces: gx, Txx (normal comp.), Txy, Txz (edge comps.)
-    if (flag & (kGx | kTxx | kTxy | kTxz)) {
-        double vR1[2] = {0, 0}, vT[2][3] = {{0, 0, 0}, {0, 0, 0}};
-        for (int s = 0; s < 2; ++s) {
-            FaceDef fd = face_def(kX0 + s);
-            Edge E[4];
-            double lg[4];
-            for (int j = 0; j < 4; ++j) E[j] = make_edge(g, fd, j, pr);
-            for (int j = 0; j < 4; ++j) lg[j] = edge_log(E[j]);
-            double h = face_h(g, fd);
-            if (flag & kGx) vR1[s] = surface_r_1(E, lg, h);
-            if (flag & (kTxy | kTxz)) {
-                surface_rr_3(g, E, lg, h, fd.nax, (double)fd.nsign, vT[s], &bad);
-            } else if (flag & kTxx) {
-                vT[s][0] = surface_rr_3_normal(g, E, h, (double)fd.nsign, &bad);
-            }
+    {
+        const int sel = ((flag & kGx) ? 4 : 0) | ((flag & (kTxy | kTxz)) ? 2 : 0) |
+                        ((flag & kTxx) ? 1 : 0);
+        switch (sel) {
+        case 1: x_group<false, false, true>(g, pr, cg, ct, flag, out, &bad); break;
+        case 2: x_group<false, true, false>(g, pr, cg, ct, flag, out, &bad); break;
+        case 3: x_group<false, true, true>(g, pr, cg, ct, flag, out, &bad); break;
+        case 4: x_group<true, false, false>(g, pr, cg, ct, flag, out, &bad); break;
+        case 5: x_group<true, false, true>(g, pr, cg, ct, flag, out, &bad); break;
+        case 6: x_group<true, true, false>(g, pr, cg, ct, flag, out, &bad); break;
+        case 7: x_group<true, true, true>(g, pr, cg, ct, flag, out, &bad); break;
+        default: break;
         }
-        if (flag & kGx) out[2] = cg * (-vR1[0] + vR1[1]);
-        if (flag & kTxz) out[5] = ct * (-vT[0][2] + vT[1][2]);
-        if (flag & kTxx) out[7] = ct * (-vT[0][0] + vT[1][0]);
-        if (flag & kTxy) out[8] = ct * (-vT[0][1] + vT[1][1]);
     }
-    // y faces: gy, Tyy (normal comp.), Tyz (edge comp.)
-    if (flag & (kGy | kTyy | kTyz)) {
-        double vR1[2] = {0, 0}, vT[2][3] = {{0, 0, 0}, {0, 0, 0}};
-        for (int s = 0; s < 2; ++s) {
-            FaceDef fd = face_def(kY0 + s);
-            Edge E[4];
-            double lg[4];
-            for (int j = 0; j < 4; ++j) E[j] = make_edge(g, fd, j, pr);
-            for (int j = 0; j < 4; ++j) lg[j] = edge_log(E[j]);
-            double h = face_h(g, fd);
-            if (flag & kGy) vR1[s] = surface_r_1(E, lg, h);
-            if (flag & kTyz) {
-                surface_rr_3(g, E, lg, h, fd.nax, (double)fd.nsign, vT[s], &bad);
-            } else if (flag & kTyy) {
-                vT[s][1] = surface_rr_3_normal(g, E, h, (double)fd.nsign, &bad);
-            }
+    {
+        const int sel = ((flag & kGy) ? 4 : 0) | ((flag & kTyz) ? 2 : 0) | ((flag & kTyy) ? 1 : 0);
+        switch (sel) {
+        case 1: y_group<false, false, true>(g, pr, cg, ct, flag, out, &bad); break;
+        case 2: y_group<false, true, false>(g, pr, cg, ct, flag, out, &bad); break;
+        case 3: y_group<false, true, true>(g, pr, cg, ct, flag, out, &bad); break;
+        case 4: y_group<true, false, false>(g, pr, cg, ct, flag, out, &bad); break;
+        case 5: y_group<true, false, true>(g, pr, cg, ct, flag, out, &bad); break;
+        case 6: y_group<true, true, false>(g, pr, cg, ct, flag, out, &bad); break;
+        case 7: y_group<true, true, true>(g, pr, cg, ct, flag, out, &bad); break;
+        default: break;
         }
-        if (flag & kGy) out[3] = cg * (-vR1[0] + vR1[1]);
-        if (flag & kTyz) out[6] = ct * (-vT[0][2] + vT[1][2]);
-        if (flag & kTyy) out[9] = ct * (-vT[0][1] + vT[1][1]);
     }
     return bad;
 }

@@ -12,5 +12,12 @@ extern "C" int hc_prism_field(const double *pt, const double *pr, double rho, ui
                               double *out10)
 {
     for (int k = 0; k < 10; ++k) out10[k] = 0.0;
-    return gra::prism_fields(pt, pr, rho, flag, out10);
+    // the compile-time instantiations the GPU kernels use, else the runtime path
+    switch (flag) {
+    case gra::kGz | gra::kTzz: return gra::prism_fields_t<gra::kGz | gra::kTzz>(pt, pr, rho, out10);
+    case gra::kGz: return gra::prism_fields_t<gra::kGz>(pt, pr, rho, out10);
+    case gra::kTzz: return gra::prism_fields_t<gra::kTzz>(pt, pr, rho, out10);
+    case gra::kAllFields: return gra::prism_fields_t<gra::kAllFields>(pt, pr, rho, out10);
+    default: return gra::prism_fields(pt, pr, rho, flag, out10);
+    }
 }
