@@ -163,6 +163,9 @@ def test_point_on_edge_is_degenerate_and_face_plane_is_not():
     pr = np.array([0.0, 50.0, 0.0, 50.0, 0.0, 50.0])
     _, st = oracle.prism_field(np.array([0.0, 0.0, 20.0]), pr, 1.0, ALL)     # on the edge
     assert st == 1
+    # the vertical edge belongs to the x and y faces only: Tzz (z faces) is regular
+    assert oracle.prism_field(np.array([0.0, 0.0, 20.0]), pr, 1.0, 0x10)[1] == 0
+    assert oracle.prism_field(np.array([0.0, 0.0, 20.0]), pr, 1.0, 0x20)[1] == 1
     v, st = oracle.prism_field(np.array([0.0, 0.0, -0.1]), pr, 1.0, ALL)     # on edge's extension
     assert st == 0 and np.all(np.isfinite(v[1:]))
     v, st = oracle.prism_field(np.array([25.0, 25.0, 0.0]), pr, 1.0, 0x2)    # on the top face
