@@ -413,8 +413,9 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         return set_error(c, GRA_BAD_ARG, "cross-gradient requested but no T operators uploaded");
     const int nseg = crg ? 7 : 4;
     const int64_t nrow = nd + nseg * nm;
-    const int splits = gemv_n_splits(nd, nm);
-    const int chunks = gemv_t_chunks(nd, nm);
+    const GemvShape shape = gemv_shape(c);
+    const int splits = shape.n_splits;
+    const int chunks = shape.t_chunks;
     hipError_t e;
 
     // work vectors
@@ -505,15 +506,15 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     // b, s, first residual and A^T b
     {
         TimedRegion tr(c, kTimeGemvN);
-        if ((e = launch_gemv_n(c.G.p, c.store, nd, nm, ld, d_mk, c.part_n.as<double>(), splits,
-                               nullptr, st)))
+        if ((e = launch_gemv_n(c.G.p, c.store, nd, ld, shape, d_mk, c.part_n.as<double>(), nullptr,
+                               st)))
             return hip_fail(c, e, "gemv_n");
     }
     hipLaunchKernelGGL(k_b_init, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
     {
         TimedRegion tr(c, kTimeGemvT);
-        if ((e = launch_gemv_t(c.G.p, c.store, nd, nm, ld, d.w, c.part_t.as<double>(), chunks,
-                               nullptr, st)))
+        if ((e = launch_gemv_t(c.G.p, c.store, nd, ld, shape, d.w, c.part_t.as<double>(), nullptr,
+                               st)))
             return hip_fail(c, e, "gemv_t");
     }
     hipLaunchKernelGGL(k_r_update<1>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
@@ -526,8 +527,8 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
         {
             TimedRegion tr(c, kTimeGemvN);
-            e = launch_gemv_n(c.G.p, c.store, nd, nm, ld, d.Pp, c.part_n.as<double>(), splits,
-                              done, st);
+            e = launch_gemv_n(c.G.p, c.store, nd, ld, shape, d.Pp, c.part_n.as<double>(), done,
+                              st);
             if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
         }
         hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
@@ -535,8 +536,8 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
         {
             TimedRegion tr(c, kTimeGemvT);
-            e = launch_gemv_t(c.G.p, c.store, nd, nm, ld, d.w, c.part_t.as<double>(), chunks,
-                              done, st);
+            e = launch_gemv_t(c.G.p, c.store, nd, ld, shape, d.w, c.part_t.as<double>(), done,
+                              st);
             if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
         }
         hipLaunchKernelGGL(k_r_update<0>, dim3(nblocks(nm)), dim3(kB), 0, st, d);

@@ -134,6 +134,19 @@ size_t dtype_size(gra_dtype d) { return d == GRA_F64 ? 8 : (d == GRA_F32 ? 4 : 2
 
 }  // namespace
 
+GemvShape gemv_shape(const Ctx &c)
+{
+    GemvShape s = default_gemv_shape(c.store, c.Nd, c.Nm);
+    const GemvShape &o = c.tune_override;
+    if (o.n_rows) s.n_rows = o.n_rows;
+    if (o.n_unroll) s.n_unroll = o.n_unroll;
+    if (o.n_splits) s.n_splits = o.n_splits;
+    if (o.t_chunks) s.t_chunks = o.t_chunks;
+    if (o.t_unroll) s.t_unroll = o.t_unroll;
+    if (o.nontemporal >= 0) s.nontemporal = o.nontemporal;
+    return s;
+}
+
 gra_status upload_csr(Ctx &c, DevCsr &d, const gra_csr *h)
 {
     std::vector<int64_t> rpT;
@@ -317,28 +330,27 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
     if (trans) HIPCHK(put(x, vec, (size_t)nin, c.stream), "upload vec");
     else HIPCHK(put_padded(x, vec, c.Nm, c.ld, c.stream), "upload vec");
     HIPCHK(y.reserve(sizeof(double) * (size_t)nout), "alloc out");
+    const GemvShape shape = gemv_shape(c);
     if (!trans) {
-        int splits = gemv_n_splits(c.Nd, c.Nm);
-        HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)splits * c.Nd), "alloc part");
+        HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * c.Nd), "alloc part");
         {
             TimedRegion tr(c, kTimeGemvN);
-            HIPCHK(launch_gemv_n(c.G.p, c.store, c.Nd, c.Nm, c.ld, x.as<double>(),
-                                 c.part_n.as<double>(), splits, nullptr, c.stream),
+            HIPCHK(launch_gemv_n(c.G.p, c.store, c.Nd, c.ld, shape, x.as<double>(),
+                                 c.part_n.as<double>(), nullptr, c.stream),
                    "gemv_n");
         }
-        HIPCHK(launch_reduce_parts(c.part_n.as<double>(), splits, c.Nd, c.Nd, nullptr,
+        HIPCHK(launch_reduce_parts(c.part_n.as<double>(), shape.n_splits, c.Nd, c.Nd, nullptr,
                                    y.as<double>(), c.stream),
                "reduce");
     } else {
-        int chunks = gemv_t_chunks(c.Nd, c.Nm);
-        HIPCHK(c.part_t.reserve(sizeof(double) * (size_t)chunks * c.ld), "alloc part");
+        HIPCHK(c.part_t.reserve(sizeof(double) * (size_t)shape.t_chunks * c.ld), "alloc part");
         {
             TimedRegion tr(c, kTimeGemvT);
-            HIPCHK(launch_gemv_t(c.G.p, c.store, c.Nd, c.Nm, c.ld, x.as<double>(),
-                                 c.part_t.as<double>(), chunks, nullptr, c.stream),
+            HIPCHK(launch_gemv_t(c.G.p, c.store, c.Nd, c.ld, shape, x.as<double>(),
+                                 c.part_t.as<double>(), nullptr, c.stream),
                    "gemv_t");
         }
-        HIPCHK(launch_reduce_parts(c.part_t.as<double>(), chunks, c.ld, c.Nm, nullptr,
+        HIPCHK(launch_reduce_parts(c.part_t.as<double>(), shape.t_chunks, c.ld, c.Nm, nullptr,
                                    y.as<double>(), c.stream),
                "reduce");
     }
@@ -527,14 +539,14 @@ gra_status gra_misfit(gra_ctx *ctx, const double *m, const double *Wd, const dou
     HIPCHK(hipMemcpyAsync(v.as<double>() + nd, dobs, sizeof(double) * nd, hipMemcpyHostToDevice,
                           c.stream),
            "dobs");
-    int splits = gemv_n_splits(c.Nd, c.Nm);
-    HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)splits * nd), "alloc");
+    const GemvShape shape = gemv_shape(c);
+    HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * nd), "alloc");
     HIPCHK(c.red.reserve(sizeof(double) * (size_t)((nd + 255) / 256 + 1)), "alloc");
     HIPCHK(c.vec_nm[11].reserve(sizeof(double)), "alloc");
-    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, c.Nm, c.ld, x.as<double>(), c.part_n.as<double>(),
-                         splits, nullptr, c.stream),
+    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, c.ld, shape, x.as<double>(), c.part_n.as<double>(),
+                         nullptr, c.stream),
            "gemv_n");
-    HIPCHK(launch_misfit(c.part_n.as<double>(), splits, nd, v.as<double>(), v.as<double>() + nd,
+    HIPCHK(launch_misfit(c.part_n.as<double>(), shape.n_splits, nd, v.as<double>(), v.as<double>() + nd,
                          c.red.as<double>(), c.vec_nm[11].as<double>(), c.stream),
            "misfit");
     HIPCHK(hipMemcpyAsync(misfit, c.vec_nm[11].p, sizeof(double), hipMemcpyDeviceToHost, c.stream),
@@ -631,12 +643,12 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
     HIPCHK(launch_barrier_update(m, c.gn_dx.as<double>(), c.gn_mmin.as<double>(),
                                  c.gn_mmax.as<double>(), nm, mt, scr, flags, c.stream),
            "barrier_update");
-    int splits = gemv_n_splits(nd, nm);
-    HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)splits * nd), "alloc");
-    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, nm, ld, mt, c.part_n.as<double>(), splits, nullptr,
+    const GemvShape shape = gemv_shape(c);
+    HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * nd), "alloc");
+    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, ld, shape, mt, c.part_n.as<double>(), nullptr,
                          c.stream),
            "gemv_n");
-    HIPCHK(launch_misfit(c.part_n.as<double>(), splits, nd, c.gn_Wd.as<double>(),
+    HIPCHK(launch_misfit(c.part_n.as<double>(), shape.n_splits, nd, c.gn_Wd.as<double>(),
                          c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
            "misfit");
     // accept the trial model
@@ -659,6 +671,60 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m)
                           c.stream),
            "model");
     HIPCHK(hipStreamSynchronize(c.stream), "model");
+    return GRA_OK;
+}
+
+gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
+{
+    CTX_GUARD(ctx);
+    GemvShape &o = c.tune_override;
+    switch (param) {
+    case GRA_TUNE_N_ROWS:
+        if (value != 0 && value != 2 && value != 4 && value != 8)
+            return set_error(c, GRA_BAD_ARG, "rows per wave: 0, 2, 4 or 8");
+        o.n_rows = (int)value;
+        break;
+    case GRA_TUNE_N_UNROLL:
+        if (value < 0 || value > 2) return set_error(c, GRA_BAD_ARG, "unroll: 0, 1 or 2");
+        o.n_unroll = (int)value;
+        break;
+    case GRA_TUNE_N_SPLITS:
+        if (value < 0 || value > 256) return set_error(c, GRA_BAD_ARG, "splits: 0..256");
+        o.n_splits = (int)value;
+        break;
+    case GRA_TUNE_T_CHUNKS:
+        if (value < 0 || value > 1024) return set_error(c, GRA_BAD_ARG, "chunks: 0..1024");
+        o.t_chunks = (int)value;
+        break;
+    case GRA_TUNE_T_UNROLL:
+        if (value != 0 && value != 4 && value != 8)
+            return set_error(c, GRA_BAD_ARG, "row unroll: 0, 4 or 8");
+        o.t_unroll = (int)value;
+        break;
+    case GRA_TUNE_NONTEMPORAL:
+        if (value < -1 || value > 1) return set_error(c, GRA_BAD_ARG, "nontemporal: -1, 0, 1");
+        o.nontemporal = (int)value;
+        break;
+    default:
+        return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
+    }
+    return GRA_OK;
+}
+
+gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
+{
+    CTX_GUARD(ctx);
+    if (!value) return set_error(c, GRA_BAD_ARG, "null output");
+    GemvShape s = gemv_shape(c);
+    switch (param) {
+    case GRA_TUNE_N_ROWS: *value = s.n_rows; break;
+    case GRA_TUNE_N_UNROLL: *value = s.n_unroll; break;
+    case GRA_TUNE_N_SPLITS: *value = s.n_splits; break;
+    case GRA_TUNE_T_CHUNKS: *value = s.t_chunks; break;
+    case GRA_TUNE_T_UNROLL: *value = s.t_unroll; break;
+    case GRA_TUNE_NONTEMPORAL: *value = s.nontemporal; break;
+    default: return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
+    }
     return GRA_OK;
 }
 

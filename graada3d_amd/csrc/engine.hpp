@@ -59,6 +59,16 @@ struct CgScalars {
     int pad;
 };
 
+// Launch shape of the two G products (see linalg_kernels.hip).
+struct GemvShape {
+    int n_rows = 4;        // rows per wave, G x (2, 4, 8)
+    int n_unroll = 2;      // column steps per loop iteration (1, 2)
+    int n_splits = 1;      // column splits (partial sums summed by the consumer)
+    int t_chunks = 1;      // row chunks of G^T w
+    int t_unroll = 8;      // rows per loop iteration (4, 8)
+    int nontemporal = 1;   // non-temporal loads of G
+};
+
 // Kernel timing classes (gra_timing_get).
 enum TimingClass { kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeClasses = 4 };
 
@@ -90,6 +100,9 @@ struct Ctx {
     bool gn_ready = false;
     DevBuf gn_dobs, gn_Wd, gn_mmin, gn_mmax, gn_mref, gn_m, gn_mtrial, gn_P, gn_dx, gn_one;
 
+    // launch-shape overrides (0 = automatic; nontemporal -1 = automatic)
+    GemvShape tune_override{0, 0, 0, 0, 0, -1};
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -118,16 +131,15 @@ hipError_t launch_forward(const double *d_prisms, const double *d_rho, int64_t N
                           const double *d_pts, int64_t N_obs, uint32_t flag, double *d_out,
                           int *d_bad, hipStream_t st);
 
-// GEMV (linalg_kernels.hip).  gemv_n writes part[s*Nd + i] for s < splits.
-int gemv_n_splits(int64_t Nd, int64_t Nm);
-hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
-                         const double *x, double *part, int splits, const int *done,
-                         hipStream_t st);
-// gemv_t writes part[c*ld + j] for c < chunks.
-int gemv_t_chunks(int64_t Nd, int64_t Nm);
-hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
-                         const double *w, double *part, int chunks, const int *done,
-                         hipStream_t st);
+// GEMV (linalg_kernels.hip).  gemv_n writes part[s*Nd + i] for s < n_splits;
+// x must hold ld entries (zero beyond Nm).  gemv_t writes part[c*ld + j] for
+// c < t_chunks.
+GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm);
+GemvShape gemv_shape(const Ctx &c);   // defaults of the context's G + overrides
+hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, const GemvShape &s,
+                         const double *x, double *part, const int *done, hipStream_t st);
+hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, const GemvShape &s,
+                         const double *w, double *part, const int *done, hipStream_t st);
 // y[i] = scale_i * sum_s part[s*stride + i]  (scale may be null => 1)
 hipError_t launch_reduce_parts(const double *part, int nparts, int64_t stride, int64_t n,
                                const double *scale, double *y, hipStream_t st);

@@ -1,18 +1,22 @@
 // Dense G products on the row-major, zero-padded G (leading dimension ld).
 //
 // Both products are HBM-bound streams over G (2 flops per element), so the
-// kernels are shaped for bandwidth: 16-byte (FP64: 2 elements) vector loads
-// per lane, several independent row streams per wave for memory-level
-// parallelism, and a deterministic two-level reduction (fixed xor butterfly in
-// the wave, then the column splits / row chunks summed in index order by the
-// consumer), so every product is bit-for-bit repeatable.
+// kernels are shaped for bandwidth: every lane moves 16 bytes of G per load
+// whatever the storage type (2 FP64, 4 FP32 or 8 BF16 elements), several
+// independent row streams are in flight per wave, G is read with
+// non-temporal loads (it is streamed once per product; the vectors stay
+// cached), and the reductions are deterministic: a fixed xor butterfly in the
+// wave, then the column splits / row chunks summed in index order by the
+// consumer, so every product is bit-for-bit repeatable.  Accumulation is FP64
+// for every storage type.
 //
 //   y = G x   (Fwd::G_vec_mul dense branch, Fwd.cpp:145): a wave owns R rows
-//             of one column split; x is loaded once per column step and reused
-//             for the R rows.
-//   z = G^T w (Fwd::GT_vec_mul dense branch, Fwd.cpp:111): a lane owns a
-//             pair of adjacent columns and walks a chunk of rows; w[row] is a
-//             wave-uniform scalar.
+//             of one column split; each x chunk is loaded once for the R rows.
+//   z = G^T w (Fwd::GT_vec_mul dense branch, Fwd.cpp:111): a lane owns the
+//             16-byte column group it loads and walks a chunk of rows; w[row]
+//             is a wave-uniform scalar.
+// Launch shapes (R, unroll, splits, chunks, non-temporal) are tunable per
+// context (gra_tune_set); the defaults come from a sweep on MI355X.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -24,74 +28,111 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlk = 256;
-constexpr int kRowsPerWave = 4;
 
-// Load 2 consecutive elements of G as doubles.
-__device__ inline double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
-__device__ inline double2 ld2(const float *p)
-{
-    float2 f = *reinterpret_cast<const float2 *>(p);
-    return make_double2((double)f.x, (double)f.y);
-}
-__device__ inline double2 ld2(const __hip_bfloat16 *p)
-{
-    uint32_t u = *reinterpret_cast<const uint32_t *>(p);
-    float lo = __uint_as_float(u << 16), hi = __uint_as_float(u & 0xffff0000u);
-    return make_double2((double)lo, (double)hi);
-}
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { static constexpr int n = 2; };
+template <> struct Vec16<float> { static constexpr int n = 4; };
+template <> struct Vec16<__hip_bfloat16> { static constexpr int n = 8; };
 
-template <typename T>
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+
+// 16-byte load of G converted to n doubles.
+template <typename T, bool NT> struct Load16;
+
+template <bool NT> struct Load16<double, NT> {
+    __device__ static inline void run(const double *p, double *o)
+    {
+        v2f64 v;
+        if (NT) v = __builtin_nontemporal_load(reinterpret_cast<const v2f64 *>(p));
+        else v = *reinterpret_cast<const v2f64 *>(p);
+        o[0] = v.x; o[1] = v.y;
+    }
+};
+template <bool NT> struct Load16<float, NT> {
+    __device__ static inline void run(const float *p, double *o)
+    {
+        v4f32 v;
+        if (NT) v = __builtin_nontemporal_load(reinterpret_cast<const v4f32 *>(p));
+        else v = *reinterpret_cast<const v4f32 *>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    }
+};
+template <bool NT> struct Load16<__hip_bfloat16, NT> {
+    __device__ static inline void run(const __hip_bfloat16 *p, double *o)
+    {
+        v4u32 v;
+        if (NT) v = __builtin_nontemporal_load(reinterpret_cast<const v4u32 *>(p));
+        else v = *reinterpret_cast<const v4u32 *>(p);
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            o[2 * q] = (double)__uint_as_float(w[q] << 16);
+            o[2 * q + 1] = (double)__uint_as_float(w[q] & 0xffff0000u);
+        }
+    }
+};
+
+// ------------------------------------------------------------------ G x
+template <typename T, int R, int U, bool NT>
 __global__ void __launch_bounds__(kBlk)
 gemv_n_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t cols_per_split,
               const double *__restrict__ x, double *__restrict__ part, const int *done)
 {
     if (done && *done) return;
+    constexpr int V = Vec16<T>::n;            // elements per lane per load
+    constexpr int STEP = V * kWave;           // columns per wave step
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wave = (int64_t)blockIdx.x * (kBlk / kWave) + (threadIdx.x >> 6);
-    const int64_t row0 = wave * kRowsPerWave;
+    const int64_t row0 = wave * R;
     const int split = blockIdx.y;
     if (row0 >= Nd) return;
     int64_t c0 = (int64_t)split * cols_per_split;
     int64_t c1 = c0 + cols_per_split;
     if (c1 > ld) c1 = ld;
-    const T *rows[kRowsPerWave];
+    const T *rows[R];
 #pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) {
+    for (int r = 0; r < R; ++r) {
         int64_t row = row0 + r < Nd ? row0 + r : Nd - 1;   // clamp: duplicate reads, result dropped
         rows[r] = G + row * ld;
     }
-    double acc[kRowsPerWave];
+    double acc[R];
 #pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) acc[r] = 0.0;
-    int64_t c = c0 + 2 * lane;
-    for (; c + 2 * kWave < c1; c += 4 * kWave) {
-        double2 xa = *reinterpret_cast<const double2 *>(x + c);
-        double2 xb = *reinterpret_cast<const double2 *>(x + c + 2 * kWave);
-        double2 ga[kRowsPerWave], gb[kRowsPerWave];
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    int64_t c = c0 + (int64_t)V * lane;
+    for (; c + (U - 1) * STEP < c1; c += U * STEP) {
+        double xv[U][V];
+        double g[U][R][V];
 #pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r) {
-            ga[r] = ld2(rows[r] + c);
-            gb[r] = ld2(rows[r] + c + 2 * kWave);
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) Load16<T, NT>::run(rows[r] + c + u * STEP, g[u][r]);
+#pragma unroll
+            for (int q = 0; q < V; q += 2) {
+                double2 xx = *reinterpret_cast<const double2 *>(x + c + u * STEP + q);
+                xv[u][q] = xx.x;
+                xv[u][q + 1] = xx.y;
+            }
         }
 #pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r) {
-            acc[r] = fma(ga[r].x, xa.x, acc[r]);
-            acc[r] = fma(ga[r].y, xa.y, acc[r]);
-            acc[r] = fma(gb[r].x, xb.x, acc[r]);
-            acc[r] = fma(gb[r].y, xb.y, acc[r]);
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int q = 0; q < V; ++q) acc[r] = fma(g[u][r][q], xv[u][q], acc[r]);
+    }
+    for (; c < c1; c += STEP) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double g[V];
+            Load16<T, NT>::run(rows[r] + c, g);
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[r] = fma(g[q], x[c + q], acc[r]);
         }
     }
-    for (; c < c1; c += 2 * kWave) {
-        double2 xa = *reinterpret_cast<const double2 *>(x + c);
 #pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r) {
-            double2 g = ld2(rows[r] + c);
-            acc[r] = fma(g.x, xa.x, acc[r]);
-            acc[r] = fma(g.y, xa.y, acc[r]);
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) {
+    for (int r = 0; r < R; ++r) {
         double v = acc[r];
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -99,47 +140,54 @@ gemv_n_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t cols_per_
     }
     if (lane == 0) {
 #pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r)
+        for (int r = 0; r < R; ++r)
             if (row0 + r < Nd) part[(int64_t)split * Nd + row0 + r] = acc[r];
     }
 }
 
-constexpr int kColsPerLane = 2;
-constexpr int kColsPerBlock = kBlk * kColsPerLane;
-
-template <typename T>
+// ---------------------------------------------------------------- G^T w
+template <typename T, int UR, bool NT>
 __global__ void __launch_bounds__(kBlk)
 gemv_t_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t rows_per_chunk,
               const double *__restrict__ w, double *__restrict__ part, const int *done)
 {
     if (done && *done) return;
-    const int64_t c = (int64_t)blockIdx.x * kColsPerBlock + kColsPerLane * threadIdx.x;
+    constexpr int V = Vec16<T>::n;
+    const int64_t c = (int64_t)blockIdx.x * (kBlk * V) + (int64_t)V * threadIdx.x;
     const int chunk = blockIdx.y;
     int64_t r0 = (int64_t)chunk * rows_per_chunk;
     int64_t r1 = r0 + rows_per_chunk;
     if (r1 > Nd) r1 = Nd;
     if (c >= ld) return;
     const T *g = G + c;
-    double ax = 0.0, ay = 0.0;
+    double acc[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) acc[q] = 0.0;
     int64_t r = r0;
-    for (; r + 4 <= r1; r += 4) {
-        double2 g0 = ld2(g + (r + 0) * ld);
-        double2 g1 = ld2(g + (r + 1) * ld);
-        double2 g2 = ld2(g + (r + 2) * ld);
-        double2 g3 = ld2(g + (r + 3) * ld);
-        double w0 = w[r], w1 = w[r + 1], w2 = w[r + 2], w3 = w[r + 3];
-        ax = fma(g0.x, w0, ax); ay = fma(g0.y, w0, ay);
-        ax = fma(g1.x, w1, ax); ay = fma(g1.y, w1, ay);
-        ax = fma(g2.x, w2, ax); ay = fma(g2.y, w2, ay);
-        ax = fma(g3.x, w3, ax); ay = fma(g3.y, w3, ay);
+    for (; r + UR <= r1; r += UR) {
+        double gv[UR][V];
+        double wv[UR];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+            Load16<T, NT>::run(g + (r + u) * ld, gv[u]);
+            wv[u] = w[r + u];
+        }
+#pragma unroll
+        for (int u = 0; u < UR; ++u)
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[q] = fma(gv[u][q], wv[u], acc[q]);
     }
     for (; r < r1; ++r) {
-        double2 g0 = ld2(g + r * ld);
+        double gv[V];
+        Load16<T, NT>::run(g + r * ld, gv);
         double w0 = w[r];
-        ax = fma(g0.x, w0, ax);
-        ay = fma(g0.y, w0, ay);
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] = fma(gv[q], w0, acc[q]);
     }
-    *reinterpret_cast<double2 *>(part + (int64_t)chunk * ld + c) = make_double2(ax, ay);
+    double *out = part + (int64_t)chunk * ld + c;
+#pragma unroll
+    for (int q = 0; q < V; q += 2)
+        *reinterpret_cast<double2 *>(out + q) = make_double2(acc[q], acc[q + 1]);
 }
 
 __global__ void reduce_parts_kernel(const double *__restrict__ part, int nparts, int64_t stride,
@@ -172,80 +220,108 @@ __global__ void expand_kernel(const T *__restrict__ G, int64_t Nd, int64_t Nm, i
     out[idx] = v;
 }
 
-}  // namespace
-
-int gemv_n_splits(int64_t Nd, int64_t Nm)
+template <typename T, int R, int U, bool NT>
+void launch_n(const void *G, int64_t Nd, int64_t ld, int splits, const double *x, double *part,
+              const int *done, hipStream_t st)
 {
-    int64_t waves = (Nd + kRowsPerWave - 1) / kRowsPerWave;
-    int64_t want = 8192;
-    int64_t s = (want + waves - 1) / waves;
-    int64_t max_by_cols = Nm / (8 * kWave) > 0 ? Nm / (8 * kWave) : 1;
-    if (s > max_by_cols) s = max_by_cols;
-    if (s > 64) s = 64;
-    if (s < 1) s = 1;
-    return (int)s;
-}
-
-int gemv_t_chunks(int64_t Nd, int64_t Nm)
-{
-    int64_t bx = (Nm + kColsPerBlock - 1) / kColsPerBlock;
-    int64_t want = 4096;   // blocks
-    int64_t ch = (want + bx - 1) / bx;
-    int64_t max_by_rows = Nd / 16 > 0 ? Nd / 16 : 1;
-    if (ch > max_by_rows) ch = max_by_rows;
-    if (ch > 256) ch = 256;
-    if (ch < 1) ch = 1;
-    return (int)ch;
-}
-
-hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
-                         const double *x, double *part, int splits, const int *done,
-                         hipStream_t st)
-{
-    (void)Nm;
-    int64_t waves = (Nd + kRowsPerWave - 1) / kRowsPerWave;
+    constexpr int STEP = Vec16<T>::n * kWave;
+    int64_t waves = (Nd + R - 1) / R;
     int64_t blocks = (waves + (kBlk / kWave) - 1) / (kBlk / kWave);
     int64_t cps = (ld + splits - 1) / splits;
-    cps = (cps + 2 * kWave - 1) / (2 * kWave) * (2 * kWave);   // keep 16-B lane alignment
+    cps = (cps + STEP - 1) / STEP * STEP;   // keep 16-B lane alignment
     dim3 grid((unsigned)blocks, (unsigned)splits), block(kBlk);
+    hipLaunchKernelGGL((gemv_n_kernel<T, R, U, NT>), grid, block, 0, st, (const T *)G, Nd, ld, cps,
+                       x, part, done);
+}
+
+template <typename T, int R, int U>
+void launch_n_nt(const void *G, int64_t Nd, int64_t ld, int splits, bool nt, const double *x,
+                 double *part, const int *done, hipStream_t st)
+{
+    if (nt) launch_n<T, R, U, true>(G, Nd, ld, splits, x, part, done, st);
+    else launch_n<T, R, U, false>(G, Nd, ld, splits, x, part, done, st);
+}
+
+template <typename T>
+void launch_n_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, const double *x,
+                    double *part, const int *done, hipStream_t st)
+{
+    const bool nt = s.nontemporal != 0;
+    if (s.n_rows == 2) {
+        if (s.n_unroll >= 2) launch_n_nt<T, 2, 2>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+        else launch_n_nt<T, 2, 1>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+    } else if (s.n_rows >= 8) {
+        if (s.n_unroll >= 2) launch_n_nt<T, 8, 2>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+        else launch_n_nt<T, 8, 1>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+    } else {
+        if (s.n_unroll >= 2) launch_n_nt<T, 4, 2>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+        else launch_n_nt<T, 4, 1>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+    }
+}
+
+template <typename T>
+void launch_t_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, const double *w,
+                    double *part, const int *done, hipStream_t st)
+{
+    constexpr int V = Vec16<T>::n;
+    int64_t bx = (ld + kBlk * V - 1) / (kBlk * V);
+    int64_t rpc = (Nd + s.t_chunks - 1) / s.t_chunks;
+    dim3 grid((unsigned)bx, (unsigned)s.t_chunks), block(kBlk);
+    const bool nt = s.nontemporal != 0;
+#define GRA_T(UR, NTV)                                                                        \
+    hipLaunchKernelGGL((gemv_t_kernel<T, UR, NTV>), grid, block, 0, st, (const T *)G, Nd, ld, \
+                       rpc, w, part, done)
+    if (s.t_unroll >= 8) {
+        if (nt) GRA_T(8, true); else GRA_T(8, false);
+    } else {
+        if (nt) GRA_T(4, true); else GRA_T(4, false);
+    }
+#undef GRA_T
+}
+
+}  // namespace
+
+// Default launch shapes.  Targets: >= ~8k waves in flight for the N product;
+// >= ~4k blocks for the T product, with as few row chunks as that allows.
+GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm)
+{
+    GemvShape s;
+    s.n_rows = 4;
+    s.n_unroll = 2;
+    s.t_unroll = 8;
+    s.nontemporal = 1;
+    const int V = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
+    int64_t waves = (Nd + s.n_rows - 1) / s.n_rows;
+    int64_t sp = (8192 + waves - 1) / waves;
+    int64_t max_by_cols = Nm / (4 * V * kWave) > 0 ? Nm / (4 * V * kWave) : 1;
+    sp = sp > max_by_cols ? max_by_cols : sp;
+    s.n_splits = (int)(sp < 1 ? 1 : (sp > 64 ? 64 : sp));
+    int64_t bx = (Nm + kBlk * V - 1) / (kBlk * V);
+    int64_t ch = (4096 + bx - 1) / bx;
+    int64_t max_by_rows = Nd / 32 > 0 ? Nd / 32 : 1;
+    ch = ch > max_by_rows ? max_by_rows : ch;
+    s.t_chunks = (int)(ch < 1 ? 1 : (ch > 256 ? 256 : ch));
+    return s;
+}
+
+hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, const GemvShape &s,
+                         const double *x, double *part, const int *done, hipStream_t st)
+{
     switch (dt) {
-    case GRA_F64:
-        hipLaunchKernelGGL(gemv_n_kernel<double>, grid, block, 0, st, (const double *)G, Nd, ld,
-                           cps, x, part, done);
-        break;
-    case GRA_F32:
-        hipLaunchKernelGGL(gemv_n_kernel<float>, grid, block, 0, st, (const float *)G, Nd, ld, cps,
-                           x, part, done);
-        break;
-    default:
-        hipLaunchKernelGGL(gemv_n_kernel<__hip_bfloat16>, grid, block, 0, st,
-                           (const __hip_bfloat16 *)G, Nd, ld, cps, x, part, done);
-        break;
+    case GRA_F64: launch_n_shape<double>(G, Nd, ld, s, x, part, done, st); break;
+    case GRA_F32: launch_n_shape<float>(G, Nd, ld, s, x, part, done, st); break;
+    default: launch_n_shape<__hip_bfloat16>(G, Nd, ld, s, x, part, done, st); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
-                         const double *w, double *part, int chunks, const int *done,
-                         hipStream_t st)
+hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, const GemvShape &s,
+                         const double *w, double *part, const int *done, hipStream_t st)
 {
-    (void)Nm;
-    int64_t bx = (ld + kColsPerBlock - 1) / kColsPerBlock;
-    int64_t rpc = (Nd + chunks - 1) / chunks;
-    dim3 grid((unsigned)bx, (unsigned)chunks), block(kBlk);
     switch (dt) {
-    case GRA_F64:
-        hipLaunchKernelGGL(gemv_t_kernel<double>, grid, block, 0, st, (const double *)G, Nd, ld,
-                           rpc, w, part, done);
-        break;
-    case GRA_F32:
-        hipLaunchKernelGGL(gemv_t_kernel<float>, grid, block, 0, st, (const float *)G, Nd, ld, rpc,
-                           w, part, done);
-        break;
-    default:
-        hipLaunchKernelGGL(gemv_t_kernel<__hip_bfloat16>, grid, block, 0, st,
-                           (const __hip_bfloat16 *)G, Nd, ld, rpc, w, part, done);
-        break;
+    case GRA_F64: launch_t_shape<double>(G, Nd, ld, s, w, part, done, st); break;
+    case GRA_F32: launch_t_shape<float>(G, Nd, ld, s, w, part, done, st); break;
+    default: launch_t_shape<__hip_bfloat16>(G, Nd, ld, s, w, part, done, st); break;
     }
     return hipGetLastError();
 }

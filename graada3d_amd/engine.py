@@ -104,6 +104,21 @@ class Context:
         return ms.value, n.value
 
 
+    def tune(self, **kw):
+        """Override G-product launch shapes (n_rows, n_unroll, n_splits,
+        t_chunks, t_unroll, nontemporal); 0 (or -1 for nontemporal) = auto."""
+        for k, v in kw.items():
+            self.check(self.L.gra_tune_set(self.h, _lib.TUNE[k], int(v)), f"tune {k}")
+
+    def tuning(self):
+        out = {}
+        for k, p in _lib.TUNE.items():
+            v = ctypes.c_int64()
+            self.check(self.L.gra_tune_get(self.h, p, ctypes.byref(v)), "tune_get")
+            out[k] = v.value
+        return out
+
+
 TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER = 0, 1, 2, 3
 
 

@@ -193,6 +193,24 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
                        double *misfit, double *cg_err, int *cg_iterations);
 gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
 
+/* --------------------------------------------------------------- tuning */
+/*
+ * Launch shapes of the G products (no counterpart in the reference, whose
+ * Eigen GEMV has none): used by the autotuner sweep.  0 restores the
+ * automatic choice for the context's current G (nontemporal: -1).  Values
+ * outside the listed sets are rejected.  Results do not depend on these
+ * settings except through the (deterministic) summation order of splits and
+ * chunks.
+ */
+#define GRA_TUNE_N_ROWS 0       /* rows per wave of G*x: 2, 4, 8           */
+#define GRA_TUNE_N_UNROLL 1     /* column steps per iteration: 1, 2        */
+#define GRA_TUNE_N_SPLITS 2     /* column splits of G*x: 1..256            */
+#define GRA_TUNE_T_CHUNKS 3     /* row chunks of G^T*w: 1..1024            */
+#define GRA_TUNE_T_UNROLL 4     /* rows per iteration of G^T*w: 4, 8       */
+#define GRA_TUNE_NONTEMPORAL 5  /* non-temporal loads of G: 0, 1           */
+gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
+gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
+
 /* ---------------------------------------------------------- measurement */
 /*
  * Average device time (ms, from GPU events on the context's stream) of the
