@@ -1,20 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the engine's hot path on one GPU.
 
-Workload (default): the L2 Gauss-Newton inversion of the 40x40x20-cell,
-50x50-point gz+Tzz survey -- dense FP64 G of 5,000 x 32,000 resident in HBM,
-Ws/Wx/Wy/Wz regularisation with (z+50)^-1.5 depth weighting, lambda_k =
-100*0.5^k, 50 CG iterations per outer iteration.  One step = one
-Gauss-Newton outer iteration on the GPU (barrier P, solve_cg with 50 CGLS
-iterations = 100 passes over G, barrier model update, misfit).  With the
-default --steps 10 the timed region is the whole 10 x 50 inversion.
+Workloads
+* inversion_small (default): the L2 Gauss-Newton inversion of the
+  40x40x20-cell, 50x50-point gz+Tzz survey -- dense G of 5,000 x 32,000
+  resident in HBM, Ws/Wx/Wy/Wz regularisation with (z+50)^-1.5 depth weighting,
+  lambda_k = 100*0.5^k, 50 CG iterations per outer iteration; default 10
+  steps = the whole inversion.
+* tensor_large: all nine components of the 64x64x32-cell model (62.5 m cells)
+  on a 100x100 grid at z = -50 m -- G of 90,000 x 131,072 (94.4 GB in FP64) --
+  L1 (IRLS, eps^2 = 1e-3) Gauss-Newton, 20 CG iterations per outer iteration.
 
-Prints one JSON line: CG iterations per second (value), G-assembly entries per
-second, the GEMV roofline (HIP events on the engine's stream) and a
-single-thread CPU baseline of the same solve_cg on the same G (the oracle, as
-the reference runs it: one core, sequential GEMV).
+One step = one Gauss-Newton outer iteration on the GPU (barrier P, solve_cg =
+CGLS with two passes over G per iteration, barrier model update, misfit, IRLS
+weights).  `value` is CG iterations per second of the plain FP64 path.  The
+same steps are then run with G stored in FP32 and BF16 (FP64 accumulation)
+and reported as speed-up and error against the FP64 run (final model and
+misfit).  G assembly throughput (entries/s) is measured for every storage
+type.  The GEMV roofline uses HIP events on the engine's stream; the CPU
+baseline is the single-threaded C oracle on a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--workload inversion_small|tensor_large] [--steps K] [--warmup W]
+                    [--stores f64,f32,bf16] [--no-cpu] [--gpus 1]
 """
 from __future__ import annotations
 
@@ -31,13 +38,144 @@ sys.path.insert(0, ROOT)
 
 METRIC = "G entries/s, CG iter/s (mixed vs FP64 plain) at stated error; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+BYTES = {"f64": 8, "f32": 4, "bf16": 2}
+
+
+def build_problem(name):
+    from graada3d_amd import mesh as M
+    from graada3d_amd.operators import depth_weights_offset, weighting_operators
+    from graada3d_amd.workloads import forward_problem_small, tensor_problem_large
+    if name == "inversion_small":
+        fp = forward_problem_small()
+        shape, cg_maxit, lp, eps2, z0 = (40, 40, 20), 50, 2, 0.0, 50.0
+        lam0 = 100.0
+    else:
+        fp = tensor_problem_large()
+        shape, cg_maxit, lp, eps2, z0 = (64, 64, 32), 20, 1, 1e-3, 50.0
+        lam0 = 100.0
+    Z = depth_weights_offset(fp.prisms, z0, 3.0)
+    W = weighting_operators(fp.prisms, *shape, Z, a_s=1e-3)
+    nm = fp.prisms.shape[0]
+    return dict(fp=fp, W=W, cg_maxit=cg_maxit, lp=lp, eps2=eps2,
+                lambdas=[lam0 * 0.5 ** k for k in range(64)],
+                m_min=np.full(nm, -1500.0), m_max=np.full(nm, 1500.0), mref=np.zeros(nm),
+                m_ini=np.zeros(nm))
+
+
+def run(args):
+    from graada3d_amd.engine import (TIME_ASSEMBLE, TIME_GEMV_N, TIME_GEMV_T, Context,
+                                     GaussNewtonInversion)
+    from graada3d_amd.workloads import add_noise
+
+    pb = build_problem(args.workload)
+    fp = pb["fp"]
+    ctx = Context(0)
+    inv = GaussNewtonInversion(fp.prisms, fp.points, fp.flag, ctx=ctx)
+    nd, nm = inv.Nd, inv.Nm
+    ncomp = nd // fp.points.shape[0]
+    print(f"[bench] {args.workload}: G {nd} x {nm}", file=sys.stderr, flush=True)
+
+    # synthetic data from the FP64 G
+    inv.compute_G("f64")
+    d_clean = inv.G_vec_mul(fp.rho)
+    dobs, sigma = add_noise(d_clean, fp.points.shape[0], ncomp, fp.seed)
+    Wd = np.repeat(1.0 / sigma, fp.points.shape[0])
+
+    results = {}
+    for store in args.stores:
+        ctx.timing(True)
+        ctx.timing_reset()
+        reps = 2 if args.workload == "inversion_small" else 1
+        for _ in range(reps):
+            inv.compute_G(store)
+        asm_ms, _ = ctx.timing_get(TIME_ASSEMBLE)
+        ctx.timing(False)
+        inv.set_regularization(*pb["W"])
+
+        def setup():
+            inv.gn_setup(dobs, Wd, pb["m_min"], pb["m_max"], pb["mref"], pb["m_ini"], pb["lp"],
+                         pb["eps2"])
+
+        setup()
+        for k in range(args.warmup):
+            inv.gn_step(pb["lambdas"][k], 0.0, pb["cg_maxit"])
+        setup()
+        ctx.timing(True)
+        ctx.timing_reset()
+        ctx.sync()
+        t0 = time.perf_counter()
+        its, mis = 0, []
+        for k in range(args.steps):
+            m_, e_, i_ = inv.gn_step(pb["lambdas"][k], 0.0, pb["cg_maxit"])
+            its += i_
+            mis.append(m_)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        gn_ms, gn_c = ctx.timing_get(TIME_GEMV_N)
+        gt_ms, gt_c = ctx.timing_get(TIME_GEMV_T)
+        ctx.timing(False)
+        gbytes = float(nd) * nm * BYTES[store]
+        results[store] = dict(
+            iter_s=its / el, ms_per_step=el / args.steps * 1e3, cg_iters=its,
+            misfit=mis[-1], model=inv.gn_model(),
+            asm_ms=asm_ms, entries_per_s=nd * nm / (asm_ms * 1e-3),
+            gemv_n=dict(avg_ms=gn_ms, launches=gn_c, GBps=gbytes / (gn_ms * 1e-3) / 1e9),
+            gemv_t=dict(avg_ms=gt_ms, launches=gt_c, GBps=gbytes / (gt_ms * 1e-3) / 1e9),
+            tuning=ctx.tuning())
+        print(f"[bench] {store}: {results[store]['iter_s']:.1f} CG iter/s, assembly "
+              f"{results[store]['entries_per_s']:.3e} entries/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, inv, pb, dobs, Wd)
+    ctx.close()
+    return pb, nd, nm, results, cpu
+
+
+def cpu_baseline(args, inv, pb, dobs, Wd):
+    import oracle
+    if args.workload == "inversion_small":
+        inv.compute_G("f64")
+        G = inv.get_G()
+        P = oracle.barrier_P(pb["m_ini"], pb["m_min"], pb["m_max"])
+        c0 = time.perf_counter()
+        oracle.solve_cg(G, 0.0, args.cpu_iters, pb["lambdas"][0], pb["m_ini"], pb["mref"], Wd, dobs,
+                        P, *pb["W"])
+        c1 = time.perf_counter()
+        return {"value": args.cpu_iters / (c1 - c0), "unit": "CG iter/s", "cores": 1,
+                "kind": "port",
+                "sample": f"one solve_cg of {args.cpu_iters} CGLS iterations (plus its setup "
+                          f"products) on the same 5000x32000 FP64 G, single-threaded C oracle "
+                          f"(sequential GEMV, as the reference's Eigen GEMV runs); {c1 - c0:.1f} s; "
+                          f"host has {os.cpu_count()} logical CPUs"}
+    # tensor_large: G (94 GB) is not copied to the host; the per-iteration CPU
+    # cost is extrapolated from single-threaded oracle products on 2,000 rows.
+    fp = pb["fp"]
+    rows = 2000
+    rng = np.random.default_rng(0)
+    Gs = rng.standard_normal((rows, fp.prisms.shape[0]))
+    x = rng.standard_normal(fp.prisms.shape[0])
+    y = rng.standard_normal(rows)
+    c0 = time.perf_counter()
+    for _ in range(3):
+        oracle.gemv(Gs, x)
+        oracle.gemv(Gs, y, trans=True)
+    c1 = time.perf_counter()
+    per_iter = (c1 - c0) / 3 * (inv.Nd / rows)
+    return {"value": 1.0 / per_iter, "unit": "CG iter/s", "cores": 1, "kind": "port",
+            "sample": f"single-threaded C oracle G x and G^T w on {rows} rows x {inv.Nm} columns "
+                      f"(3 repetitions, {c1 - c0:.1f} s), extrapolated to {inv.Nd} rows; host has "
+                      f"{os.cpu_count()} logical CPUs"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="inversion_small",
+                    choices=["inversion_small", "tensor_large"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--stores", default="f64,f32,bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=40)
     args = ap.parse_args()
@@ -45,102 +183,71 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine's "
               "multi-GPU sharding is a design note, not part of this build.", file=sys.stderr)
         sys.exit(2)
+    small = args.workload == "inversion_small"
+    if args.steps is None:
+        args.steps = 10 if small else 3
+    if args.warmup is None:
+        args.warmup = 2 if small else 1
+    args.stores = [s for s in args.stores.split(",") if s]
+    if "f64" not in args.stores:
+        args.stores = ["f64"] + args.stores
+    args.stores = ["f64"] + [s for s in args.stores if s != "f64"]
 
-    from graada3d_amd.engine import (TIME_ASSEMBLE, TIME_GEMV_N, TIME_GEMV_T, Context,
-                                     GaussNewtonInversion)
-    from graada3d_amd.workloads import forward_problem_small, inversion_small
-
-    ctx = Context(0)
-    fp = forward_problem_small()
-    inv = GaussNewtonInversion(fp.prisms, fp.points, fp.flag, ctx=ctx)
-    nd, nm = inv.Nd, inv.Nm
-
-    # G assembly (FP64 face/edge kernel), timed with GPU events
-    inv.compute_G("f64")
-    ctx.timing(True)
-    ctx.timing_reset()
-    for _ in range(3):
-        inv.compute_G("f64")
-    asm_ms, _ = ctx.timing_get(TIME_ASSEMBLE)
-    ctx.timing(False)
-    d_clean = inv.G_vec_mul(fp.rho)
-    ip = inversion_small(d_clean)
-    inv.set_regularization(ip.Ws, ip.Wx, ip.Wy, ip.Wz)
-
-    def setup():
-        inv.gn_setup(ip.dobs, ip.Wd, ip.m_min, ip.m_max, ip.mref, ip.m_ini)
-
-    setup()
-    for k in range(args.warmup):
-        inv.gn_step(ip.lambdas[k % len(ip.lambdas)], ip.cg_tol, ip.cg_maxit)
-    setup()
-    ctx.timing(True)
-    ctx.timing_reset()
-    ctx.sync()
-    t0 = time.perf_counter()
-    total_its = 0
-    misfits = []
-    for k in range(args.steps):
-        lam = ip.lambdas[k % len(ip.lambdas)]
-        mis, err, its = inv.gn_step(lam, ip.cg_tol, ip.cg_maxit)
-        total_its += its
-        misfits.append(mis)
-    ctx.sync()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    gn_ms, gn_cnt = ctx.timing_get(TIME_GEMV_N)
-    gt_ms, gt_cnt = ctx.timing_get(TIME_GEMV_T)
-    ctx.timing(False)
-
-    g_bytes = float(nd) * nm * 8.0
-    kern = {
-        "gemv_n": {"avg_ms": gn_ms, "launches": gn_cnt, "GBps": g_bytes / (gn_ms * 1e-3) / 1e9},
-        "gemv_t": {"avg_ms": gt_ms, "launches": gt_cnt, "GBps": g_bytes / (gt_ms * 1e-3) / 1e9},
-    }
-    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches"])
-    achieved = kern[dom]["GBps"]
-
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        G = inv.get_G()
-        P = oracle.barrier_P(ip.m_ini, ip.m_min, ip.m_max)
-        c0 = time.perf_counter()
-        oracle.solve_cg(G, 0.0, args.cpu_iters, ip.lambdas[0], ip.m_ini, ip.mref, ip.Wd, ip.dobs,
-                        P, ip.Ws, ip.Wx, ip.Wy, ip.Wz)
-        c1 = time.perf_counter()
-        cpu = {"value": args.cpu_iters / (c1 - c0), "unit": "CG iter/s", "cores": 1,
-               "kind": "port",
-               "sample": f"one solve_cg of {args.cpu_iters} CGLS iterations (plus its setup "
-                         f"products) on the same 5000x32000 FP64 G, single-threaded C oracle "
-                         f"(sequential GEMV, as the reference's Eigen GEMV runs); "
-                         f"{c1 - c0:.1f} s; host has {os.cpu_count()} logical CPUs"}
-        del G
-
+    pb, nd, nm, res, cpu = run(args)
+    ref = res["f64"]
+    mixed = {}
+    for s, r in res.items():
+        if s == "f64":
+            continue
+        mixed[s] = {"iter_s": r["iter_s"], "speedup_vs_f64": r["iter_s"] / ref["iter_s"],
+                    "model_rel_err": float(np.linalg.norm(r["model"] - ref["model"])
+                                           / np.linalg.norm(ref["model"])),
+                    "misfit_rel_err": abs(r["misfit"] - ref["misfit"]) / ref["misfit"],
+                    "gemv_GBps": {"n": r["gemv_n"]["GBps"], "t": r["gemv_t"]["GBps"]},
+                    "g_entries_per_s": r["entries_per_s"]}
+    dom = "gemv_t" if ref["gemv_t"]["avg_ms"] >= ref["gemv_n"]["avg_ms"] else "gemv_n"
+    achieved = ref[dom]["GBps"]
+    # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs
+    # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
+    # measured for the 5000 x 32000 FP64 G only.
+    traffic = None
+    if small:
+        traffic = {"gemv_t": 1280.6e6 + 16.9e6, "gemv_n": 1282.1e6 + 0.3e6}[dom] / 1e9
     out = {
         "metric": METRIC,
-        "value": total_its / elapsed,
+        "value": ref["iter_s"],
         "unit": "CG iter/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": ref["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "inversion_small: L2 Gauss-Newton on the 40x40x20 mesh, "
-                               "50x50 gz+Tzz points, dense FP64 G 5000x32000 resident in HBM, "
-                               "50 CGLS iterations per step (plain FP64, 2 passes over G/iter)",
-                   "Nd": nd, "Nm": nm, "cg_iters_per_step": ip.cg_maxit,
-                   "lambda_schedule": "100*0.5^k"},
+        "config": {
+            "workload": ("inversion_small: L2 Gauss-Newton on the 40x40x20 mesh, 50x50 gz+Tzz "
+                         "points, dense FP64 G 5000x32000 resident in HBM, 50 CGLS iterations per "
+                         "step (plain FP64, 2 passes over G per iteration)") if small else
+                        ("tensor_large: L1 IRLS Gauss-Newton on the 64x64x32 mesh, 100x100 points "
+                         "at z=-50 m, all 9 components, dense FP64 G 90000x131072 (94.4 GB) "
+                         "resident in HBM, 20 CGLS iterations per step"),
+            "Nd": nd, "Nm": nm, "cg_iters_per_step": pb["cg_maxit"],
+            "lambda_schedule": "100*0.5^k"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-        "kernels": kern,
-        "g_assembly": {"entries_per_s": nd * nm / (asm_ms * 1e-3), "avg_ms": asm_ms,
-                       "entries": nd * nm, "kernel": "face/edge FP64, gz+Tzz"},
-        "final_misfit": misfits[-1] if misfits else None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "traffic_unit": "GB per launch"},
+        "kernels": {"gemv_n": ref["gemv_n"], "gemv_t": ref["gemv_t"]},
+        "g_assembly": {"entries_per_s": ref["entries_per_s"], "avg_ms": ref["asm_ms"],
+                       "entries": nd * nm, "store": "f64",
+                       "kernel": "face/edge FP64 evaluation, " + ("gz+Tzz" if small else "9 comps")},
+        "final_misfit": ref["misfit"],
+        "mixed": mixed,
+        "mixed_error_definition": "||m - m_f64|| / ||m_f64|| of the final model and |misfit - "
+                                  "misfit_f64| / misfit_f64 after the same steps, against the "
+                                  "plain FP64 GPU run (itself checked against the CPU oracle)",
+        "tuning": ref["tuning"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

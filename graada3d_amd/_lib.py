@@ -56,7 +56,7 @@ class GraCgArgs(ctypes.Structure):
 
 class GraGnSetup(ctypes.Structure):
     _fields_ = [("dobs", c_dp), ("Wd", c_dp), ("m_min", c_dp), ("m_max", c_dp),
-                ("mref", c_dp), ("m_ini", c_dp)]
+                ("mref", c_dp), ("m_ini", c_dp), ("lp", ctypes.c_int), ("eps2", ctypes.c_double)]
 
 
 _lib = None

@@ -387,12 +387,15 @@ __global__ void __launch_bounds__(kB) k_misfit_final(const double *red, int64_t 
 }
 
 __global__ void k_irls(const int64_t *rp, const int32_t *ci, const double *v, int64_t n,
-                       const double *m, double e, double eps2, double *WL)
+                       const double *m, const double *mref, double e, double eps2, double *WL)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double x = 0.0;
-    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) x += v[k] * m[ci[k]];
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+        int64_t j = ci[k];
+        x += v[k] * (mref ? m[j] - mref[j] : m[j]);
+    }
     WL[i] = pow(x * x + eps2, e);
 }
 
@@ -590,10 +593,12 @@ hipError_t launch_misfit(const double *part, int splits, int64_t Nd, const doubl
 }
 
 hipError_t launch_irls(const int64_t *rp, const int32_t *ci, const double *v, int64_t n,
-                       const double *m, int p, double eps2, double *WL, hipStream_t st)
+                       const double *m, const double *mref, int p, double eps2, double *WL,
+                       hipStream_t st)
 {
     double ex = -(2 - p) / 4.0;
-    hipLaunchKernelGGL(k_irls, dim3(nblocks(n)), dim3(kB), 0, st, rp, ci, v, n, m, ex, eps2, WL);
+    hipLaunchKernelGGL(k_irls, dim3(nblocks(n)), dim3(kB), 0, st, rp, ci, v, n, m, mref, ex, eps2,
+                       WL);
     return hipGetLastError();
 }
 

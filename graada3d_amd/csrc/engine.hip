@@ -568,8 +568,8 @@ gra_status gra_irls_weights(gra_ctx *ctx, const gra_csr *W, const double *m, int
     HIPCHK(put(v, W->val, (size_t)W->nnz, c.stream), "v");
     HIPCHK(put(dm, m, (size_t)n, c.stream), "m");
     HIPCHK(dout.reserve(sizeof(double) * (size_t)n), "alloc");
-    HIPCHK(launch_irls(rp.as<int64_t>(), ci.as<int32_t>(), v.as<double>(), n, dm.as<double>(), p,
-                       eps2, dout.as<double>(), c.stream),
+    HIPCHK(launch_irls(rp.as<int64_t>(), ci.as<int32_t>(), v.as<double>(), n, dm.as<double>(),
+                       nullptr, p, eps2, dout.as<double>(), c.stream),
            "irls");
     HIPCHK(hipMemcpyAsync(WL, dout.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c.stream),
            "WL");
@@ -603,6 +603,10 @@ gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
     HIPCHK(c.gn_dx.reserve(sizeof(double) * (size_t)nm), "alloc");
     HIPCHK(c.gn_one.reserve(sizeof(double) * 2 * (size_t)nm + sizeof(int) * (size_t)(nm + 1) + 64),
            "alloc");
+    c.gn_lp = s->lp;
+    c.gn_eps2 = s->eps2;
+    c.gn_have_wl = false;
+    if (s->lp != 2) HIPCHK(c.gn_WL.reserve(sizeof(double) * 4 * (size_t)nm), "alloc WL");
     HIPCHK(hipStreamSynchronize(c.stream), "gn setup");
     c.gn_ready = true;
     return GRA_OK;
@@ -635,6 +639,13 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
     a.quirk_mode = 1;
     CgDiag dg{c.gn_Wd.as<double>(), c.gn_P.as<double>(), nullptr, nullptr, nullptr, nullptr,
               nullptr, nullptr, nullptr, nullptr};
+    if (c.gn_have_wl) {
+        double *wl = c.gn_WL.as<double>();
+        dg.WLs = wl;
+        dg.WLx = wl + nm;
+        dg.WLy = wl + 2 * nm;
+        dg.WLz = wl + 3 * nm;
+    }
     double err = NAN;
     int its = 0;
     gra_status s = cgls_solve(c, a, m, c.gn_mref.as<double>(), c.gn_dobs.as<double>(), dg,
@@ -654,6 +665,17 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
     // accept the trial model
     HIPCHK(hipMemcpyAsync(m, mt, sizeof(double) * (size_t)nm, hipMemcpyDeviceToDevice, c.stream),
            "accept");
+    // IRLS weights for the next step from the accepted model (GN.cpp:962-1022)
+    if (c.gn_lp != 2) {
+        double *wl = c.gn_WL.as<double>();
+        DevCsr *W[4] = {&c.Ws, &c.Wx, &c.Wy, &c.Wz};
+        for (int q = 0; q < 4; ++q)
+            HIPCHK(launch_irls(W[q]->rp.as<int64_t>(), W[q]->ci.as<int32_t>(), W[q]->v.as<double>(),
+                               nm, m, q == 0 ? c.gn_mref.as<double>() : nullptr, c.gn_lp, c.gn_eps2,
+                               wl + q * nm, c.stream),
+                   "irls");
+        c.gn_have_wl = true;
+    }
     double hm = NAN;
     HIPCHK(hipMemcpyAsync(&hm, dmis, sizeof(double), hipMemcpyDeviceToHost, c.stream), "misfit");
     HIPCHK(hipStreamSynchronize(c.stream), "gn step");

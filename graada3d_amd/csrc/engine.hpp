@@ -99,6 +99,10 @@ struct Ctx {
     // resident Gauss-Newton state
     bool gn_ready = false;
     DevBuf gn_dobs, gn_Wd, gn_mmin, gn_mmax, gn_mref, gn_m, gn_mtrial, gn_P, gn_dx, gn_one;
+    DevBuf gn_WL;              // 4 x Nm: WL_s, WL_x, WL_y, WL_z (IRLS)
+    int gn_lp = 2;
+    double gn_eps2 = 0.0;
+    bool gn_have_wl = false;
 
     // launch-shape overrides (0 = automatic; nontemporal -1 = automatic)
     GemvShape tune_override{0, 0, 0, 0, 0, -1};
@@ -164,8 +168,10 @@ hipError_t launch_barrier_update(const double *m, const double *dx, const double
                                  int *flags, hipStream_t st);
 hipError_t launch_misfit(const double *part, int splits, int64_t Nd, const double *Wd,
                          const double *dobs, double *red, double *out, hipStream_t st);
+// WL_i = ((W (m - mref))_i^2 + eps2)^(-(2-p)/4); mref may be null.
 hipError_t launch_irls(const int64_t *rp, const int32_t *ci, const double *v, int64_t n,
-                       const double *m, int p, double eps2, double *WL, hipStream_t st);
+                       const double *m, const double *mref, int p, double eps2, double *WL,
+                       hipStream_t st);
 
 // Host helpers
 gra_status set_error(Ctx &c, gra_status s, const std::string &msg);

@@ -281,23 +281,25 @@ void launch_t_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, c
 
 }  // namespace
 
-// Default launch shapes.  Targets: >= ~8k waves in flight for the N product;
-// >= ~4k blocks for the T product, with as few row chunks as that allows.
+// Default launch shapes, from a sweep on MI355X (5,000 x 32,000 G in each
+// storage type): G x is fastest with 2 / 4 / 8 rows per wave for FP64 / FP32 /
+// BF16 and ~8k waves in flight; G^T w with long row chunks, ~256 blocks
+// (one per CU) in total; non-temporal loads help both.
 GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm)
 {
     GemvShape s;
-    s.n_rows = 4;
+    const int V = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
+    s.n_rows = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
     s.n_unroll = 2;
     s.t_unroll = 8;
     s.nontemporal = 1;
-    const int V = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
     int64_t waves = (Nd + s.n_rows - 1) / s.n_rows;
     int64_t sp = (8192 + waves - 1) / waves;
     int64_t max_by_cols = Nm / (4 * V * kWave) > 0 ? Nm / (4 * V * kWave) : 1;
     sp = sp > max_by_cols ? max_by_cols : sp;
     s.n_splits = (int)(sp < 1 ? 1 : (sp > 64 ? 64 : sp));
     int64_t bx = (Nm + kBlk * V - 1) / (kBlk * V);
-    int64_t ch = (4096 + bx - 1) / bx;
+    int64_t ch = (256 + bx - 1) / bx;
     int64_t max_by_rows = Nd / 32 > 0 ? Nd / 32 : 1;
     ch = ch > max_by_rows ? max_by_rows : ch;
     s.t_chunks = (int)(ch < 1 ? 1 : (ch > 256 ? 256 : ch));

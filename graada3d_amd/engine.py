@@ -269,11 +269,11 @@ class GaussNewtonInversion(Fwd):
         return out
 
     # resident Gauss-Newton loop (model and CG state stay on the GPU)
-    def gn_setup(self, dobs, Wd, m_min, m_max, mref, m_ini):
+    def gn_setup(self, dobs, Wd, m_min, m_max, mref, m_ini, lp: int = 2, eps2: float = 0.0):
         nm, nd = self.Nm, self.Nd
         self._gn_keep = [_f64(dobs, nd), _f64(Wd, nd), _f64(m_min, nm), _f64(m_max, nm),
                          _f64(mref, nm), _f64(m_ini, nm)]
-        s = GraGnSetup(*[_p(a) for a in self._gn_keep])
+        s = GraGnSetup(*[_p(a) for a in self._gn_keep], int(lp), float(eps2))
         self.ctx.check(self.ctx.L.gra_gn_setup_problem(self.ctx.h, ctypes.byref(s)), "gn_setup")
 
     def gn_step(self, lam, cg_tol, cg_maxit):

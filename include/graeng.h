@@ -176,8 +176,12 @@ gra_status gra_irls_weights(gra_ctx *ctx, const gra_csr *W, const double *m, int
  * GaussNewtonInversion.cpp:815-1048, with the lambda schedule chosen by the
  * caller and every trial accepted).  gra_gn_setup uploads the problem (the
  * regularisation operators come from gra_set_regularization);
- * gra_gn_step runs: P from m; dx = solve_cg(...); m_trial = barrier update;
- * misfit(m_trial); m = m_trial.  WL_* stay identity (L2).
+ * gra_gn_step runs: P from m; dx = solve_cg(...) (reference operator,
+ * quirk_mode 1); m_trial = barrier update; misfit(m_trial); m = m_trial; and,
+ * for an Lp norm (lp != 2), the IRLS weights WL_s = ((Ws(m-mref))^2 +
+ * eps2)^(-(2-lp)/4), WL_x/y/z likewise from Wx/Wy/Wz m, used by the next step
+ * (GaussNewtonInversion.cpp:962-1022 with the trial model accepted).  WL_* start
+ * as the identity.
  */
 typedef struct gra_gn_setup {
     const double *dobs;    /* [Nd] */
@@ -186,6 +190,8 @@ typedef struct gra_gn_setup {
     const double *m_max;   /* [Nm] */
     const double *mref;    /* [Nm] */
     const double *m_ini;   /* [Nm] */
+    int lp;                /* Lp exponent of the model norms (2 = L2, no IRLS) */
+    double eps2;           /* IRLS epsilon^2 (GaussNewtonInversion.h:57)      */
 } gra_gn_setup;
 
 gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *setup);

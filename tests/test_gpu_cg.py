@@ -124,6 +124,49 @@ def test_gauss_newton_resident_loop_vs_oracle(ctx):
         assert rel_norm(inv.gn_model(), m) < 1e-10, k
 
 
+def test_gauss_newton_l1_irls_vs_oracle(ctx):
+    """Resident loop with IRLS (p = 1): the weights of step k come from the
+    model accepted at step k-1 and enter solve_cg with the reference's
+    operator asymmetries (t_s without WL_s)."""
+    inv, G, aux = small_inversion(ctx)
+    nm = inv.Nm
+    lo, hi = np.full(nm, -1000.0), np.full(nm, 1000.0)
+    Wd = np.full(inv.Nd, 1.0 / (0.01 * np.abs(aux["dobs"]).max()))
+    inv.gn_setup(aux["dobs"], Wd, lo, hi, np.zeros(nm), np.zeros(nm), lp=1, eps2=1e-3)
+    m = np.zeros(nm)
+    WL = {}
+    for k in range(4):
+        lam = 50.0 * 0.5 ** k
+        mis, err, its = inv.gn_step(lam, 0.0, 15)
+        P = oracle.barrier_P(m, lo, hi)
+        dxo, erro, itso = oracle.solve_cg(G, 0.0, 15, lam, m, np.zeros(nm), Wd, aux["dobs"], P,
+                                          aux["Ws"], aux["Wx"], aux["Wy"], aux["Wz"], WL=WL,
+                                          quirks=True)
+        m = oracle.barrier_update(m, dxo, lo, hi, m.copy())
+        miso = oracle.misfit(G, m, Wd, aux["dobs"])
+        WL = {"s": oracle.irls_weights(aux["Ws"], m, 1, 1e-3),
+              "x": oracle.irls_weights(aux["Wx"], m, 1, 1e-3),
+              "y": oracle.irls_weights(aux["Wy"], m, 1, 1e-3),
+              "z": oracle.irls_weights(aux["Wz"], m, 1, 1e-3)}
+        assert its == itso == 15
+        assert abs(mis - miso) < 1e-10 * miso, k
+        assert rel_norm(inv.gn_model(), m) < 1e-10, k
+
+
+def test_launch_shapes_do_not_change_results_beyond_rounding(ctx):
+    inv, G, _ = small_inversion(ctx, nx=20, ny=18, nz=9, npt=15)
+    rng = np.random.default_rng(2)
+    x, y = rng.standard_normal(inv.Nm), rng.standard_normal(inv.Nd)
+    a, b = inv.G_vec_mul(x), inv.GT_vec_mul(y)
+    for kw in (dict(n_rows=8, n_unroll=1, n_splits=3, nontemporal=0),
+               dict(n_rows=2, n_unroll=2, n_splits=1, t_chunks=1, t_unroll=4),
+               dict(t_chunks=37, t_unroll=8, nontemporal=1)):
+        inv.ctx.tune(**kw)
+        assert rel_norm(inv.G_vec_mul(x), a) < 1e-14
+        assert rel_norm(inv.GT_vec_mul(y), b) < 1e-14
+    inv.ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1)
+
+
 def test_helpers_vs_oracle(ctx):
     inv, G, aux = small_inversion(ctx)
     nm = inv.Nm
