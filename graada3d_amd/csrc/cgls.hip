@@ -407,7 +407,7 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 
 gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const double *d_mref,
                       const double *d_dobs, const CgDiag &dg, double *d_dx, double *err,
-                      int *iterations)
+                      int *iterations, bool have_Gmk)
 {
     hipStream_t st = c.stream;
     const int64_t nd = c.Nd, nm = c.Nm, ld = c.ld;
@@ -430,8 +430,10 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         if ((e = c.vec_nm[q].reserve(vnm)) != hipSuccess) return hip_fail(c, e, "alloc vec");
     if ((e = c.vec_nd[0].reserve(sizeof(double) * (size_t)nd)) != hipSuccess)
         return hip_fail(c, e, "alloc w");
+    if (have_Gmk && c.part_n.bytes < sizeof(double) * (size_t)splits * nd) have_Gmk = false;
     if ((e = c.part_n.reserve(sizeof(double) * (size_t)splits * nd)) != hipSuccess)
         return hip_fail(c, e, "alloc part_n");
+    c.partn_holds_gn_model = false;
     if ((e = c.part_t.reserve(sizeof(double) * (size_t)chunks * ld)) != hipSuccess)
         return hip_fail(c, e, "alloc part_t");
     int64_t nblk_row = nblocks(nrow), nblk_nm = nblocks(nm);
@@ -507,7 +509,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(kB), 0, st, d.sc);
 
     // b, s, first residual and A^T b
-    {
+    if (!have_Gmk) {
         TimedRegion tr(c, kTimeGemvN);
         if ((e = launch_gemv_n(c.G.p, c.store, nd, ld, shape, d_mk, c.part_n.as<double>(), nullptr,
                                st)))

@@ -268,6 +268,7 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
     c.G.release();
     c.Nd = c.Nm = c.N_obs = c.ld = 0;
     c.gn_ready = false;
+    c.partn_holds_gn_model = false;
     HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
     if (ld > Nm) HIPCHK(hipMemsetAsync(c.G.p, 0, c.G.bytes, c.stream), "clear G padding");
     DevBuf dpr, dpt;
@@ -331,6 +332,7 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
     else HIPCHK(put_padded(x, vec, c.Nm, c.ld, c.stream), "upload vec");
     HIPCHK(y.reserve(sizeof(double) * (size_t)nout), "alloc out");
     const GemvShape shape = gemv_shape(c);
+    c.partn_holds_gn_model = false;
     if (!trans) {
         HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * c.Nd), "alloc part");
         {
@@ -540,6 +542,7 @@ gra_status gra_misfit(gra_ctx *ctx, const double *m, const double *Wd, const dou
                           c.stream),
            "dobs");
     const GemvShape shape = gemv_shape(c);
+    c.partn_holds_gn_model = false;
     HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * nd), "alloc");
     HIPCHK(c.red.reserve(sizeof(double) * (size_t)((nd + 255) / 256 + 1)), "alloc");
     HIPCHK(c.vec_nm[11].reserve(sizeof(double)), "alloc");
@@ -605,6 +608,7 @@ gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
            "alloc");
     c.gn_lp = s->lp;
     c.gn_eps2 = s->eps2;
+    c.partn_holds_gn_model = false;
     c.gn_have_wl = false;
     if (s->lp != 2) HIPCHK(c.gn_WL.reserve(sizeof(double) * 4 * (size_t)nm), "alloc WL");
     HIPCHK(hipStreamSynchronize(c.stream), "gn setup");
@@ -649,7 +653,7 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
     double err = NAN;
     int its = 0;
     gra_status s = cgls_solve(c, a, m, c.gn_mref.as<double>(), c.gn_dobs.as<double>(), dg,
-                              c.gn_dx.as<double>(), &err, &its);
+                              c.gn_dx.as<double>(), &err, &its, c.partn_holds_gn_model);
     if (s != GRA_OK) return s;
     HIPCHK(launch_barrier_update(m, c.gn_dx.as<double>(), c.gn_mmin.as<double>(),
                                  c.gn_mmax.as<double>(), nm, mt, scr, flags, c.stream),
@@ -662,6 +666,8 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
     HIPCHK(launch_misfit(c.part_n.as<double>(), shape.n_splits, nd, c.gn_Wd.as<double>(),
                          c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
            "misfit");
+    // part_n now holds G * m_trial, which becomes the next step's G * m_k
+    c.partn_holds_gn_model = true;
     // accept the trial model
     HIPCHK(hipMemcpyAsync(m, mt, sizeof(double) * (size_t)nm, hipMemcpyDeviceToDevice, c.stream),
            "accept");
@@ -700,6 +706,7 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
 {
     CTX_GUARD(ctx);
     GemvShape &o = c.tune_override;
+    c.partn_holds_gn_model = false;   // the split layout of the partials may change
     switch (param) {
     case GRA_TUNE_N_ROWS:
         if (value != 0 && value != 2 && value != 4 && value != 8)
@@ -724,7 +731,7 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
         o.t_unroll = (int)value;
         break;
     case GRA_TUNE_NONTEMPORAL:
-        if (value < -1 || value > 1) return set_error(c, GRA_BAD_ARG, "nontemporal: -1, 0, 1");
+        if (value < -1 || value > 3) return set_error(c, GRA_BAD_ARG, "nontemporal: -1..3");
         o.nontemporal = (int)value;
         break;
     default:

@@ -66,7 +66,7 @@ struct GemvShape {
     int n_splits = 1;      // column splits (partial sums summed by the consumer)
     int t_chunks = 1;      // row chunks of G^T w
     int t_unroll = 8;      // rows per loop iteration (4, 8)
-    int nontemporal = 1;   // non-temporal loads of G
+    int nontemporal = 3;   // non-temporal loads of G: bit 0 for G x, bit 1 for G^T w
 };
 
 // Kernel timing classes (gra_timing_get).
@@ -100,6 +100,9 @@ struct Ctx {
     bool gn_ready = false;
     DevBuf gn_dobs, gn_Wd, gn_mmin, gn_mmax, gn_mref, gn_m, gn_mtrial, gn_P, gn_dx, gn_one;
     DevBuf gn_WL;              // 4 x Nm: WL_s, WL_x, WL_y, WL_z (IRLS)
+    // part_n holds the column-split partials of G * gn_m (left by the last
+    // gn_step's misfit product); the next step reuses them for G * m_k.
+    bool partn_holds_gn_model = false;
     int gn_lp = 2;
     double gn_eps2 = 0.0;
     bool gn_have_wl = false;
@@ -158,9 +161,11 @@ struct CgDiag {
     const double *WLs, *WLx, *WLy, *WLz, *WLtx, *WLty, *WLtz;  // may be null (identity)
     const double *S;                                            // may be null
 };
+// have_Gmk: c.part_n already holds the partials of G * d_mk (same launch
+// shape), so the initial product is not recomputed.
 gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const double *d_mref,
                       const double *d_dobs, const CgDiag &dg, double *d_dx, double *err,
-                      int *iterations);
+                      int *iterations, bool have_Gmk = false);
 hipError_t launch_barrier_P(const double *m, const double *mmin, const double *mmax, int64_t n,
                             double *P, hipStream_t st);
 hipError_t launch_barrier_update(const double *m, const double *dx, const double *mmin,

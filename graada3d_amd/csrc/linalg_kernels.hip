@@ -246,7 +246,7 @@ template <typename T>
 void launch_n_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, const double *x,
                     double *part, const int *done, hipStream_t st)
 {
-    const bool nt = s.nontemporal != 0;
+    const bool nt = (s.nontemporal & 1) != 0;
     if (s.n_rows == 2) {
         if (s.n_unroll >= 2) launch_n_nt<T, 2, 2>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
         else launch_n_nt<T, 2, 1>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
@@ -267,7 +267,7 @@ void launch_t_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, c
     int64_t bx = (ld + kBlk * V - 1) / (kBlk * V);
     int64_t rpc = (Nd + s.t_chunks - 1) / s.t_chunks;
     dim3 grid((unsigned)bx, (unsigned)s.t_chunks), block(kBlk);
-    const bool nt = s.nontemporal != 0;
+    const bool nt = (s.nontemporal & 2) != 0;
 #define GRA_T(UR, NTV)                                                                        \
     hipLaunchKernelGGL((gemv_t_kernel<T, UR, NTV>), grid, block, 0, st, (const T *)G, Nd, ld, \
                        rpc, w, part, done)
@@ -292,7 +292,7 @@ GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm)
     s.n_rows = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
     s.n_unroll = 2;
     s.t_unroll = 8;
-    s.nontemporal = 1;
+    s.nontemporal = 3;
     int64_t waves = (Nd + s.n_rows - 1) / s.n_rows;
     int64_t sp = (8192 + waves - 1) / waves;
     int64_t max_by_cols = Nm / (4 * V * kWave) > 0 ? Nm / (4 * V * kWave) : 1;

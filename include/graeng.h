@@ -213,7 +213,7 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
 #define GRA_TUNE_N_SPLITS 2     /* column splits of G*x: 1..256            */
 #define GRA_TUNE_T_CHUNKS 3     /* row chunks of G^T*w: 1..1024            */
 #define GRA_TUNE_T_UNROLL 4     /* rows per iteration of G^T*w: 4, 8       */
-#define GRA_TUNE_NONTEMPORAL 5  /* non-temporal loads of G: 0, 1           */
+#define GRA_TUNE_NONTEMPORAL 5  /* non-temporal loads of G, mask: 1 G*x, 2 G^T*w */
 gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
 
