@@ -14,14 +14,16 @@ Workloads
 One step = one Gauss-Newton outer iteration on the GPU (barrier P, solve_cg =
 CGLS with two passes over G per iteration, barrier model update, misfit, IRLS
 weights).  `value` is CG iterations per second of the plain FP64 path.  The
-same steps are then run with G stored in FP32 and BF16 (FP64 accumulation)
-and reported as speed-up and error against the FP64 run (final model and
-misfit).  G assembly throughput (entries/s) is measured for every storage
-type.  The GEMV roofline uses HIP events on the engine's stream; the CPU
-baseline is the single-threaded C oracle on a bounded sample.
+same steps are then run with G stored in FP32 and BF16 (FP64 accumulation),
+and with the single-read iteration (one fused pass over G per CG iteration)
+for every storage type, and reported as speed-up and error against the plain
+FP64 run (final model and misfit).  G assembly throughput (entries/s) is
+measured for every storage type.  Kernel rooflines use HIP events on the
+engine's stream; the CPU baseline is the single-threaded C oracle on a
+bounded sample.
 
     python bench.py [--workload inversion_small|tensor_large] [--steps K] [--warmup W]
-                    [--stores f64,f32,bf16] [--no-cpu] [--gpus 1]
+                    [--stores f64,f32,bf16] [--no-fused] [--no-cpu] [--gpus 1]
 """
 from __future__ import annotations
 
@@ -63,8 +65,8 @@ def build_problem(name):
 
 
 def run(args):
-    from graada3d_amd.engine import (TIME_ASSEMBLE, TIME_GEMV_N, TIME_GEMV_T, Context,
-                                     GaussNewtonInversion)
+    from graada3d_amd.engine import (TIME_ASSEMBLE, TIME_FUSED, TIME_GEMV_N, TIME_GEMV_T,
+                                     Context, GaussNewtonInversion)
     from graada3d_amd.workloads import add_noise
 
     pb = build_problem(args.workload)
@@ -82,19 +84,28 @@ def run(args):
     Wd = np.repeat(1.0 / sigma, fp.points.shape[0])
 
     results = {}
-    for store in args.stores:
+    variants = [(s, False) for s in args.stores]
+    if args.fused:
+        variants += [(s, True) for s in args.stores]
+    asm = {}
+    for store, fused in variants:
+        key = store + ("_fused" if fused else "")
+        # the context holds one G: assemble it in this variant's storage type
+        # (timed the first time, twice on the small workload)
         ctx.timing(True)
         ctx.timing_reset()
-        reps = 2 if args.workload == "inversion_small" else 1
+        reps = 2 if args.workload == "inversion_small" and store not in asm else 1
         for _ in range(reps):
             inv.compute_G(store)
-        asm_ms, _ = ctx.timing_get(TIME_ASSEMBLE)
+        if store not in asm:
+            asm[store], _ = ctx.timing_get(TIME_ASSEMBLE)
         ctx.timing(False)
+        asm_ms = asm[store]
         inv.set_regularization(*pb["W"])
 
         def setup():
             inv.gn_setup(dobs, Wd, pb["m_min"], pb["m_max"], pb["mref"], pb["m_ini"], pb["lp"],
-                         pb["eps2"])
+                         pb["eps2"], fused=fused)
 
         setup()
         for k in range(args.warmup):
@@ -113,17 +124,24 @@ def run(args):
         el = time.perf_counter() - t0
         gn_ms, gn_c = ctx.timing_get(TIME_GEMV_N)
         gt_ms, gt_c = ctx.timing_get(TIME_GEMV_T)
+        fu_ms, fu_c = ctx.timing_get(TIME_FUSED)
         ctx.timing(False)
         gbytes = float(nd) * nm * BYTES[store]
-        results[store] = dict(
+
+        def kern(ms, n):
+            return dict(avg_ms=ms, launches=n, GBps=gbytes / (ms * 1e-3) / 1e9 if n else None)
+
+        results[key] = dict(
+            store=store, fused=fused,
             iter_s=its / el, ms_per_step=el / args.steps * 1e3, cg_iters=its,
             misfit=mis[-1], model=inv.gn_model(),
             asm_ms=asm_ms, entries_per_s=nd * nm / (asm_ms * 1e-3),
-            gemv_n=dict(avg_ms=gn_ms, launches=gn_c, GBps=gbytes / (gn_ms * 1e-3) / 1e9),
-            gemv_t=dict(avg_ms=gt_ms, launches=gt_c, GBps=gbytes / (gt_ms * 1e-3) / 1e9),
+            gemv_n=kern(gn_ms, gn_c), gemv_t=kern(gt_ms, gt_c), fused_product=kern(fu_ms, fu_c),
             tuning=ctx.tuning())
-        print(f"[bench] {store}: {results[store]['iter_s']:.1f} CG iter/s, assembly "
-              f"{results[store]['entries_per_s']:.3e} entries/s", file=sys.stderr, flush=True)
+        if fused and fu_c == 0:
+            raise RuntimeError("the fused product did not run (grid not co-resident?)")
+        print(f"[bench] {key}: {results[key]['iter_s']:.1f} CG iter/s, assembly "
+              f"{results[key]['entries_per_s']:.3e} entries/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -177,6 +195,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--stores", default="f64,f32,bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fused", dest="fused", action="store_false",
+                    help="skip the single-read (fused) CG iteration variants")
     ap.add_argument("--cpu-iters", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -203,8 +223,12 @@ def main():
                     "model_rel_err": float(np.linalg.norm(r["model"] - ref["model"])
                                            / np.linalg.norm(ref["model"])),
                     "misfit_rel_err": abs(r["misfit"] - ref["misfit"]) / ref["misfit"],
-                    "gemv_GBps": {"n": r["gemv_n"]["GBps"], "t": r["gemv_t"]["GBps"]},
                     "g_entries_per_s": r["entries_per_s"]}
+        if r["fused"]:
+            mixed[s]["fused_GBps"] = r["fused_product"]["GBps"]
+            mixed[s]["fused_avg_ms"] = r["fused_product"]["avg_ms"]
+        else:
+            mixed[s]["gemv_GBps"] = {"n": r["gemv_n"]["GBps"], "t": r["gemv_t"]["GBps"]}
     dom = "gemv_t" if ref["gemv_t"]["avg_ms"] >= ref["gemv_n"]["avg_ms"] else "gemv_n"
     achieved = ref[dom]["GBps"]
     # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs

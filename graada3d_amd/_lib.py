@@ -31,7 +31,8 @@ EXPORTS = [
     "gra_timing_get", "gra_tune_set", "gra_tune_get",
 ]
 
-TUNE = {"n_rows": 0, "n_unroll": 1, "n_splits": 2, "t_chunks": 3, "t_unroll": 4, "nontemporal": 5}
+TUNE = {"n_rows": 0, "n_unroll": 1, "n_splits": 2, "t_chunks": 3, "t_unroll": 4, "nontemporal": 5,
+        "fused_wgs": 6}
 
 
 class GraError(RuntimeError):
@@ -51,12 +52,13 @@ class GraCgArgs(ctypes.Structure):
                 ("WL_s", c_dp), ("WL_x", c_dp), ("WL_y", c_dp), ("WL_z", c_dp),
                 ("WL_tx", c_dp), ("WL_ty", c_dp), ("WL_tz", c_dp),
                 ("use_cross_gradient", ctypes.c_int), ("a_crg", ctypes.c_double),
-                ("S_crg", c_dp), ("quirk_mode", ctypes.c_int)]
+                ("S_crg", c_dp), ("quirk_mode", ctypes.c_int), ("fused", ctypes.c_int)]
 
 
 class GraGnSetup(ctypes.Structure):
     _fields_ = [("dobs", c_dp), ("Wd", c_dp), ("m_min", c_dp), ("m_max", c_dp),
-                ("mref", c_dp), ("m_ini", c_dp), ("lp", ctypes.c_int), ("eps2", ctypes.c_double)]
+                ("mref", c_dp), ("m_ini", c_dp), ("lp", ctypes.c_int), ("eps2", ctypes.c_double),
+                ("fused", ctypes.c_int)]
 
 
 _lib = None

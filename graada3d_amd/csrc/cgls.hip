@@ -20,6 +20,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "engine.hpp"
 
@@ -54,6 +55,12 @@ struct CgDev {
     double *red;
     double *red2;
     CgScalars *sc;
+    // single-read fused iteration (see fused_kernels.hip)
+    double *rloop;            // loop-operator residual of s0 (vec8 with WL_ty), or null
+    const double *zpart;      // [zgroups][ld] G^T (Wd^2 G Pp) partials per row group
+    int zgroups;
+    const double *t2part;     // [nt2] ||t_d||^2 partials of the row blocks
+    int64_t nt2;
 };
 
 __device__ inline double dgv(const double *d, int64_t i) { return d ? d[i] : 1.0; }
@@ -165,6 +172,7 @@ __global__ void __launch_bounds__(kB) k_r_update(CgDev d)
         for (int c = 1; c < d.chunks; ++c) gsum += d.part_t[(int64_t)c * d.ld + j];
         double Pj = d.P[j];
         double r = Pj * gsum;
+        double rl = r;
         double atb = 0.0 + Pj * (-gsum);
         const double *seg = d.s + d.nd;
         for (int q = 0; q < d.nseg; ++q) {
@@ -174,13 +182,15 @@ __global__ void __launch_bounds__(kB) k_r_update(CgDev d)
             double v = row_dot(d.AT[q], j, pre, d.P, R, seg + (int64_t)q * d.nm);
             r = r + v;
             if (MODE == 1) {
-                double vb = (q == 6 && d.quirks)
-                                ? -row_dot(d.AT[q], j, pre, d.P, d.WL[6], seg + (int64_t)q * d.nm)
-                                : -v;
-                atb += vb;
+                double vl = (q == 6 && d.quirks)
+                                ? row_dot(d.AT[q], j, pre, d.P, d.WL[6], seg + (int64_t)q * d.nm)
+                                : v;
+                rl = rl + vl;
+                atb += -vl;
             }
         }
         d.r[j] = r;
+        if (MODE == 1 && d.rloop) d.rloop[j] = rl;
         r2 = r * r;
         atb2 = atb * atb;
     }
@@ -288,6 +298,71 @@ __global__ void __launch_bounds__(kB) k_scalar_end(CgDev d, int64_t nblk, int it
         sc->iterations = iter + 1;
         if (sc->err < tol) sc->done = 1;
     }
+}
+
+// ---- single-read fused iteration -------------------------------------------
+// t rows of the regularisation blocks (the data rows t_d come from the fused
+// kernel) with t^2 block partials.
+__global__ void __launch_bounds__(kB) k_t_rows_reg(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t h = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double t2 = 0.0;
+    if (h < d.nrow - d.nd) {
+        int q = (int)(h / d.nm);
+        int64_t i = h - (int64_t)q * d.nm;
+        double pre;
+        const double *L;
+        t_row_factors(d, q, pre, L);
+        double t = row_dot(d.A[q], i, pre, L, nullptr, d.Pp);
+        d.t[d.nd + h] = t;
+        t2 = t * t;
+    }
+    double s = block_sum(t2);
+    if (threadIdx.x == 0) d.red[blockIdx.x] = s;
+}
+
+// alpha = r2 / (||t_d||^2 + ||t_reg||^2), both sums in a fixed order.
+__global__ void __launch_bounds__(kB) k_alpha_fused(CgDev d, int64_t nblk)
+{
+    if (d.sc->done) return;
+    double td2 = sum_partials(d.t2part, d.nt2);
+    double tr2 = sum_partials(d.red, nblk);
+    if (threadIdx.x == 0) {
+        double t2 = td2 + tr2;
+        double a = d.sc->r2 / t2;
+        d.sc->t2 = t2;
+        d.sc->alpha = a;
+        if (!(t2 != 0.0) || !isfinite(a)) d.sc->breakdown = 1;
+    }
+}
+
+// r <- r + alpha A_r^T t: data part P_j (sum over row groups of the fused
+// G^T (Wd^2 G Pp)), regularisation part with the loop operator's weights.
+// In the first iteration the base is the loop-operator residual of s0 when it
+// differs from the initial one (reference quirk at GN.cpp:1508).
+template <bool FIRST>
+__global__ void __launch_bounds__(kB) k_q_update(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double r2 = 0.0;
+    if (j < d.nm) {
+        double z = d.zpart[j];
+        for (int g = 1; g < d.zgroups; ++g) z += d.zpart[(int64_t)g * d.ld + j];
+        double q = d.P[j] * z;
+        const double *seg = d.t + d.nd;
+        for (int k = 0; k < d.nseg; ++k) {
+            double pre = k < 4 ? d.sl : 1.0;
+            q = q + row_dot(d.AT[k], j, pre, d.P, d.WL[k], seg + (int64_t)k * d.nm);
+        }
+        double base = (FIRST && d.rloop) ? d.rloop[j] : d.r[j];
+        double r = base + d.sc->alpha * q;
+        d.r[j] = r;
+        r2 = r * r;
+    }
+    double s = block_sum(r2);
+    if (threadIdx.x == 0) d.red[blockIdx.x] = s;
 }
 
 // T*_scaled = (a*S_i) * T0_ij, and for the transpose (a*S_j) * T0_ji.
@@ -420,6 +495,20 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     const int splits = shape.n_splits;
     const int chunks = shape.t_chunks;
     hipError_t e;
+    FusedPlan plan;
+    bool fused = a.fused && fused_plan(c.store, nd, ld, c.device, c.fused_wgs, &plan);
+    int *fused_err = c.status.as<int>() + 1;
+    if (fused) {
+        if ((e = c.fused_scratch.reserve(fused_scratch_bytes(plan, ld))) != hipSuccess)
+            return hip_fail(c, e, "alloc fused scratch");
+        if ((e = c.fused_coef.reserve(sizeof(double) * (size_t)nd)) != hipSuccess)
+            return hip_fail(c, e, "alloc fused coef");
+        if ((e = hipMemsetAsync(fused_err, 0, sizeof(int), st)) != hipSuccess)
+            return hip_fail(c, e, "clear status");
+    }
+    const bool need_rloop = fused && crg && a.quirk_mode;
+    if (need_rloop && (e = c.rloop.reserve(sizeof(double) * (size_t)ld)) != hipSuccess)
+        return hip_fail(c, e, "alloc rloop");
 
     // work vectors
     size_t need_rows = sizeof(double) * (size_t)nrow;
@@ -501,6 +590,11 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     d.red = c.red.as<double>();
     d.red2 = d.red + nred;
     d.sc = c.scal.as<CgScalars>();
+    d.rloop = need_rloop ? c.rloop.as<double>() : nullptr;
+    if (fused) {
+        d.zgroups = plan.S;
+        d.nt2 = (int64_t)plan.S * plan.nrb_group;
+    }
 
     // zero p, Pp (incl. padding up to ld) and dx
     hipLaunchKernelGGL(k_fill, dim3(nblocks(ld)), dim3(kB), 0, st, d.p, ld, 0.0);
@@ -527,33 +621,69 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg init");
 
     const int *done = &d.sc->done;
-    for (int it = 0; it < a.maxit; ++it) {
-        TimedRegion tr_it(c, kTimeCgIter);
-        hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
-        {
-            TimedRegion tr(c, kTimeGemvN);
-            e = launch_gemv_n(c.G.p, c.store, nd, ld, shape, d.Pp, c.part_n.as<double>(), done,
-                              st);
-            if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
+    if (!fused) {
+        for (int it = 0; it < a.maxit; ++it) {
+            TimedRegion tr_it(c, kTimeCgIter);
+            hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+            {
+                TimedRegion tr(c, kTimeGemvN);
+                e = launch_gemv_n(c.G.p, c.store, nd, ld, shape, d.Pp, c.part_n.as<double>(), done,
+                                  st);
+                if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
+            }
+            hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, d, nblk_row);
+            hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
+            {
+                TimedRegion tr(c, kTimeGemvT);
+                e = launch_gemv_t(c.G.p, c.store, nd, ld, shape, d.w, c.part_t.as<double>(), done,
+                                  st);
+                if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
+            }
+            hipLaunchKernelGGL(k_r_update<0>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_nm, it, a.tol);
+            if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
         }
-        hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
-        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, d, nblk_row);
-        hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
-        {
-            TimedRegion tr(c, kTimeGemvT);
-            e = launch_gemv_t(c.G.p, c.store, nd, ld, shape, d.w, c.part_t.as<double>(), done,
-                              st);
-            if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
+    } else {
+        const int64_t nreg = nrow - nd;
+        const int64_t nblk_reg = nblocks(nreg);
+        for (int it = 0; it < a.maxit; ++it) {
+            TimedRegion tr_it(c, kTimeCgIter);
+            hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+            {
+                TimedRegion tr(c, kTimeFused);
+                double *zp = nullptr, *t2p = nullptr;
+                e = launch_fused(plan, c.G.p, nd, ld, d.Pp, dg.Wd, c.fused_coef.as<double>(), d.t,
+                                 c.fused_scratch.p, fused_err, done, &zp, &t2p, st);
+                if (e != hipSuccess) return hip_fail(c, e, "fused product");
+                d.zpart = zp;
+                d.t2part = t2p;
+            }
+            hipLaunchKernelGGL(k_t_rows_reg, dim3((unsigned)nblk_reg), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_alpha_fused, dim3(1), dim3(kB), 0, st, d, nblk_reg);
+            hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
+            if (it == 0)
+                hipLaunchKernelGGL(k_q_update<true>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+            else
+                hipLaunchKernelGGL(k_q_update<false>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_nm, it, a.tol);
+            if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
         }
-        hipLaunchKernelGGL(k_r_update<0>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
-        hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_nm, it, a.tol);
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
     }
 
     CgScalars hs;
+    int ferr = 0;
     if ((e = hipMemcpyAsync(&hs, d.sc, sizeof(hs), hipMemcpyDeviceToHost, st)) != hipSuccess)
         return hip_fail(c, e, "copy scalars");
+    if (fused && (e = hipMemcpyAsync(&ferr, fused_err, sizeof(int), hipMemcpyDeviceToHost, st)) !=
+                     hipSuccess)
+        return hip_fail(c, e, "copy status");
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(c, e, "cg sync");
+    c.last_solve_fused = fused;
+    if (ferr)
+        return set_error(c, GRA_HIP_ERROR,
+                         "fused product: a cross-workgroup wait timed out (code " +
+                             std::to_string(ferr) + ")");
     if (a.maxit > 0 && err) *err = hs.err;
     if (iterations) *iterations = a.maxit > 0 ? (hs.done ? hs.iterations : a.maxit) : 0;
     if (hs.breakdown)

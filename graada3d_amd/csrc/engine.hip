@@ -608,6 +608,7 @@ gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
            "alloc");
     c.gn_lp = s->lp;
     c.gn_eps2 = s->eps2;
+    c.gn_fused = s->fused;
     c.partn_holds_gn_model = false;
     c.gn_have_wl = false;
     if (s->lp != 2) HIPCHK(c.gn_WL.reserve(sizeof(double) * 4 * (size_t)nm), "alloc WL");
@@ -641,6 +642,7 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
     a.maxit = cg_maxit;
     a.lambda = lambda;
     a.quirk_mode = 1;
+    a.fused = c.gn_fused;
     CgDiag dg{c.gn_Wd.as<double>(), c.gn_P.as<double>(), nullptr, nullptr, nullptr, nullptr,
               nullptr, nullptr, nullptr, nullptr};
     if (c.gn_have_wl) {
@@ -734,6 +736,10 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
         if (value < -1 || value > 3) return set_error(c, GRA_BAD_ARG, "nontemporal: -1..3");
         o.nontemporal = (int)value;
         break;
+    case GRA_TUNE_FUSED_WGS:
+        if (value < 0 || value > 65536) return set_error(c, GRA_BAD_ARG, "fused workgroups: 0..65536");
+        c.fused_wgs = (int)value;
+        break;
     default:
         return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
     }
@@ -752,6 +758,7 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_T_CHUNKS: *value = s.t_chunks; break;
     case GRA_TUNE_T_UNROLL: *value = s.t_unroll; break;
     case GRA_TUNE_NONTEMPORAL: *value = s.nontemporal; break;
+    case GRA_TUNE_FUSED_WGS: *value = c.fused_wgs; break;
     default: return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
     }
     return GRA_OK;

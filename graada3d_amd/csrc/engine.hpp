@@ -70,7 +70,20 @@ struct GemvShape {
 };
 
 // Kernel timing classes (gra_timing_get).
-enum TimingClass { kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeClasses = 4 };
+enum TimingClass {
+    kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeFused = 4,
+    kTimeClasses = 5
+};
+
+// Launch plan of the single-read fused product (fused_kernels.hip).
+struct FusedPlan {
+    int NT = 0, S = 0;            // column tiles, row groups (grid = NT * S)
+    int64_t rows_per_group = 0;
+    int64_t nrb_group = 0;        // row blocks per group
+    int RB = 0;                   // rows per block
+    int threads = 0;
+    const void *kernel = nullptr;
+};
 
 struct Ctx {
     int device = 0;
@@ -90,6 +103,7 @@ struct Ctx {
 
     // scratch
     DevBuf part_n, part_t;     // GEMV partial sums
+    DevBuf fused_scratch, fused_coef, rloop;
     DevBuf vec_nd[4], vec_nm[12], vec_row[3];
     DevBuf red;                // block partial sums for reductions
     DevBuf scal;               // CgScalars
@@ -103,12 +117,15 @@ struct Ctx {
     // part_n holds the column-split partials of G * gn_m (left by the last
     // gn_step's misfit product); the next step reuses them for G * m_k.
     bool partn_holds_gn_model = false;
+    bool last_solve_fused = false;   // the last CG solve used the single-read product
+    int gn_fused = 0;
     int gn_lp = 2;
     double gn_eps2 = 0.0;
     bool gn_have_wl = false;
 
     // launch-shape overrides (0 = automatic; nontemporal -1 = automatic)
     GemvShape tune_override{0, 0, 0, 0, 0, -1};
+    int fused_wgs = 0;            // 0: two per CU
 
     // timing
     bool timing = false;
@@ -155,6 +172,16 @@ hipError_t launch_reduce_parts(const double *part, int nparts, int64_t stride, i
 hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
                            double *out, int col_major, hipStream_t st);
 
+// Fused product (fused_kernels.hip).  fused_plan returns false when the
+// cooperative grid cannot be co-resident on `device`; target_wgs <= 0 means
+// two workgroups per CU, larger values are capped at the co-resident limit.
+bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, FusedPlan *out);
+size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld);
+hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
+                        const double *x, const double *Wd, double *coef, double *tdat,
+                        void *scratch, int *err, const int *done, double **zpart_out,
+                        double **t2part_out, hipStream_t st);
+
 // CGLS (cgls.hip)
 struct CgDiag {
     const double *Wd, *P;
@@ -162,7 +189,9 @@ struct CgDiag {
     const double *S;                                            // may be null
 };
 // have_Gmk: c.part_n already holds the partials of G * d_mk (same launch
-// shape), so the initial product is not recomputed.
+// shape), so the initial product is not recomputed.  a.fused selects the
+// single-read iteration (falls back to the two-pass one when the fused grid
+// cannot be co-resident).
 gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const double *d_mref,
                       const double *d_dobs, const CgDiag &dg, double *d_dx, double *err,
                       int *iterations, bool have_Gmk = false);

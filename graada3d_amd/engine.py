@@ -119,7 +119,7 @@ class Context:
         return out
 
 
-TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER = 0, 1, 2, 3
+TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER, TIME_FUSED = 0, 1, 2, 3, 4
 
 
 def csr_struct(m, keep: list) -> GraCsr:
@@ -206,7 +206,8 @@ class GaussNewtonInversion(Fwd):
         self.ctx.check(st, "set_regularization")
 
     def solve_cg(self, tol, maxit, lam, mk, mref, Wd, dobs, P, WL_s=None, WL_x=None, WL_y=None,
-                 WL_z=None, WL_tx=None, WL_ty=None, WL_tz=None, quirks: bool = True):
+                 WL_z=None, WL_tx=None, WL_ty=None, WL_tz=None, quirks: bool = True,
+                 fused: bool = False):
         """GaussNewtonInversion::solve_cg; diagonal operators as vectors (None =
         identity).  Regularisation operators come from set_regularization.
         Returns (dx, err, iterations)."""
@@ -228,6 +229,7 @@ class GaussNewtonInversion(Fwd):
         a.a_crg = float(self.a_crg)
         a.S_crg = _p(S)
         a.quirk_mode = 1 if quirks else 0
+        a.fused = 1 if fused else 0
         dx = np.empty(nm)
         err = ctypes.c_double(np.nan)
         its = ctypes.c_int(0)
@@ -269,11 +271,12 @@ class GaussNewtonInversion(Fwd):
         return out
 
     # resident Gauss-Newton loop (model and CG state stay on the GPU)
-    def gn_setup(self, dobs, Wd, m_min, m_max, mref, m_ini, lp: int = 2, eps2: float = 0.0):
+    def gn_setup(self, dobs, Wd, m_min, m_max, mref, m_ini, lp: int = 2, eps2: float = 0.0,
+                 fused: bool = False):
         nm, nd = self.Nm, self.Nd
         self._gn_keep = [_f64(dobs, nd), _f64(Wd, nd), _f64(m_min, nm), _f64(m_max, nm),
                          _f64(mref, nm), _f64(m_ini, nm)]
-        s = GraGnSetup(*[_p(a) for a in self._gn_keep], int(lp), float(eps2))
+        s = GraGnSetup(*[_p(a) for a in self._gn_keep], int(lp), float(eps2), 1 if fused else 0)
         self.ctx.check(self.ctx.L.gra_gn_setup_problem(self.ctx.h, ctypes.byref(s)), "gn_setup")
 
     def gn_step(self, lam, cg_tol, cg_maxit):

@@ -138,6 +138,12 @@ typedef struct gra_cg_args {
      *    WL_z/WL_x/WL_y, :1582-1584; initial residual's Ty term uses WL_y,
      *    :1508).  0: consistent operator (A^T of the same A).            */
     int quirk_mode;
+    /* 0: the reference's iteration, two passes over G (r = A^T s).
+     * 1: single-read iteration: one fused pass computes G Pp and
+     *    G^T Wd^2 G Pp together and r is updated as r + alpha A^T t (equal in
+     *    exact arithmetic, different rounding).  Falls back to 0 when the
+     *    fused grid cannot be co-resident on the device.               */
+    int fused;
 } gra_cg_args;
 
 /*
@@ -192,6 +198,7 @@ typedef struct gra_gn_setup {
     const double *m_ini;   /* [Nm] */
     int lp;                /* Lp exponent of the model norms (2 = L2, no IRLS) */
     double eps2;           /* IRLS epsilon^2 (GaussNewtonInversion.h:57)      */
+    int fused;             /* CG iteration as gra_cg_args.fused               */
 } gra_gn_setup;
 
 gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *setup);
@@ -214,6 +221,8 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
 #define GRA_TUNE_T_CHUNKS 3     /* row chunks of G^T*w: 1..1024            */
 #define GRA_TUNE_T_UNROLL 4     /* rows per iteration of G^T*w: 4, 8       */
 #define GRA_TUNE_NONTEMPORAL 5  /* non-temporal loads of G, mask: 1 G*x, 2 G^T*w */
+#define GRA_TUNE_FUSED_WGS 6    /* workgroups of the fused CG product: 1..65536,
+                                   capped at what can be co-resident      */
 gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
 
@@ -221,7 +230,8 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
 /*
  * Average device time (ms, from GPU events on the context's stream) of the
  * launches of kernel class `which` since the last gra_timing_reset:
- * 0 = GEMV G*x, 1 = GEMV G^T*x, 2 = G assembly, 3 = one full CG iteration.
+ * 0 = GEMV G*x, 1 = GEMV G^T*x, 2 = G assembly, 3 = one full CG iteration,
+ * 4 = fused single-read product (G Pp and G^T Wd^2 G Pp).
  * Timing events are recorded only while enabled.
  */
 gra_status gra_timing_enable(gra_ctx *ctx, int on);

@@ -142,10 +142,12 @@ def _csr(m, keep):
 
 
 def solve_cg(G, tol, maxit, lam, mk, mref, Wd, dobs, P, Ws, Wx, Wy, Wz,
-             WL=None, crg=None, quirks=True):
+             WL=None, crg=None, quirks=True, recurrence=False):
     """Oracle ``solve_cg``.  G dense row-major; W* scipy sparse; WL a dict of
     diagonal vectors with keys s,x,y,z,tx,ty,tz (missing = identity); crg
     None or dict(a=..., S=vec or None, Tx=, Ty=, Tz= scipy sparse).
+    ``recurrence`` selects the r <- r + alpha A^T t update of the
+    single-read iteration instead of the reference's r = A^T s.
     Returns (dx, err, iterations)."""
     L = lib()
     keep = []
@@ -178,13 +180,14 @@ def solve_cg(G, tol, maxit, lam, mk, mref, Wd, dobs, P, Ws, Wx, Wy, Wz,
     fn.argtypes = ([ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                     ctypes.POINTER(ctypes.c_int), ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
                    + [CP] * 8 + [ctypes.c_int, ctypes.c_double, _dp] + [CP] * 6 + [_dp] * 7
-                   + [ctypes.c_int, _dp, _i64, _i64, _dp])
+                   + [ctypes.c_int, ctypes.c_int, _dp, _i64, _i64, _dp])
     G = _c(G)
     arrs = [_c(mk), _c(mref), _c(Wd), _c(dobs), _c(P)]
     fn(float(tol), int(maxit), ctypes.byref(err), ctypes.byref(its), float(lam),
        *[_ptr(a) for a in arrs], *[ctypes.byref(c) for c in cs],
        1 if crg is not None else 0, a_crg, _ptr(S), *[ctypes.byref(c) for c in ts],
-       *[_ptr(a) for a in wl], 1 if quirks else 0, _ptr(G), G.shape[0], G.shape[1], _ptr(dx))
+       *[_ptr(a) for a in wl], 1 if quirks else 0, 1 if recurrence else 0, _ptr(G), G.shape[0],
+       G.shape[1], _ptr(dx))
     return dx, err.value, its.value
 
 

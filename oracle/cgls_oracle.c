@@ -143,6 +143,13 @@ static double sumsq(const double *v, int64_t n)
  * CSR operator are supplied by the caller.  Returns dx in `dx` (length nm),
  * writes *err and *iterations as the reference does (err is left untouched
  * when maxit == 0, as in the reference).
+ *
+ * recurrence = 0 is the reference's iteration (r = A^T s after every update of
+ * s, GN.cpp:1637-1672).  recurrence = 1 updates r <- r + alpha A^T t instead,
+ * equal in exact arithmetic because s <- s + alpha t and A^T is linear; in the
+ * first iteration the base is A^T s0 with the loop's weights (WL_ty in the
+ * last cross-gradient term), since the initial residual of the reference uses
+ * WL_y there (GN.cpp:1508).  This is the single-read iteration's algorithm.
  */
 int ora_solve_cg(double tol, int maxit, double *err, int *iterations, double lambda,
                  const double *mk, const double *mref, const double *Wd, const double *dobs,
@@ -154,7 +161,8 @@ int ora_solve_cg(double tol, int maxit, double *err, int *iterations, double lam
                  const ora_csr *Tx0T, const ora_csr *Ty0T, const ora_csr *Tz0T,
                  const double *WLs, const double *WLx, const double *WLy, const double *WLz,
                  const double *WLtx, const double *WLty, const double *WLtz,
-                 int quirks, const double *G, int64_t nd, int64_t nm, double *dx)
+                 int quirks, int recurrence, const double *G, int64_t nd, int64_t nm,
+                 double *dx)
 {
     cg_ops o;
     memset(&o, 0, sizeof(o));
@@ -235,6 +243,13 @@ int ora_solve_cg(double tol, int maxit, double *err, int *iterations, double lam
         free(one);
     }
     double ATb2 = sumsq(ATb, nm);
+    double *rloop = NULL, *q = NULL;
+    if (recurrence) {
+        rloop = (double *)calloc((size_t)nm, sizeof(double));
+        q = (double *)calloc((size_t)nm, sizeof(double));
+        if (quirks && use_crg) apply_AT(&o, s, rloop, WLty);
+        else memcpy(rloop, r, sizeof(double) * (size_t)nm);
+    }
     double beta = 0.0, r2 = 0.0, t2, rlast2, alpha;
     int i;
     for (i = 0; i < maxit; i++) {
@@ -256,7 +271,13 @@ int ora_solve_cg(double tol, int maxit, double *err, int *iterations, double lam
         alpha = r2 / t2;
         for (int64_t k = 0; k < nrow; k++) s[k] += alpha * t[k];
         for (int64_t k = 0; k < nm; k++) dx[k] += alpha * p[k];
-        apply_AT(&o, s, r, WLty);
+        if (recurrence) {
+            apply_AT(&o, t, q, WLty);
+            const double *base = i == 0 ? rloop : r;
+            for (int64_t k = 0; k < nm; k++) r[k] = base[k] + alpha * q[k];
+        } else {
+            apply_AT(&o, s, r, WLty);
+        }
         rlast2 = r2;
         r2 = sumsq(r, nm);
         beta = r2 / rlast2;
@@ -269,7 +290,7 @@ int ora_solve_cg(double tol, int maxit, double *err, int *iterations, double lam
     if (i == maxit) *iterations = maxit;
 
     free(s); free(b); free(t); free(r); free(p); free(Pp); free(ATb); free(w);
-    free(o.tmp_nd); free(o.tmp_nm); free(o.tmp_nm2); free(gv);
+    free(o.tmp_nd); free(o.tmp_nm); free(o.tmp_nm2); free(gv); free(rloop); free(q);
     for (int q = 0; q < 6; q++) free(tv[q]);
     return 0;
 }

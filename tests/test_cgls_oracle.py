@@ -80,14 +80,17 @@ def stacked(p, lam, quirks=False):
         [B @ P for B in rblocks_init]
 
 
-def dense_cgls(p, lam, maxit, tol, quirks):
-    """The reference's iteration, restated with dense matrices."""
+def dense_cgls(p, lam, maxit, tol, quirks, recurrence=False):
+    """The reference's iteration, restated with dense matrices.  With
+    ``recurrence`` the gradient is updated as r + alpha A_r^T t, starting from
+    A_r^T s0 (the loop operator) in the first iteration."""
     A_t, b, A_r, A_r0 = stacked(p, lam, quirks)
     At = np.vstack(A_t)
     Ar = np.vstack(A_r)
     Ar0 = np.vstack(A_r0)
     s = -b
     r = Ar0.T @ s
+    r_loop0 = Ar.T @ s
     ATb = Ar.T @ b
     ATb2 = ATb @ ATb
     p_ = np.zeros_like(r)
@@ -102,7 +105,10 @@ def dense_cgls(p, lam, maxit, tol, quirks):
         s = s + alpha * t
         dx = dx + alpha * p_
         rl = r @ r
-        r = Ar.T @ s
+        if recurrence:
+            r = (r_loop0 if i == 0 else r) + alpha * (Ar.T @ t)
+        else:
+            r = Ar.T @ s
         beta = (r @ r) / rl
         err = (r @ r) / ATb2
         if err < tol:
@@ -111,10 +117,10 @@ def dense_cgls(p, lam, maxit, tol, quirks):
     return dx, err, its
 
 
-def run_oracle(p, lam, maxit, tol, quirks):
+def run_oracle(p, lam, maxit, tol, quirks, recurrence=False):
     return oracle.solve_cg(p["G"], tol, maxit, lam, p["mk"], p["mref"], p["Wd"], p["dobs"],
                            p["P"], p["Ws"], p["Wx"], p["Wy"], p["Wz"], WL=p["WL"], crg=p["crg"],
-                           quirks=quirks)
+                           quirks=quirks, recurrence=recurrence)
 
 
 @pytest.mark.parametrize("crg", [False, True])
@@ -136,6 +142,32 @@ def test_matches_dense_restatement_with_quirks(crg, wl):
     assert its == its2 == 15
     assert np.linalg.norm(dx - dx2) < 1e-10 * np.linalg.norm(dx2)
     assert abs(err - err2) < 1e-8 * abs(err2)
+
+
+@pytest.mark.parametrize("quirks,crg,wl", [(True, False, False), (True, True, True),
+                                           (False, True, True)])
+def test_recurrence_matches_dense_restatement(quirks, crg, wl):
+    """The single-read iteration's update r <- r + alpha A^T t (with the loop
+    operator's A^T s0 as the first base) against the dense restatement; with
+    quirks and cross-gradient the first base differs from the initial
+    residual, so using the latter fails here by O(1)."""
+    p = problem(5, crg=crg, wl=wl)
+    dx, err, its = run_oracle(p, 2.5, 10, 1e-30, quirks, recurrence=True)
+    dx2, err2, its2 = dense_cgls(p, 2.5, 10, 1e-30, quirks, recurrence=True)
+    assert its == its2 == 10
+    assert np.linalg.norm(dx - dx2) < 1e-11 * np.linalg.norm(dx2)
+    assert abs(err - err2) < 1e-11 * abs(err2)
+
+
+@pytest.mark.parametrize("quirks,crg", [(True, False), (True, True), (False, True)])
+def test_recurrence_equals_reference_iteration_to_rounding(quirks, crg):
+    """r = A^T s and r + alpha A^T t are the same vector in exact arithmetic:
+    a few iterations on a well-conditioned problem agree to rounding."""
+    p = problem(6, crg=crg, wl=True)
+    a = run_oracle(p, 1.0, 6, 1e-30, quirks)
+    b = run_oracle(p, 1.0, 6, 1e-30, quirks, recurrence=True)
+    assert np.linalg.norm(a[0] - b[0]) < 1e-12 * np.linalg.norm(a[0])
+    assert abs(a[1] - b[1]) < 1e-10 * a[1]
 
 
 def test_quirks_inert_when_weights_are_identity():

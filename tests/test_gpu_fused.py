@@ -1,0 +1,139 @@
+"""The single-read CG iteration (one fused pass over G per iteration) against
+the CPU oracle and against the two-pass iteration.
+
+The fused iteration updates the gradient by the recurrence r <- r + alpha A^T t
+instead of recomputing r = A^T s, so it agrees with the reference's iteration
+to rounding, not bit for bit; CG amplifies that rounding with the iteration
+count and the conditioning (on the cross-gradient problem below, a 1e-16
+relative perturbation of G moves dx by 1e-13 after 20 iterations and by
+3e-10 after 30).  Bars after 20 FP64 iterations: dx <= 1e-10 relative to the
+oracle run with the same recurrence (the bar of the two-pass path against the
+oracle's reference iteration) and <= 1e-8 relative to the oracle's reference
+iteration; the same iteration counts, bit-for-bit repeatable results, and the
+fused product actually launched once per iteration.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from graada3d_amd.engine import TIME_FUSED, TIME_GEMV_N
+from helpers import rel_norm
+from test_gpu_cg import _cg_inputs, small_inversion
+
+pytestmark = pytest.mark.gpu
+
+FUSED_TOL = 1e-10       # against the oracle's recurrence
+REF_TOL = 1e-8          # against the oracle's reference iteration
+
+
+def _solve(inv, aux, quirks, WL, fused, its=20, lam=3.0, args=None):
+    mk, mref, Wd, P = args if args is not None else _cg_inputs(inv, aux)[:4]
+    kw = {f"WL_{k}": v for k, v in WL.items()}
+    inv.ctx.timing(True)
+    inv.ctx.timing_reset()
+    out = inv.solve_cg(1e-30, its, lam, mk, mref, Wd, aux["dobs"], P, quirks=quirks,
+                       fused=fused, **kw)
+    n_fused = inv.ctx.timing_get(TIME_FUSED)[1]
+    n_gemv = inv.ctx.timing_get(TIME_GEMV_N)[1]
+    inv.ctx.timing(False)
+    return out, (mk, mref, Wd, P), n_fused, n_gemv
+
+
+def _oracle(inv, G, aux, args, quirks, WL, crg, its=20, lam=3.0, recurrence=True):
+    mk, mref, Wd, P = args
+    crg_d = None
+    if crg:
+        Tx, Ty, Tz = aux["T"]
+        crg_d = dict(a=inv.a_crg, S=inv.S_crg, Tx=Tx, Ty=Ty, Tz=Tz)
+    return oracle.solve_cg(G, 1e-30, its, lam, mk, mref, Wd, aux["dobs"], P, aux["Ws"],
+                           aux["Wx"], aux["Wy"], aux["Wz"], WL=WL, crg=crg_d, quirks=quirks,
+                           recurrence=recurrence)
+
+
+@pytest.mark.parametrize("quirks,wl,crg", [(True, False, False), (True, True, False),
+                                           (False, True, False), (True, True, True),
+                                           (False, True, True)])
+def test_fused_solve_cg_vs_oracle(ctx, quirks, wl, crg):
+    inv, G, aux = small_inversion(ctx, crg=crg)
+    WL = {}
+    if wl:
+        rng = np.random.default_rng(7)
+        WL = {k: rng.uniform(0.5, 2.0, inv.Nm) for k in ("s", "x", "y", "z", "tx", "ty", "tz")}
+    (dx, err, its), args, n_fused, n_gemv = _solve(inv, aux, quirks, WL, fused=True)
+    assert n_fused == 20 and n_gemv == 1          # 1 G*m_k before the loop
+    dxo, erro, itso = _oracle(inv, G, aux, args, quirks, WL, crg)
+    assert its == itso == 20
+    assert rel_norm(dx, dxo) < FUSED_TOL
+    assert abs(err - erro) < 1e-8 * abs(erro)
+    dxr, _, _ = _oracle(inv, G, aux, args, quirks, WL, crg, recurrence=False)
+    assert rel_norm(dx, dxr) < REF_TOL
+    (dx2, err2, _), _, _, _ = _solve(inv, aux, quirks, WL, fused=True, args=args)
+    assert np.array_equal(dx, dx2) and err == err2
+
+
+@pytest.mark.parametrize("wgs", [1, 3, 7, 64, 0])
+def test_fused_grid_shapes(ctx, wgs):
+    """Row groups from 1 (one workgroup streams every row block of its tile)
+    to the default; more tiles than rows per group and the reverse."""
+    inv, G, aux = small_inversion(ctx, nx=20, ny=18, nz=9, npt=25, flag=0x3FE)
+    try:
+        ctx.tune(fused_wgs=wgs * ((inv.Nm + 255) // 256) if wgs else 0)
+        (dx, err, its), args, n_fused, _ = _solve(inv, aux, True, {}, fused=True, its=12)
+    finally:
+        ctx.tune(fused_wgs=0)
+    assert n_fused == 12
+    dxo, erro, itso = _oracle(inv, G, aux, args, True, {}, False, its=12)
+    assert its == itso == 12
+    assert rel_norm(dx, dxo) < FUSED_TOL
+
+
+@pytest.mark.parametrize("store", ["f32", "bf16"])
+def test_fused_reduced_storage_matches_two_pass(ctx, store):
+    """With G stored in FP32/BF16 both iterations read the same stored values
+    and accumulate in FP64: they agree to FP64 rounding of the recurrence."""
+    inv, G, aux = small_inversion(ctx, nx=20, ny=18, nz=9, npt=15)
+    inv.compute_G(store)
+    try:
+        (dx_f, err_f, _), args, n_fused, _ = _solve(inv, aux, True, {}, fused=True, its=20)
+        (dx_2, err_2, _), _, n_f2, _ = _solve(inv, aux, True, {}, fused=False, its=20, args=args)
+    finally:
+        inv.compute_G("f64")
+    assert n_fused == 20 and n_f2 == 0
+    assert rel_norm(dx_f, dx_2) < REF_TOL
+
+
+def test_fused_tolerance_stop(ctx):
+    inv, G, aux = small_inversion(ctx)
+    mk, mref, Wd, P, _ = _cg_inputs(inv, aux)
+    dx, err, its = inv.solve_cg(1e-5, 500, 1.0, mk, mref, Wd, aux["dobs"], P, fused=True)
+    dxo, erro, itso = oracle.solve_cg(G, 1e-5, 500, 1.0, mk, mref, Wd, aux["dobs"], P, aux["Ws"],
+                                      aux["Wx"], aux["Wy"], aux["Wz"], recurrence=True)
+    assert its == itso < 500 and err < 1e-5
+    assert rel_norm(dx, dxo) < 1e-9
+
+
+def test_fused_gauss_newton_l1_irls_vs_oracle(ctx):
+    inv, G, aux = small_inversion(ctx)
+    nm = inv.Nm
+    lo, hi = np.full(nm, -1000.0), np.full(nm, 1000.0)
+    Wd = np.full(inv.Nd, 1.0 / (0.01 * np.abs(aux["dobs"]).max()))
+    inv.gn_setup(aux["dobs"], Wd, lo, hi, np.zeros(nm), np.zeros(nm), lp=1, eps2=1e-3,
+                 fused=True)
+    m = np.zeros(nm)
+    WL = {}
+    for k in range(3):
+        lam = 50.0 * 0.5 ** k
+        mis, err, its = inv.gn_step(lam, 0.0, 15)
+        P = oracle.barrier_P(m, lo, hi)
+        dxo, erro, itso = oracle.solve_cg(G, 0.0, 15, lam, m, np.zeros(nm), Wd, aux["dobs"], P,
+                                          aux["Ws"], aux["Wx"], aux["Wy"], aux["Wz"], WL=WL,
+                                          quirks=True, recurrence=True)
+        m = oracle.barrier_update(m, dxo, lo, hi, m.copy())
+        miso = oracle.misfit(G, m, Wd, aux["dobs"])
+        WL = {"s": oracle.irls_weights(aux["Ws"], m, 1, 1e-3),
+              "x": oracle.irls_weights(aux["Wx"], m, 1, 1e-3),
+              "y": oracle.irls_weights(aux["Wy"], m, 1, 1e-3),
+              "z": oracle.irls_weights(aux["Wz"], m, 1, 1e-3)}
+        assert its == itso == 15
+        assert abs(mis - miso) < 1e-10 * miso, k
+        assert rel_norm(inv.gn_model(), m) < 1e-10, k
