@@ -363,6 +363,28 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
     return GRA_OK;
 }
 
+gra_status gra_col_norms(gra_ctx *ctx, double *norms)
+{
+    CTX_GUARD(ctx);
+    if (!norms) return set_error(c, GRA_BAD_ARG, "null output");
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    DevBuf &y = c.vec_nd[2];
+    HIPCHK(y.reserve(sizeof(double) * (size_t)c.Nm), "alloc out");
+    const GemvShape shape = gemv_shape(c);
+    HIPCHK(c.part_t.reserve(sizeof(double) * (size_t)shape.t_chunks * c.ld), "alloc part");
+    HIPCHK(launch_col_sumsq(c.G.p, c.store, c.Nd, c.ld, shape, c.part_t.as<double>(), c.stream),
+           "column sums of squares");
+    HIPCHK(launch_reduce_parts(c.part_t.as<double>(), shape.t_chunks, c.ld, c.Nm, nullptr,
+                               y.as<double>(), c.stream),
+           "reduce");
+    HIPCHK(hipMemcpyAsync(norms, y.p, sizeof(double) * (size_t)c.Nm, hipMemcpyDeviceToHost,
+                          c.stream),
+           "download");
+    HIPCHK(hipStreamSynchronize(c.stream), "col_norms");
+    for (int64_t j = 0; j < c.Nm; ++j) norms[j] = std::sqrt(norms[j]);
+    return GRA_OK;
+}
+
 gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product)
 {
     return gemv_host(ctx, 0, vec, product);

@@ -164,6 +164,8 @@ hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, co
                          const double *x, double *part, const int *done, hipStream_t st);
 hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, const GemvShape &s,
                          const double *w, double *part, const int *done, hipStream_t st);
+hipError_t launch_col_sumsq(const void *G, gra_dtype dt, int64_t Nd, int64_t ld,
+                            const GemvShape &s, double *part, hipStream_t st);
 // y[i] = scale_i * sum_s part[s*stride + i]  (scale may be null => 1)
 hipError_t launch_reduce_parts(const double *part, int nparts, int64_t stride, int64_t n,
                                const double *scale, double *y, hipStream_t st);

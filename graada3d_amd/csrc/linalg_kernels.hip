@@ -146,7 +146,8 @@ gemv_n_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t cols_per_
 }
 
 // ---------------------------------------------------------------- G^T w
-template <typename T, int UR, bool NT>
+// SQ: column sums of squares instead (w unused).
+template <typename T, int UR, bool NT, bool SQ = false>
 __global__ void __launch_bounds__(kBlk)
 gemv_t_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t rows_per_chunk,
               const double *__restrict__ w, double *__restrict__ part, const int *done)
@@ -170,19 +171,19 @@ gemv_t_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t rows_per_
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
             Load16<T, NT>::run(g + (r + u) * ld, gv[u]);
-            wv[u] = w[r + u];
+            wv[u] = SQ ? 0.0 : w[r + u];
         }
 #pragma unroll
         for (int u = 0; u < UR; ++u)
 #pragma unroll
-            for (int q = 0; q < V; ++q) acc[q] = fma(gv[u][q], wv[u], acc[q]);
+            for (int q = 0; q < V; ++q) acc[q] = fma(gv[u][q], SQ ? gv[u][q] : wv[u], acc[q]);
     }
     for (; r < r1; ++r) {
         double gv[V];
         Load16<T, NT>::run(g + r * ld, gv);
-        double w0 = w[r];
+        double w0 = SQ ? 0.0 : w[r];
 #pragma unroll
-        for (int q = 0; q < V; ++q) acc[q] = fma(gv[q], w0, acc[q]);
+        for (int q = 0; q < V; ++q) acc[q] = fma(gv[q], SQ ? gv[q] : w0, acc[q]);
     }
     double *out = part + (int64_t)chunk * ld + c;
 #pragma unroll
@@ -324,6 +325,32 @@ hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, co
     case GRA_F64: launch_t_shape<double>(G, Nd, ld, s, w, part, done, st); break;
     case GRA_F32: launch_t_shape<float>(G, Nd, ld, s, w, part, done, st); break;
     default: launch_t_shape<__hip_bfloat16>(G, Nd, ld, s, w, part, done, st); break;
+    }
+    return hipGetLastError();
+}
+
+// Column sums of squares of G into `part` ([t_chunks][ld] partials, summed in
+// chunk order by the consumer).
+hipError_t launch_col_sumsq(const void *G, gra_dtype dt, int64_t Nd, int64_t ld,
+                            const GemvShape &s, double *part, hipStream_t st)
+{
+    int V = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
+    int64_t bx = (ld + kBlk * V - 1) / (kBlk * V);
+    int64_t rpc = (Nd + s.t_chunks - 1) / s.t_chunks;
+    dim3 grid((unsigned)bx, (unsigned)s.t_chunks), block(kBlk);
+    switch (dt) {
+    case GRA_F64:
+        hipLaunchKernelGGL((gemv_t_kernel<double, 8, true, true>), grid, block, 0, st,
+                           (const double *)G, Nd, ld, rpc, nullptr, part, nullptr);
+        break;
+    case GRA_F32:
+        hipLaunchKernelGGL((gemv_t_kernel<float, 8, true, true>), grid, block, 0, st,
+                           (const float *)G, Nd, ld, rpc, nullptr, part, nullptr);
+        break;
+    default:
+        hipLaunchKernelGGL((gemv_t_kernel<__hip_bfloat16, 8, true, true>), grid, block, 0, st,
+                           (const __hip_bfloat16 *)G, Nd, ld, rpc, nullptr, part, nullptr);
+        break;
     }
     return hipGetLastError();
 }

@@ -176,6 +176,14 @@ class Fwd:
     def compute_gobs(self, rho):
         return self.G_vec_mul(rho)
 
+    def col_norms(self):
+        """||G(:, j)||_2 per column (the reads of set_depth_weighting(beta, 1),
+        InversionBase.cpp:519-526)."""
+        self._need_G()
+        out = np.empty(self.Nm)
+        self.ctx.check(self.ctx.L.gra_col_norms(self.ctx.h, _p(out)), "col_norms")
+        return out
+
     def get_G(self, col_major: bool = False):
         """Host copy of G as float64 (Nd x Nm)."""
         self._need_G()

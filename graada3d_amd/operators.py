@@ -65,6 +65,12 @@ def depth_weights_reference(prisms: np.ndarray, points: np.ndarray, beta: float)
     return 1.0 / np.power(np.abs(zp - zc), beta * 0.5)
 
 
+def depth_weights_sensitivity(col_norms: np.ndarray, Nd: int) -> np.ndarray:
+    """Z_ii = sqrt(||G(:, i)||_2 / Nd)  (set_depth_weighting flag 1,
+    InversionBase.cpp:519-526); col_norms from Fwd.col_norms()."""
+    return np.sqrt(np.asarray(col_norms, float) / Nd)
+
+
 def depth_weights_offset(prisms: np.ndarray, z0: float, beta: float) -> np.ndarray:
     """Z_ii = (z_c + z0)^(-beta/2), the explicit depth weighting of the
     benchmark inversions."""

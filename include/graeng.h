@@ -101,6 +101,13 @@ gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product);
 gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product);
 
 /*
+ * Column 2-norms of the stored G: norms[j] = ||G(:, j)||_2, j < Nm.  Replaces
+ * the G.col(i).norm() reads of InversionBase::set_depth_weighting(beta, 1)
+ * (InversionBase.cpp:519-526), whose weight is sqrt(norm / Nd).
+ */
+gra_status gra_col_norms(gra_ctx *ctx, double *norms);
+
+/*
  * Replaces the matrix-free forward loop of compute.cpp:246-260:
  * out[i*n_comp + k] = sum_j field_k(point i, prism j, rho_j), no G stored.
  */

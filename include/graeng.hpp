@@ -6,6 +6,9 @@
 //     G_vec_mul(vec, product)        Fwd::G_vec_mul          (Fwd.cpp:114-147)
 //     GT_vec_mul(vec, product)       Fwd::GT_vec_mul         (Fwd.cpp:80-113)
 //     compute_gobs(rho)              Fwd::compute_gobs       (Fwd.h:23-27)
+//     col_norms(norms)               G.col(i).norm() reads of
+//                                    InversionBase::set_depth_weighting(beta, 1)
+//                                    (InversionBase.cpp:519-526)
 //   graeng::GaussNewtonInversion::solve_cg
 //                               <-> GaussNewtonInversion::solve_cg (GaussNewtonInversion.h:70-93)
 //
@@ -133,6 +136,11 @@ public:
         if ((int64_t)vec.size() != Nd) throw Error(GRA_BAD_ARG, "GT_vec_mul: size");
         product.resize(Nm);
         check(ctx_.get(), gra_GT_vec_mul(ctx_.get(), vec.data(), product.data()), "GT_vec_mul");
+    }
+    template <class V> void col_norms(V &norms) const
+    {
+        norms.resize(Nm);
+        check(ctx_.get(), gra_col_norms(ctx_.get(), norms.data()), "col_norms");
     }
     template <class V> V compute_gobs(const V &rho) const
     {

@@ -75,9 +75,10 @@ int main(int argc, char **argv)
         graeng::Context ctx(0);
         graeng::GaussNewtonInversion inv(ctx, prisms, points, flag);
         inv.compute_G();
-        std::vector<double> Gx, GTy;
+        std::vector<double> Gx, GTy, norms;
         inv.G_vec_mul(x, Gx);
         inv.GT_vec_mul(y, GTy);
+        inv.col_norms(norms);
         graeng::Csr I = diag(std::vector<double>(Nm, 1.0)), Wdm = diag(Wd), Pm = diag(P);
         double err = 0.0;
         int its = 0;
@@ -87,6 +88,7 @@ int main(int argc, char **argv)
         fwrite(Gx.data(), sizeof(double), Gx.size(), o);
         fwrite(GTy.data(), sizeof(double), GTy.size(), o);
         fwrite(dx.data(), sizeof(double), dx.size(), o);
+        fwrite(norms.data(), sizeof(double), norms.size(), o);
         double tail[2] = {err, (double)its};
         fwrite(tail, sizeof(double), 2, o);
         fclose(o);

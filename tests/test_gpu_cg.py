@@ -56,6 +56,22 @@ def test_gemv_vs_oracle(ctx):
     assert np.array_equal(a, inv.G_vec_mul(x))           # repeatable
 
 
+@pytest.mark.parametrize("store", ["f64", "f32", "bf16"])
+def test_col_norms_vs_numpy(ctx, store):
+    """||G(:, j)||_2 of the stored values (set_depth_weighting flag 1)."""
+    inv, G, _ = small_inversion(ctx, nx=20, ny=18, nz=9, npt=15)
+    inv.compute_G(store)
+    try:
+        Gs = inv.get_G()
+        n = inv.col_norms()
+    finally:
+        inv.compute_G("f64")
+    ref = np.linalg.norm(Gs, axis=0)
+    assert np.abs(n / ref - 1.0).max() < 1e-14
+    if store == "f64":
+        assert np.array_equal(Gs, G)
+
+
 def _cg_inputs(inv, aux, wl=False):
     nm, nd = inv.Nm, inv.Nd
     rng = aux["rng"]

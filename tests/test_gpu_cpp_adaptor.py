@@ -53,9 +53,11 @@ def test_cpp_adaptor_matches_oracle(tmp_path):
     assert r.returncode == 0, r.stderr
     out = np.frombuffer(fout.read_bytes(), np.float64)
     Gx, GTy, dx = out[:nd], out[nd:nd + nm], out[nd + nm:nd + 2 * nm]
+    norms = out[nd + 2 * nm:nd + 3 * nm]
     err, its = out[-2], int(out[-1])
     assert rel_norm(Gx, G @ x) < 1e-12
     assert rel_norm(GTy, G.T @ y) < 1e-12
+    assert np.abs(norms / np.linalg.norm(G, axis=0) - 1.0).max() < 1e-14
     dxo, erro, itso = oracle.solve_cg(G, tol, maxit, lam, mk, mref, Wd, dobs, P, Ws, Wx, Wy, Wz)
     assert its == itso == maxit
     assert rel_norm(dx, dxo) < 1e-10
