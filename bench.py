@@ -13,7 +13,10 @@ Workloads
 
 One step = one Gauss-Newton outer iteration on the GPU (barrier P, solve_cg =
 CGLS with two passes over G per iteration, barrier model update, misfit, IRLS
-weights).  `value` is CG iterations per second of the plain FP64 path.  The
+weights).  Before the timed steps of each storage type the G-product launch
+shapes are chosen by gra_autotune (timed back-to-back product pairs on the
+resident G; its time is reported, not counted).  `value` is CG iterations
+per second of the plain FP64 path.  The
 same steps are then run with G stored in FP32 and BF16 (FP64 accumulation),
 and with the single-read iteration (one fused pass over G per CG iteration)
 for every storage type, and reported as speed-up and error against the plain
@@ -102,6 +105,12 @@ def run(args):
         ctx.timing(False)
         asm_ms = asm[store]
         inv.set_regularization(*pb["W"])
+        ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1)
+        tune_s = 0.0
+        if args.autotune and not fused:
+            t0 = time.perf_counter()
+            ctx.autotune()
+            tune_s = time.perf_counter() - t0
 
         def setup():
             inv.gn_setup(dobs, Wd, pb["m_min"], pb["m_max"], pb["mref"], pb["m_ini"], pb["lp"],
@@ -137,7 +146,7 @@ def run(args):
             misfit=mis[-1], model=inv.gn_model(),
             asm_ms=asm_ms, entries_per_s=nd * nm / (asm_ms * 1e-3),
             gemv_n=kern(gn_ms, gn_c), gemv_t=kern(gt_ms, gt_c), fused_product=kern(fu_ms, fu_c),
-            tuning=ctx.tuning())
+            tuning=ctx.tuning(), autotune_s=tune_s)
         if fused and fu_c == 0:
             raise RuntimeError("the fused product did not run (grid not co-resident?)")
         print(f"[bench] {key}: {results[key]['iter_s']:.1f} CG iter/s, assembly "
@@ -195,6 +204,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--stores", default="f64,f32,bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="use the default G-product launch shapes instead of gra_autotune's")
     ap.add_argument("--no-fused", dest="fused", action="store_false",
                     help="skip the single-read (fused) CG iteration variants")
     ap.add_argument("--cpu-iters", type=int, default=40)
@@ -272,6 +283,7 @@ def main():
                                   "misfit_f64| / misfit_f64 after the same steps, against the "
                                   "plain FP64 GPU run (itself checked against the CPU oracle)",
         "tuning": ref["tuning"],
+        "autotune_s": ref["autotune_s"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

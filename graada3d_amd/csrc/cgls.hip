@@ -202,6 +202,61 @@ __global__ void __launch_bounds__(kB) k_r_update(CgDev d)
     }
 }
 
+// Loop residual r = A^T s with eight lanes per cell j: lane 0 sums the data
+// part's chunk partials, lane 1+q forms segment q's transposed row product;
+// lane 0 then adds them in the order of the single-lane form
+// r = ((P_j g_j + v_0) + v_1) + ..., so results are identical to it while the
+// sparse-row latency chains run in parallel.  FUSED: the single-read
+// iteration's r <- base + alpha * (P_j z_j + sum_q v_q) over t instead of s,
+// z_j the sum of the fused product's row-group partials; in the first
+// iteration the base is the loop-operator residual of s0 when it differs from
+// the initial one (reference quirk at GN.cpp:1508).
+constexpr int kLanesPerCell = 8;
+
+template <bool FUSED, bool FIRST>
+__global__ void __launch_bounds__(kB) k_r_loop(CgDev d)
+{
+    if (d.sc->done) return;
+    const int sub = threadIdx.x & (kLanesPerCell - 1);
+    const int64_t j = ((int64_t)blockIdx.x * kB + threadIdx.x) / kLanesPerCell;
+    const bool ok = j < d.nm;
+    double v = 0.0;
+    if (ok) {
+        if (sub == 0) {
+            if (FUSED) {
+                v = d.zpart[j];
+                for (int g = 1; g < d.zgroups; ++g) v += d.zpart[(int64_t)g * d.ld + j];
+            } else {
+                v = d.part_t[j];
+                for (int c = 1; c < d.chunks; ++c) v += d.part_t[(int64_t)c * d.ld + j];
+            }
+        } else if (sub <= d.nseg) {
+            const int q = sub - 1;
+            const double *seg = (FUSED ? d.t : d.s) + d.nd;
+            double pre = q < 4 ? d.sl : 1.0;
+            v = row_dot(d.AT[q], j, pre, d.P, d.WL[q], seg + (int64_t)q * d.nm);
+        }
+    }
+    // lane group of cell j: lanes sub = 0..7 of the same wave
+    const int lane = threadIdx.x & 63, base_lane = lane & ~(kLanesPerCell - 1);
+    double vq[kLanesPerCell];
+#pragma unroll
+    for (int q = 0; q < kLanesPerCell; ++q) vq[q] = __shfl(v, base_lane + q, 64);
+    double r2 = 0.0;
+    if (ok && sub == 0) {
+        double r = d.P[j] * vq[0];
+        for (int q = 0; q < d.nseg; ++q) r = r + vq[1 + q];
+        if (FUSED) {
+            double b = (FIRST && d.rloop) ? d.rloop[j] : d.r[j];
+            r = b + d.sc->alpha * r;
+        }
+        d.r[j] = r;
+        r2 = r * r;
+    }
+    double s = block_sum(r2);
+    if (threadIdx.x == 0) d.red[blockIdx.x] = s;
+}
+
 // After the initial residual: r2, ATb2.
 __global__ void __launch_bounds__(kB) k_scalar_init(CgDev d, int64_t nblk)
 {
@@ -335,34 +390,6 @@ __global__ void __launch_bounds__(kB) k_alpha_fused(CgDev d, int64_t nblk)
         d.sc->alpha = a;
         if (!(t2 != 0.0) || !isfinite(a)) d.sc->breakdown = 1;
     }
-}
-
-// r <- r + alpha A_r^T t: data part P_j (sum over row groups of the fused
-// G^T (Wd^2 G Pp)), regularisation part with the loop operator's weights.
-// In the first iteration the base is the loop-operator residual of s0 when it
-// differs from the initial one (reference quirk at GN.cpp:1508).
-template <bool FIRST>
-__global__ void __launch_bounds__(kB) k_q_update(CgDev d)
-{
-    if (d.sc->done) return;
-    int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double r2 = 0.0;
-    if (j < d.nm) {
-        double z = d.zpart[j];
-        for (int g = 1; g < d.zgroups; ++g) z += d.zpart[(int64_t)g * d.ld + j];
-        double q = d.P[j] * z;
-        const double *seg = d.t + d.nd;
-        for (int k = 0; k < d.nseg; ++k) {
-            double pre = k < 4 ? d.sl : 1.0;
-            q = q + row_dot(d.AT[k], j, pre, d.P, d.WL[k], seg + (int64_t)k * d.nm);
-        }
-        double base = (FIRST && d.rloop) ? d.rloop[j] : d.r[j];
-        double r = base + d.sc->alpha * q;
-        d.r[j] = r;
-        r2 = r * r;
-    }
-    double s = block_sum(r2);
-    if (threadIdx.x == 0) d.red[blockIdx.x] = s;
 }
 
 // T*_scaled = (a*S_i) * T0_ij, and for the transpose (a*S_j) * T0_ji.
@@ -526,7 +553,8 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     if ((e = c.part_t.reserve(sizeof(double) * (size_t)chunks * ld)) != hipSuccess)
         return hip_fail(c, e, "alloc part_t");
     int64_t nblk_row = nblocks(nrow), nblk_nm = nblocks(nm);
-    int64_t nred = nblk_row > nblk_nm ? nblk_row : nblk_nm;
+    const int64_t nblk_r = nblocks(nm * kLanesPerCell);   // k_r_loop blocks
+    int64_t nred = nblk_row > nblk_r ? nblk_row : nblk_r;
     if ((e = c.red.reserve(sizeof(double) * 2 * (size_t)nred)) != hipSuccess)
         return hip_fail(c, e, "alloc red");
     if ((e = c.scal.reserve(sizeof(CgScalars))) != hipSuccess) return hip_fail(c, e, "alloc sc");
@@ -640,8 +668,8 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
                                   st);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
             }
-            hipLaunchKernelGGL(k_r_update<0>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
-            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_nm, it, a.tol);
+            hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_r, it, a.tol);
             if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
         }
     } else {
@@ -663,10 +691,12 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
             hipLaunchKernelGGL(k_alpha_fused, dim3(1), dim3(kB), 0, st, d, nblk_reg);
             hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
             if (it == 0)
-                hipLaunchKernelGGL(k_q_update<true>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+                hipLaunchKernelGGL((k_r_loop<true, true>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
+                                   d);
             else
-                hipLaunchKernelGGL(k_q_update<false>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
-            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_nm, it, a.tol);
+                hipLaunchKernelGGL((k_r_loop<true, false>), dim3((unsigned)nblk_r), dim3(kB), 0,
+                                   st, d);
+            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_r, it, a.tol);
             if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
         }
     }

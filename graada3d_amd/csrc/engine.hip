@@ -768,6 +768,113 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
     return GRA_OK;
 }
 
+// Average duration (ms) of G x and of G^T w over `reps` alternating launches
+// issued back to back on the context stream, as the CG iteration issues them.
+static hipError_t time_pair(Ctx &c, const GemvShape &s, int reps, const double *x,
+                            const double *w, double *ms_n, double *ms_t)
+{
+    std::vector<hipEvent_t> ev(3 * (size_t)reps, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto &v : ev)
+        if ((e = hipEventCreate(&v)) != hipSuccess) break;
+    if (e == hipSuccess) {
+        // one untimed pair first
+        e = launch_gemv_n(c.G.p, c.store, c.Nd, c.ld, s, x, c.part_n.as<double>(), nullptr, c.stream);
+        if (e == hipSuccess)
+            e = launch_gemv_t(c.G.p, c.store, c.Nd, c.ld, s, w, c.part_t.as<double>(), nullptr,
+                              c.stream);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) {
+            if ((e = hipEventRecord(ev[3 * r], c.stream)) != hipSuccess) break;
+            e = launch_gemv_n(c.G.p, c.store, c.Nd, c.ld, s, x, c.part_n.as<double>(), nullptr,
+                              c.stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 1], c.stream);
+            if (e == hipSuccess)
+                e = launch_gemv_t(c.G.p, c.store, c.Nd, c.ld, s, w, c.part_t.as<double>(), nullptr,
+                                  c.stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 2], c.stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    double tn = 0.0, tt = 0.0;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        float a = 0.f, b = 0.f;
+        if ((e = hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1])) != hipSuccess) break;
+        if ((e = hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2])) != hipSuccess) break;
+        tn += a;
+        tt += b;
+    }
+    for (auto &v : ev)
+        if (v) (void)hipEventDestroy(v);
+    *ms_n = tn / reps;
+    *ms_t = tt / reps;
+    return e;
+}
+
+gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t)
+{
+    CTX_GUARD(ctx);
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    const double gbytes = (double)c.Nd * (double)c.Nm * (c.store == GRA_F64 ? 8 : c.store == GRA_F32 ? 4 : 2);
+    if (reps <= 0) reps = gbytes < 8e9 ? 6 : 2;
+    c.partn_holds_gn_model = false;
+    constexpr int kMaxSplits = 16, kMaxChunks = 32;
+    HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)kMaxSplits * c.Nd), "alloc part_n");
+    HIPCHK(c.part_t.reserve(sizeof(double) * (size_t)kMaxChunks * c.ld), "alloc part_t");
+    DevBuf &x = c.vec_nd[1], &w = c.vec_nd[2];
+    HIPCHK(x.reserve(sizeof(double) * (size_t)c.ld), "alloc x");
+    HIPCHK(w.reserve(sizeof(double) * (size_t)(c.Nd > c.ld ? c.Nd : c.ld)), "alloc w");
+    HIPCHK(hipMemsetAsync(x.p, 0, x.bytes, c.stream), "clear x");
+    HIPCHK(hipMemsetAsync(w.p, 0, w.bytes, c.stream), "clear w");
+    const double *xp = x.as<double>(), *wp = w.as<double>();
+
+    GemvShape best = gemv_shape(c);
+    double bn = 0.0, bt = 0.0;
+    HIPCHK(time_pair(c, best, reps, xp, wp, &bn, &bt), "time default");
+    // G^T w: row chunks x rows per iteration
+    const int64_t max_chunks = c.Nd / 32 > 0 ? c.Nd / 32 : 1;
+    for (int ch : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16, 24, 32}) {
+        if (ch > max_chunks) break;
+        for (int un : {4, 8}) {
+            GemvShape s = best;
+            s.t_chunks = ch;
+            s.t_unroll = un;
+            double tn, tt;
+            HIPCHK(time_pair(c, s, reps, xp, wp, &tn, &tt), "time G^T w");
+            if (tt < bt) { best = s; bt = tt; bn = tn; }
+        }
+    }
+    // G x: rows per wave x column steps x column splits
+    const int V = c.store == GRA_F64 ? 2 : (c.store == GRA_F32 ? 4 : 8);
+    const int64_t max_splits = c.Nm / (4 * V * 64) > 0 ? c.Nm / (4 * V * 64) : 1;
+    for (int rows : {2, 4, 8}) {
+        if (c.store == GRA_F64 && rows == 8) continue;
+        for (int un : {1, 2})
+            for (int sp : {1, 2, 4, 8, 16}) {
+                if (sp > max_splits) continue;
+                GemvShape s = best;
+                s.n_rows = rows;
+                s.n_unroll = un;
+                s.n_splits = sp;
+                double tn, tt;
+                HIPCHK(time_pair(c, s, reps, xp, wp, &tn, &tt), "time G x");
+                if (tn < bn) { best = s; bn = tn; bt = tt; }
+            }
+    }
+    // non-temporal loads: judged on the pair (they interact across kernels)
+    for (int nt = 0; nt <= 3; ++nt) {
+        if (nt == best.nontemporal) continue;
+        GemvShape s = best;
+        s.nontemporal = nt;
+        double tn, tt;
+        HIPCHK(time_pair(c, s, reps, xp, wp, &tn, &tt), "time non-temporal");
+        if (tn + tt < bn + bt) { best = s; bn = tn; bt = tt; }
+    }
+    c.tune_override = best;
+    if (ms_n) *ms_n = bn;
+    if (ms_t) *ms_t = bt;
+    return GRA_OK;
+}
+
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
 {
     CTX_GUARD(ctx);

@@ -110,6 +110,14 @@ class Context:
         for k, v in kw.items():
             self.check(self.L.gra_tune_set(self.h, _lib.TUNE[k], int(v)), f"tune {k}")
 
+    def autotune(self, reps: int = 0):
+        """Pick the G-product launch shapes for the current G by timing
+        back-to-back G x / G^T w pairs; returns (ms G x, ms G^T w)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self.check(self.L.gra_autotune(self.h, int(reps), ctypes.byref(a), ctypes.byref(b)),
+                   "autotune")
+        return a.value, b.value
+
     def tuning(self):
         out = {}
         for k, p in _lib.TUNE.items():

@@ -232,6 +232,17 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
                                    capped at what can be co-resident      */
 gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
+/*
+ * Autotuner for the context's current G (no counterpart in the reference):
+ * times `reps` (<= 0: automatic) alternating G x / G^T w launches issued back
+ * to back, as the CG iteration issues them, for candidate shapes (coordinate
+ * descent: G^T w row chunks and unroll, then G x rows / unroll / splits, then
+ * the non-temporal mask, judged on the pair) and keeps the fastest as the
+ * context's overrides (gra_tune_get reads them; gra_tune_set(..., 0) resets).
+ * Writes the winners' average launch times in ms.  Overwrites the product
+ * scratch buffers, so a Gauss-Newton step after it recomputes G m_k.
+ */
+gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t);
 
 /* ---------------------------------------------------------- measurement */
 /*

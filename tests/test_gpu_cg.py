@@ -183,6 +183,22 @@ def test_launch_shapes_do_not_change_results_beyond_rounding(ctx):
     inv.ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1)
 
 
+def test_autotune_keeps_results(ctx):
+    inv, G, _ = small_inversion(ctx, nx=20, ny=18, nz=9, npt=15)
+    rng = np.random.default_rng(3)
+    x, y = rng.standard_normal(inv.Nm), rng.standard_normal(inv.Nd)
+    try:
+        ms_n, ms_t = inv.ctx.autotune(reps=2)
+        t = inv.ctx.tuning()
+        assert ms_n > 0 and ms_t > 0
+        assert t["n_rows"] in (2, 4, 8) and t["n_unroll"] in (1, 2) and 1 <= t["n_splits"] <= 16
+        assert 1 <= t["t_chunks"] <= 32 and t["t_unroll"] in (4, 8) and 0 <= t["nontemporal"] <= 3
+        assert rel_norm(inv.G_vec_mul(x), oracle.gemv(G, x)) < 1e-14
+        assert rel_norm(inv.GT_vec_mul(y), oracle.gemv(G, y, trans=True)) < 1e-14
+    finally:
+        inv.ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1)
+
+
 def test_helpers_vs_oracle(ctx):
     inv, G, aux = small_inversion(ctx)
     nm = inv.Nm
