@@ -407,7 +407,7 @@ gra_status gra_forward(gra_ctx *ctx, const double *prisms, const double *rho, in
         return set_error(c, GRA_BAD_ARG, "prism bounds must be finite with x0<x1, y0<y1, z0<z1");
     if (!points_valid(pts, N_obs)) return set_error(c, GRA_BAD_ARG, "non-finite point");
     const int nc = ncomp(field_flag);
-    DevBuf dpr, drho, dpt, dout;
+    DevBuf &dpr = c.fwd_prisms, &drho = c.fwd_rho, &dpt = c.fwd_pts, &dout = c.fwd_out;
     HIPCHK(put(dpr, prisms, (size_t)(6 * Nm), c.stream), "upload prisms");
     HIPCHK(put(drho, rho, (size_t)Nm, c.stream), "upload rho");
     HIPCHK(put(dpt, pts, (size_t)(3 * N_obs), c.stream), "upload points");

@@ -105,6 +105,7 @@ struct Ctx {
     DevBuf part_n, part_t;     // GEMV partial sums
     DevBuf fused_scratch, fused_coef, rloop;
     DevBuf vec_nd[4], vec_nm[12], vec_row[3];
+    DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf red;                // block partial sums for reductions
     DevBuf scal;               // CgScalars
     DevBuf status;             // int flags written by kernels

@@ -1,0 +1,95 @@
+"""The serving harness: seeded request stream, payload shapes, residency LRU
+(CPU), and one short replay on the GPU checked against the oracle."""
+import numpy as np
+import pytest
+
+from graada3d_amd import serving as S
+
+
+def test_stream_is_seeded_and_follows_the_mix():
+    pool = S.default_cg_pool()
+    a = S.request_stream(200.0, 5.0, 11, pool)
+    b = S.request_stream(200.0, 5.0, 11, pool)
+    assert [(r.t_arrival, r.kind, r.params) for r in a] == \
+           [(r.t_arrival, r.kind, r.params) for r in b]
+    n = len(a)
+    assert 850 < n < 1150                                   # Poisson(1000)
+    t = np.array([r.t_arrival for r in a])
+    assert np.all(np.diff(t) > 0) and t[-1] < 5.0
+    fwd = [r for r in a if r.kind == "forward"]
+    assert 0.75 < len(fwd) / n < 0.85
+    cells = np.array([r.params["cells"] for r in fwd])
+    assert cells.min() >= 5000 and cells.max() <= 130000
+    # log-uniform: the median is near the geometric mean sqrt(5e3 * 1.3e5)
+    assert 0.7 < np.median(cells) / np.sqrt(5e3 * 1.3e5) < 1.4
+    for r in fwd:
+        f = r.params["flag"]
+        assert f & 1 == 0 and 0 < f <= 0x3FE
+        assert 1.0 <= r.params["height"] <= 100.0
+
+
+def test_forward_payload_shapes():
+    p = dict(cells=20000, npts=5000, flag=0x12, height=10.0, seed=3)
+    prisms, rho, pts = S.forward_payload(p)
+    assert abs(prisms.shape[0] - 20000) / 20000 < 0.25
+    assert abs(pts.shape[0] - 5000) / 5000 < 0.05
+    assert np.all(pts[:, 2] == -10.0) and np.all(prisms[:, 4] >= 0.0)
+    assert rho.shape == (prisms.shape[0],) and np.all(np.abs(rho) <= 500.0)
+
+
+class _FakeCtx:
+    def __init__(self, log, name):
+        self.log, self.name = log, name
+
+    def close(self):
+        self.log.append(self.name)
+
+
+def test_resident_cache_lru_admission_and_eviction():
+    pool = S.default_cg_pool()
+    closed = []
+
+    class Cache(S.ResidentCache):
+        def _admit(self, prob, need):
+            return dict(ctx=_FakeCtx(closed, prob.name), bytes=need)
+
+    sizes = [S.ResidentCache.g_bytes(p) for p in pool]
+    budget = sizes[0] + sizes[1] + 1.0           # room for problems 0 and 1 only
+    c = Cache(pool, budget)
+    c.get(0)
+    c.get(1)
+    c.get(0)                                     # hit; 1 is now least recent
+    assert c.admissions == 2 and c.evictions == 0
+    c.get(2)                                     # evicts 1 (LRU), maybe 0 too
+    assert closed[0] == pool[1].name
+    assert c.used <= budget
+    assert 2 in c.entries
+    with pytest.raises(MemoryError):
+        S.ResidentCache(pool, 1.0).get(0)
+
+
+@pytest.mark.gpu
+def test_short_replay_on_gpu():
+    import oracle
+    from graada3d_amd.engine import Context, forward
+    pool = [S.CgProblem("tiny", (10, 10, 5), (1000.0, 1000.0, 500.0), 8, 0x12, 1.0)]
+    reqs = S.request_stream(40.0, 0.5, 5, pool)
+    for r in reqs:                               # keep the forward requests small
+        if r.kind == "forward":
+            r.params.update(cells=min(r.params["cells"], 6000), npts=min(r.params["npts"], 1500))
+    res = S.serve(reqs, pool, budget_bytes=1e9)
+    assert res["requests"] == len(reqs) > 5
+    lat, svc = res["latency_ms"], res["service_ms"]
+    assert 0 < svc["p50"] <= lat["p50"] and lat["p99"] <= lat["max"]
+    assert res["cache"]["admissions"] <= 1
+    # the served forward path against the oracle on one request's payload
+    r = next(q for q in reqs if q.kind == "forward")
+    prisms, rho, pts = S.forward_payload(r.params)
+    ctx = Context(0)
+    try:
+        got = forward(prisms, rho, pts[:50], r.params["flag"], ctx=ctx)
+    finally:
+        ctx.close()
+    ref, st = oracle.forward(prisms, rho, pts[:50], r.params["flag"])
+    ref = np.asarray(ref).reshape(got.shape)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
