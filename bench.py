@@ -208,14 +208,15 @@ def run_serving(args):
     reqs = S.request_stream(args.rate, args.duration, args.seed, pool)
     print(f"[bench] serving: {len(reqs)} requests over {args.duration} s at {args.rate}/s",
           file=sys.stderr, flush=True)
-    res = S.serve(reqs, pool, progress=50)
+    res = S.serve(reqs, pool, progress=50, forward_method=args.forward_method)
     cpu = None
     if not args.no_cpu:
         import oracle
         # CPU: the oracle's forward on the first forward request's mesh and
         # components, on a bounded slice of its points, extrapolated per request
         r = next(q for q in reqs if q.kind == "forward")
-        prisms, rho, pts = S.forward_payload(r.params)
+        q = S.forward_payload(r.params)
+        prisms, rho, pts = q["prisms"], q["rho"], q["pts"]
         n = max(1, min(pts.shape[0], int(2e6 // prisms.shape[0])))
         c0 = time.perf_counter()
         oracle.forward(prisms, rho, pts[:n], r.params["flag"])
@@ -237,7 +238,8 @@ def run_serving(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"serving: Poisson {args.rate:g} req/s for {args.duration:g} s "
                                f"(seed {args.seed}), 80% matrix-free forward (5k-130k cells, "
-                               f"1k-20k points, 1-9 comps, FP32 out), 20% 20-iteration CGLS on "
+                               f"1k-20k points, 1-9 comps, FP32 out, {args.forward_method} "
+                               f"form), 20% 20-iteration CGLS on "
                                f"resident G (pool: " + ", ".join(p.name for p in pool) + ")"},
         "latency_ms": res["latency_ms"], "service_ms": res["service_ms"],
         "by_kind": res["by_kind"], "cg_warm_service_ms": res["cg_warm_service_ms"],
@@ -257,6 +259,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: arrival window, s")
     ap.add_argument("--seed", type=int, default=20240605, help="serving: stream seed")
+    ap.add_argument("--forward-method", default="nodes", choices=["nodes", "prisms"],
+                    help="serving: node-sum (rectilinear) or per-prism face/edge forward")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--stores", default="f64,f32,bf16")

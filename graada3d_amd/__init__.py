@@ -11,7 +11,9 @@ reference's interfaces and the input builders:
 * ``build``      - compiles libgraeng.so with hipcc for gfx950
 """
 from .engine import (GZ, GX, GY, TZZ, TXZ, TYZ, TXX, TXY, TYY, Context, Fwd,  # noqa: F401
-                     GaussNewtonInversion, GraError, component_bits, flag_from_names, forward)
+                     GaussNewtonInversion, GraError, component_bits, flag_from_names, forward,
+                     forward_rectilinear)
 
-__all__ = ["Context", "Fwd", "GaussNewtonInversion", "forward", "GraError", "flag_from_names",
+__all__ = ["Context", "Fwd", "GaussNewtonInversion", "forward", "forward_rectilinear",
+           "GraError", "flag_from_names",
            "component_bits", "GZ", "GX", "GY", "TZZ", "TXZ", "TYZ", "TXX", "TXY", "TYY"]

@@ -429,6 +429,56 @@ gra_status gra_forward(gra_ctx *ctx, const double *prisms, const double *rho, in
     return GRA_OK;
 }
 
+static bool increasing(const double *v, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i]) || (i && !(v[i] > v[i - 1]))) return false;
+    return true;
+}
+
+gra_status gra_forward_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, const double *ys,
+                                   int64_t ny, const double *zs, int64_t nz, const double *rho,
+                                   const double *pts, int64_t N_obs, uint32_t field_flag,
+                                   double *out)
+{
+    CTX_GUARD(ctx);
+    if (!xs || !ys || !zs || !rho || !pts || !out || nx <= 0 || ny <= 0 || nz <= 0 || N_obs <= 0)
+        return set_error(c, GRA_BAD_ARG, "empty input");
+    if ((field_flag & 1u) || (field_flag & ~0x3FFu) || !(field_flag & 0x3FEu))
+        return set_error(c, GRA_BAD_ARG, "field_flag must select components among bits 1..9");
+    if (!increasing(xs, nx + 1) || !increasing(ys, ny + 1) || !increasing(zs, nz + 1))
+        return set_error(c, GRA_BAD_ARG, "node coordinates must be finite and strictly increasing");
+    if ((nx + 1) * (ny + 1) > (int64_t)INT32_MAX / (nz + 1))
+        return set_error(c, GRA_BAD_ARG, "mesh too large (more than 2^31 nodes)");
+    if (!points_valid(pts, N_obs)) return set_error(c, GRA_BAD_ARG, "non-finite point");
+    for (int64_t i = 0; i < N_obs; ++i)
+        if (!(pts[3 * i + 2] < zs[0]))
+            return set_error(c, GRA_BAD_ARG,
+                             "points must lie strictly above the mesh (z < zs[0], z down)");
+    const int nc = ncomp(field_flag);
+    const int64_t Nm = nx * ny * nz, NN = (nx + 1) * (ny + 1) * (nz + 1);
+    std::vector<double> axes((size_t)(nx + ny + nz + 3));
+    std::copy(xs, xs + nx + 1, axes.begin());
+    std::copy(ys, ys + ny + 1, axes.begin() + nx + 1);
+    std::copy(zs, zs + nz + 1, axes.begin() + nx + ny + 2);
+    HIPCHK(put(c.fwd_axes, axes.data(), axes.size(), c.stream), "upload axes");
+    HIPCHK(put(c.fwd_rho, rho, (size_t)Nm, c.stream), "upload rho");
+    HIPCHK(put(c.fwd_pts, pts, (size_t)(3 * N_obs), c.stream), "upload points");
+    HIPCHK(c.fwd_w.reserve(sizeof(double) * (size_t)NN), "alloc node weights");
+    HIPCHK(c.fwd_out.reserve(sizeof(double) * (size_t)N_obs * nc), "alloc out");
+    const double *da = c.fwd_axes.as<double>();
+    HIPCHK(launch_node_forward(da, da + nx + 1, da + nx + ny + 2, nx, ny, nz,
+                               c.fwd_rho.as<double>(), c.fwd_w.as<double>(),
+                               c.fwd_pts.as<double>(), N_obs, field_flag, c.fwd_out.as<double>(),
+                               c.stream),
+           "node forward");
+    HIPCHK(hipMemcpyAsync(out, c.fwd_out.p, sizeof(double) * (size_t)N_obs * nc,
+                          hipMemcpyDeviceToHost, c.stream),
+           "download");
+    HIPCHK(hipStreamSynchronize(c.stream), "forward_rectilinear");
+    return GRA_OK;
+}
+
 gra_status gra_set_regularization(gra_ctx *ctx, const gra_csr *Ws, const gra_csr *Wx,
                                   const gra_csr *Wy, const gra_csr *Wz, const gra_csr *Tx0,
                                   const gra_csr *Ty0, const gra_csr *Tz0)

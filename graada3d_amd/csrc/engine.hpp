@@ -106,6 +106,7 @@ struct Ctx {
     DevBuf fused_scratch, fused_coef, rloop;
     DevBuf vec_nd[4], vec_nm[12], vec_row[3];
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
+    DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
     DevBuf red;                // block partial sums for reductions
     DevBuf scal;               // CgScalars
     DevBuf status;             // int flags written by kernels
@@ -152,6 +153,12 @@ hipError_t launch_assemble(const double *d_prisms, int64_t Nm, const double *d_p
                            uint32_t flag, gra_dtype store, void *G, int64_t ld, int *d_bad,
                            hipStream_t st);
 // Matrix-free forward: out[i*nc+k].
+// Node-sum forward on a rectilinear mesh (rectilinear_kernels.hip); w holds
+// (nx+1)(ny+1)(nz+1) node weights.
+hipError_t launch_node_forward(const double *xs, const double *ys, const double *zs, int64_t nx,
+                               int64_t ny, int64_t nz, const double *rho, double *w,
+                               const double *pts, int64_t N_obs, uint32_t flag, double *out,
+                               hipStream_t st);
 hipError_t launch_forward(const double *d_prisms, const double *d_rho, int64_t Nm,
                           const double *d_pts, int64_t N_obs, uint32_t flag, double *d_out,
                           int *d_bad, hipStream_t st);

@@ -310,6 +310,23 @@ class GaussNewtonInversion(Fwd):
         return m
 
 
+def forward_rectilinear(xs, ys, zs, rho, points, field_flag: int, ctx: Context | None = None):
+    """Forward of a rectilinear mesh (node coordinates xs, ys, zs; cell order
+    i*ny*nz + j*nz + k) by the node-sum form: (N_obs, n_comp) array.  Points
+    must lie above the mesh."""
+    ctx = ctx or Context()
+    xs, ys, zs = _f64(xs), _f64(ys), _f64(zs)
+    nx, ny, nz = xs.size - 1, ys.size - 1, zs.size - 1
+    points = _f64(points).reshape(-1, 3)
+    rho = _f64(rho, nx * ny * nz, "rho")
+    nc = len(component_bits(field_flag))
+    out = np.empty((points.shape[0], nc))
+    st = ctx.L.gra_forward_rectilinear(ctx.h, _p(xs), nx, _p(ys), ny, _p(zs), nz, _p(rho),
+                                       _p(points), points.shape[0], int(field_flag), _p(out))
+    ctx.check(st, "forward_rectilinear")
+    return out
+
+
 def forward(prisms, rho, points, field_flag: int, ctx: Context | None = None):
     """compute.cpp:246-260 matrix-free forward: (N_obs, n_comp) array."""
     ctx = ctx or Context()

@@ -11,8 +11,10 @@ Stream (seeded numpy PCG64):
   log-uniform cell count in [5k, 130k] (nx = ny = 2 nz), a square grid of a
   log-uniform point count in [1k, 20k] at a height uniform in [1, 100] m
   above the mesh top, a uniformly drawn number (1-9) of distinct components,
-  density uniform in [-500, 500] kg/m^3 per cell; run matrix-free
-  (gra_forward, FP64) and returned as FP32;
+  density uniform in [-500, 500] kg/m^3 per cell; run matrix-free in FP64
+  (by default the node-sum form for rectilinear meshes,
+  gra_forward_rectilinear; `forward_method="prisms"` uses the per-prism
+  face/edge kernel, gra_forward) and returned as FP32;
 * 20 % inversion requests: 20 CGLS iterations (L2, lambda log-uniform in
   [0.1, 100]) on one of a pool of problems whose dense G stays resident in
   HBM between requests.  Residency is a byte-budgeted LRU of engine
@@ -34,9 +36,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import mesh as M
-from .engine import Context, GaussNewtonInversion, forward
+from .engine import Context, GaussNewtonInversion, forward, forward_rectilinear
 from .operators import depth_weights_offset, weighting_operators
-from .workloads import add_noise, forward_problem_small
+from .workloads import add_noise
 
 EXTENT = (4000.0, 4000.0, 2000.0)
 
@@ -99,6 +101,7 @@ def request_stream(rate: float, duration: float, seed: int, pool, p_forward: flo
 
 
 def forward_payload(p: dict):
+    """dict(xs, ys, zs, prisms, rho, pts) of a forward request."""
     cells, npts = p["cells"], p["npts"]
     nz = max(1, int(round((cells / 4.0) ** (1.0 / 3.0))))
     nx = max(1, int(round(np.sqrt(cells / nz))))
@@ -106,7 +109,8 @@ def forward_payload(p: dict):
     n = max(2, int(round(np.sqrt(npts))))
     pts = M.grid_points(n, 0.0, EXTENT[0], n, 0.0, EXTENT[1], -p["height"])
     rho = np.random.default_rng(p["seed"]).uniform(-500.0, 500.0, prisms.shape[0])
-    return prisms, rho, pts
+    return dict(xs=M.linspaced(nx + 1, 0.0, EXTENT[0]), ys=M.linspaced(nx + 1, 0.0, EXTENT[1]),
+                zs=M.linspaced(nz + 1, 0.0, EXTENT[2]), prisms=prisms, rho=rho, pts=pts)
 
 
 class ResidentCache:
@@ -180,7 +184,7 @@ def _percentiles(x):
 
 
 def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahead: int = 32,
-          progress=None):
+          progress=None, forward_method: str = "nodes"):
     """Replays `requests` in arrival order; returns a summary dict (times in ms)."""
     payloads, cv = {}, threading.Condition()
     stop = [False]
@@ -206,8 +210,14 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
     rec = []
     try:
         # warm the forward path once (first-call module loading is not a request)
-        w = forward_payload(dict(cells=64, npts=16, flag=0x3FE, height=10.0, seed=1))
-        forward(w[0], w[1], w[2], 0x3FE, ctx=fctx)
+        def run_forward(q, flag):
+            if forward_method == "nodes":
+                return forward_rectilinear(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"], flag,
+                                           ctx=fctx)
+            return forward(q["prisms"], q["rho"], q["pts"], flag, ctx=fctx)
+
+        run_forward(forward_payload(dict(cells=64, npts=16, flag=0x3FE, height=10.0, seed=1)),
+                    0x3FE)
         t_start = time.perf_counter()
         for k, r in enumerate(requests):
             now = time.perf_counter() - t_start
@@ -217,11 +227,12 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
                 with cv:
                     while r.rid not in payloads:
                         cv.wait()
-                    prisms, rho, pts = payloads.pop(r.rid)
+                    q = payloads.pop(r.rid)
                     cv.notify_all()
                 t0 = time.perf_counter() - t_start
-                out = forward(prisms, rho, pts, r.params["flag"], ctx=fctx).astype(np.float32)
-                size = dict(cells=prisms.shape[0], points=pts.shape[0], comps=out.shape[1])
+                out = run_forward(q, r.params["flag"]).astype(np.float32)
+                size = dict(cells=q["prisms"].shape[0], points=q["pts"].shape[0],
+                            comps=out.shape[1])
                 cold = False
             else:
                 t0 = time.perf_counter() - t_start

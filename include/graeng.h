@@ -114,6 +114,24 @@ gra_status gra_col_norms(gra_ctx *ctx, double *norms);
 gra_status gra_forward(gra_ctx *ctx, const double *prisms, const double *rho, int64_t Nm,
                        const double *pts, int64_t N_obs, uint32_t field_flag, double *out);
 
+/*
+ * The same forward for a rectilinear mesh: cells are the tensor product of
+ * the node coordinates xs[nx+1], ys[ny+1], zs[nz+1] (strictly increasing; z
+ * down), rho[(i*ny + j)*nz + k] the density of cell (i, j, k) -- the cell
+ * order of Mesh::generate_regular_mesh (Mesh.cpp:828-846).  The field is
+ * summed over shared mesh nodes (one corner-function evaluation per node
+ * instead of the per-prism face/edge integrals of GravFormula.cpp), which
+ * agrees with gra_forward to ~1e-11 of the largest field value but loses
+ * relative accuracy far outside the mesh (corner-sum cancellation; ~1e-6
+ * relative at 5 mesh widths).  Points must lie strictly above the mesh
+ * (z < zs[0]), else GRA_BAD_ARG.  No counterpart in the reference, whose
+ * compute tool takes an explicit prism list.
+ */
+gra_status gra_forward_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, const double *ys,
+                                   int64_t ny, const double *zs, int64_t nz, const double *rho,
+                                   const double *pts, int64_t N_obs, uint32_t field_flag,
+                                   double *out);
+
 /* ------------------------------------------------------------- inversion */
 /*
  * Uploads the regularisation operators used by gra_solve_cg: Ws, Wx, Wy, Wz

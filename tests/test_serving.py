@@ -30,8 +30,13 @@ def test_stream_is_seeded_and_follows_the_mix():
 
 def test_forward_payload_shapes():
     p = dict(cells=20000, npts=5000, flag=0x12, height=10.0, seed=3)
-    prisms, rho, pts = S.forward_payload(p)
+    q = S.forward_payload(p)
+    prisms, rho, pts = q["prisms"], q["rho"], q["pts"]
     assert abs(prisms.shape[0] - 20000) / 20000 < 0.25
+    nx, ny, nz = q["xs"].size - 1, q["ys"].size - 1, q["zs"].size - 1
+    assert nx * ny * nz == prisms.shape[0]
+    assert np.array_equal(prisms[:, 0], np.repeat(q["xs"][:-1], ny * nz))
+    assert np.array_equal(prisms[:, 5], np.tile(q["zs"][1:], nx * ny))
     assert abs(pts.shape[0] - 5000) / 5000 < 0.05
     assert np.all(pts[:, 2] == -10.0) and np.all(prisms[:, 4] >= 0.0)
     assert rho.shape == (prisms.shape[0],) and np.all(np.abs(rho) <= 500.0)
@@ -71,7 +76,7 @@ def test_resident_cache_lru_admission_and_eviction():
 @pytest.mark.gpu
 def test_short_replay_on_gpu():
     import oracle
-    from graada3d_amd.engine import Context, forward
+    from graada3d_amd.engine import Context, forward, forward_rectilinear
     pool = [S.CgProblem("tiny", (10, 10, 5), (1000.0, 1000.0, 500.0), 8, 0x12, 1.0)]
     reqs = S.request_stream(40.0, 0.5, 5, pool)
     for r in reqs:                               # keep the forward requests small
@@ -84,12 +89,15 @@ def test_short_replay_on_gpu():
     assert res["cache"]["admissions"] <= 1
     # the served forward path against the oracle on one request's payload
     r = next(q for q in reqs if q.kind == "forward")
-    prisms, rho, pts = S.forward_payload(r.params)
+    q = S.forward_payload(r.params)
     ctx = Context(0)
     try:
-        got = forward(prisms, rho, pts[:50], r.params["flag"], ctx=ctx)
+        got = forward_rectilinear(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"][:50],
+                                  r.params["flag"], ctx=ctx)
+        got_p = forward(q["prisms"], q["rho"], q["pts"][:50], r.params["flag"], ctx=ctx)
     finally:
         ctx.close()
-    ref, st = oracle.forward(prisms, rho, pts[:50], r.params["flag"])
-    ref = np.asarray(ref).reshape(got.shape)
-    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+    ref, st = oracle.forward(q["prisms"], q["rho"], q["pts"][:50], r.params["flag"])
+    for k in range(got.shape[1]):
+        assert np.abs(got[:, k] - ref[:, k]).max() <= 1e-9 * np.abs(ref[:, k]).max()
+        assert np.abs(got_p[:, k] - ref[:, k]).max() <= 1e-10 * np.abs(ref[:, k]).max()
