@@ -271,7 +271,8 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
     c.partn_holds_gn_model = false;
     HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
     if (ld > Nm) HIPCHK(hipMemsetAsync(c.G.p, 0, c.G.bytes, c.stream), "clear G padding");
-    DevBuf dpr, dpt;
+    DevBuf dpr;
+    DevBuf &dpt = c.pts_dev;
     HIPCHK(put(dpr, prisms, (size_t)(6 * Nm), c.stream), "upload prisms");
     HIPCHK(put(dpt, pts, (size_t)(3 * N_obs), c.stream), "upload points");
     int *bad = c.status.as<int>();
@@ -360,6 +361,69 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
                           c.stream),
            "download");
     HIPCHK(hipStreamSynchronize(c.stream), "gemv");
+    return GRA_OK;
+}
+
+gra_status gra_expand_cols(gra_ctx *ctx, const int64_t *split_cols, int64_t n_split,
+                           const double *child_prisms)
+{
+    CTX_GUARD(ctx);
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    if (!split_cols || !child_prisms || n_split <= 0)
+        return set_error(c, GRA_BAD_ARG, "empty split list");
+    for (int64_t s = 0; s < n_split; ++s)
+        if (split_cols[s] < 0 || split_cols[s] >= c.Nm || (s && split_cols[s] <= split_cols[s - 1]))
+            return set_error(c, GRA_BAD_ARG,
+                             "split columns must be strictly increasing column indices < Nm");
+    const int64_t nchild = 8 * n_split;
+    if (!prisms_valid(child_prisms, nchild))
+        return set_error(c, GRA_BAD_ARG, "child prism bounds must be finite with x0<x1, y0<y1, z0<z1");
+    const int64_t Nm2 = c.Nm + 7 * n_split;
+    if (Nm2 > INT32_MAX) return set_error(c, GRA_BAD_ARG, "Nm exceeds int32 column indices");
+    const int64_t ld2 = (Nm2 + 63) / 64 * 64, ldc = (nchild + 63) / 64 * 64;
+    // new column j <- old column src[j] (>= 0) or child -src[j]-1, children of a
+    // parent replacing it in place (AdaptiveInversion.cpp:58-104)
+    std::vector<int32_t> src((size_t)Nm2);
+    int64_t out = 0, s = 0;
+    for (int64_t col = 0; col < c.Nm; ++col) {
+        if (s < n_split && split_cols[s] == col) {
+            for (int j = 0; j < 8; ++j) src[(size_t)out++] = -(int32_t)(8 * s + j) - 1;
+            ++s;
+        } else {
+            src[(size_t)out++] = (int32_t)col;
+        }
+    }
+    const size_t esz = dtype_size(c.store);
+    DevBuf dch, dsrc, Gc, G2;
+    HIPCHK(put(dch, child_prisms, (size_t)(6 * nchild), c.stream), "upload child prisms");
+    HIPCHK(put(dsrc, src.data(), src.size(), c.stream), "upload column map");
+    HIPCHK(Gc.reserve(esz * (size_t)c.Nd * (size_t)ldc), "allocate child columns");
+    HIPCHK(G2.reserve(esz * (size_t)c.Nd * (size_t)ld2), "allocate expanded G");
+    if (ld2 > Nm2) HIPCHK(hipMemsetAsync(G2.p, 0, G2.bytes, c.stream), "clear G padding");
+    int *bad = c.status.as<int>();
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), c.stream), "clear status");
+    {
+        TimedRegion tr(c, kTimeAssemble);
+        HIPCHK(launch_assemble(dch.as<double>(), nchild, c.pts_dev.as<double>(), c.N_obs, c.flag,
+                               c.store, Gc.p, ldc, bad, c.stream),
+               "assemble child columns");
+    }
+    HIPCHK(launch_remap_cols(c.G.p, c.ld, Gc.p, ldc, dsrc.as<int32_t>(), c.Nd, Nm2, G2.p, ld2,
+                             (int)esz, c.stream),
+           "remap columns");
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream), "status");
+    HIPCHK(hipStreamSynchronize(c.stream), "expand");
+    std::swap(c.G.p, G2.p);
+    std::swap(c.G.bytes, G2.bytes);
+    c.Nm = Nm2;
+    c.ld = ld2;
+    c.gn_ready = false;
+    c.have_crg_ops = false;
+    c.partn_holds_gn_model = false;
+    if (hbad)
+        return set_error(c, GRA_DEGENERATE_GEOMETRY,
+                         "an observation point lies on a child prism edge (Integral.cpp:77)");
     return GRA_OK;
 }
 

@@ -107,6 +107,7 @@ struct Ctx {
     DevBuf vec_nd[4], vec_nm[12], vec_row[3];
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
+    DevBuf pts_dev;                                  // the assembled G's points (expansion)
     DevBuf red;                // block partial sums for reductions
     DevBuf scal;               // CgScalars
     DevBuf status;             // int flags written by kernels
@@ -172,6 +173,9 @@ hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, co
                          const double *x, double *part, const int *done, hipStream_t st);
 hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, const GemvShape &s,
                          const double *w, double *part, const int *done, hipStream_t st);
+hipError_t launch_remap_cols(const void *G, int64_t ld, const void *Gc, int64_t ldc,
+                             const int32_t *src, int64_t Nd, int64_t Nm2, void *G2, int64_t ld2,
+                             int elem_bytes, hipStream_t st);
 hipError_t launch_col_sumsq(const void *G, gra_dtype dt, int64_t Nd, int64_t ld,
                             const GemvShape &s, double *part, hipStream_t st);
 // y[i] = scale_i * sum_s part[s*stride + i]  (scale may be null => 1)

@@ -329,6 +329,47 @@ hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, co
     return hipGetLastError();
 }
 
+// G2[row, j] = src[j] >= 0 ? G[row, src[j]] : Gc[row, -src[j] - 1], copying the
+// stored bits (storage element of E bytes).
+template <typename E>
+__global__ void remap_cols_kernel(const E *__restrict__ G, int64_t ld, const E *__restrict__ Gc,
+                                  int64_t ldc, const int32_t *__restrict__ src, int64_t Nd,
+                                  int64_t Nm2, E *__restrict__ G2, int64_t ld2)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = blockIdx.y + (int64_t)gridDim.y * blockIdx.z;
+    if (j >= Nm2 || row >= Nd) return;
+    const int32_t s = src[j];
+    G2[row * ld2 + j] = s >= 0 ? G[row * ld + s] : Gc[row * ldc + (-(int64_t)s - 1)];
+}
+
+hipError_t launch_remap_cols(const void *G, int64_t ld, const void *Gc, int64_t ldc,
+                             const int32_t *src, int64_t Nd, int64_t Nm2, void *G2, int64_t ld2,
+                             int elem_bytes, hipStream_t st)
+{
+    if (Nd <= 0 || Nm2 <= 0) return hipSuccess;
+    const int64_t gy = Nd < 65535 ? Nd : 65535;
+    dim3 grid((unsigned)((Nm2 + kBlk - 1) / kBlk), (unsigned)gy, (unsigned)((Nd + gy - 1) / gy));
+    switch (elem_bytes) {
+    case 8:
+        hipLaunchKernelGGL(remap_cols_kernel<uint64_t>, grid, dim3(kBlk), 0, st,
+                           (const uint64_t *)G, ld, (const uint64_t *)Gc, ldc, src, Nd, Nm2,
+                           (uint64_t *)G2, ld2);
+        break;
+    case 4:
+        hipLaunchKernelGGL(remap_cols_kernel<uint32_t>, grid, dim3(kBlk), 0, st,
+                           (const uint32_t *)G, ld, (const uint32_t *)Gc, ldc, src, Nd, Nm2,
+                           (uint32_t *)G2, ld2);
+        break;
+    default:
+        hipLaunchKernelGGL(remap_cols_kernel<uint16_t>, grid, dim3(kBlk), 0, st,
+                           (const uint16_t *)G, ld, (const uint16_t *)Gc, ldc, src, Nd, Nm2,
+                           (uint16_t *)G2, ld2);
+        break;
+    }
+    return hipGetLastError();
+}
+
 // Column sums of squares of G into `part` ([t_chunks][ld] partials, summed in
 // chunk order by the consumer).
 hipError_t launch_col_sumsq(const void *G, gra_dtype dt, int64_t Nd, int64_t ld,

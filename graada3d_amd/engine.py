@@ -184,6 +184,27 @@ class Fwd:
     def compute_gobs(self, rho):
         return self.G_vec_mul(rho)
 
+    def expand_G(self, split_cols, child_prisms):
+        """AdaptiveInversion::expand_G (AdaptiveInversion.cpp:23-106): replace
+        the columns of the split cells (old column indices, ascending) by their
+        8 children's columns, computed on the GPU.  Returns the new Nm."""
+        self._need_G()
+        sc = np.ascontiguousarray(split_cols, dtype=np.int64)
+        ch = _f64(child_prisms).reshape(-1, 6)
+        if ch.shape[0] != 8 * sc.size:
+            raise ValueError("need 8 child prisms per split cell")
+        st = self.ctx.L.gra_expand_cols(self.ctx.h, sc.ctypes.data_as(_lib.c_i64p), sc.size,
+                                        _p(ch))
+        self.ctx.check(st, "expand_G")
+        parts, last = [], 0
+        for s, col in enumerate(sc):
+            parts += [self.prisms[last:col], ch[8 * s:8 * s + 8]]
+            last = col + 1
+        parts.append(self.prisms[last:])
+        self.prisms = np.ascontiguousarray(np.vstack(parts))
+        self.Nm = self.prisms.shape[0]
+        return self.Nm
+
     def col_norms(self):
         """||G(:, j)||_2 per column (the reads of set_depth_weighting(beta, 1),
         InversionBase.cpp:519-526)."""

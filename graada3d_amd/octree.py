@@ -1,0 +1,126 @@
+"""Octree meshes with the reference's leaf (= G column) order and 2:1 balance.
+
+* Level 0 is the regular mesh of Mesh::generate_regular_mesh
+  (Mesh.cpp:796-846): cells in x-outer, y, z-inner order.
+* Mesh::refinement (Mesh.cpp:1462-1976) splits a leaf at its centre
+  (RectPrism::get_center, 0.5*(a+b)) into 8 children c_child[4i+2j+k] over the
+  (x, y, z) halves (Mesh.cpp:1579-1598), which replace the parent in place in
+  the leaf list (Mesh.cpp:1934-1941).  Before a cell is split, every face
+  neighbour coarser than it is split first (the flag1/flag2/flag3 face tests,
+  Mesh.cpp:1474-1556), so face neighbours never differ by more than one
+  level.
+
+`refine(targets)` splits the given leaves (indices in the current leaf list)
+and the coarser neighbours their balance needs, and returns what
+`Fwd.expand_G` (AdaptiveInversion::expand_G) consumes: the ascending old
+indices of every split cell and their children's prisms in child order.
+Because children replace parents in place, the new leaf order does not depend
+on the order of the splits.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .mesh import linspaced
+
+
+class _Leaf:
+    __slots__ = ("level", "key", "bounds", "orig", "children")
+
+    def __init__(self, level, key, bounds, orig=None):
+        self.level, self.key, self.bounds, self.orig = level, key, bounds, orig
+        self.children = None
+
+
+class OctreeMesh:
+    def __init__(self, x_lim, nx, y_lim, ny, z_lim, nz):
+        self.base = (nx, ny, nz)
+        xs, ys, zs = linspaced(nx + 1, *x_lim), linspaced(ny + 1, *y_lim), linspaced(nz + 1, *z_lim)
+        self.leaves = []
+        for i in range(nx):
+            for j in range(ny):
+                for k in range(nz):
+                    self.leaves.append(_Leaf(0, (i, j, k),
+                                             (xs[i], xs[i + 1], ys[j], ys[j + 1], zs[k], zs[k + 1])))
+        self._index = {(0,) + lf.key: lf for lf in self.leaves}
+
+    @property
+    def n_cells(self) -> int:
+        return len(self.leaves)
+
+    def prisms(self) -> np.ndarray:
+        return np.array([lf.bounds for lf in self.leaves], dtype=float).reshape(-1, 6)
+
+    def levels(self) -> np.ndarray:
+        return np.array([lf.level for lf in self.leaves], dtype=np.int64)
+
+    # -- neighbours ---------------------------------------------------------
+    def _coarser_neighbour(self, lf: _Leaf, axis: int, side: int):
+        L = lf.level
+        pos = list(lf.key)
+        pos[axis] += 1 if side else -1
+        if pos[axis] < 0 or pos[axis] >= self.base[axis] << L:
+            return None                                   # domain boundary
+        for lv in range(L - 1, -1, -1):
+            sh = L - lv
+            key = (lv, pos[0] >> sh, pos[1] >> sh, pos[2] >> sh)
+            nb = self._index.get(key)
+            if nb is not None:
+                return nb
+        return None
+
+    # -- splitting ----------------------------------------------------------
+    def _split(self, lf: _Leaf):
+        if lf.children is not None:
+            return
+        for axis in range(3):
+            for side in (0, 1):
+                nb = self._coarser_neighbour(lf, axis, side)
+                if nb is not None:
+                    self._split(nb)
+        x0, x1, y0, y1, z0, z1 = lf.bounds
+        xm, ym, zm = 0.5 * (x1 + x0), 0.5 * (y1 + y0), 0.5 * (z1 + z0)
+        xb, yb, zb = (x0, xm, x1), (y0, ym, y1), (z0, zm, z1)
+        L = lf.level + 1
+        kids = []
+        for i in range(2):
+            for j in range(2):
+                for k in range(2):
+                    key = (2 * lf.key[0] + i, 2 * lf.key[1] + j, 2 * lf.key[2] + k)
+                    c = _Leaf(L, key, (xb[i], xb[i + 1], yb[j], yb[j + 1], zb[k], zb[k + 1]))
+                    kids.append(c)
+                    self._index[(L,) + key] = c
+        del self._index[(lf.level,) + lf.key]
+        lf.children = kids
+
+    def refine(self, targets):
+        """Split the leaves at `targets` (current indices) plus the coarser
+        face neighbours 2:1 balance requires.  Returns (split_cols, child_prisms):
+        ascending old leaf indices of all split cells and an (8*n, 6) array of
+        their children in child order."""
+        for idx, lf in enumerate(self.leaves):
+            lf.orig = idx
+        for t in sorted(set(int(t) for t in targets)):
+            self._split(self.leaves[t])
+        split, kids, new = [], [], []
+        for lf in self.leaves:
+            if lf.children is None:
+                new.append(lf)
+                continue
+            if any(c.children is not None for c in lf.children):
+                raise AssertionError("a child was split within the same refinement")
+            split.append(lf.orig)
+            kids.extend(c.bounds for c in lf.children)
+            new.extend(lf.children)
+        self.leaves = new
+        return np.array(split, dtype=np.int64), np.array(kids, dtype=float).reshape(-1, 6)
+
+    def is_balanced(self) -> bool:
+        """Every face neighbour of every leaf is at most one level coarser."""
+        for lf in self.leaves:
+            for axis in range(3):
+                for side in (0, 1):
+                    nb = self._coarser_neighbour(lf, axis, side)
+                    if nb is not None and nb.level < lf.level - 1:
+                        return False
+        return True

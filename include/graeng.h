@@ -101,6 +101,22 @@ gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product);
 gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product);
 
 /*
+ * Replaces AdaptiveInversion::expand_G (AdaptiveInversion.cpp:23-106): the
+ * cells at the n_split old columns split_cols[] (strictly increasing) are
+ * refined, and each parent's column is replaced in place by the columns of
+ * its 8 children, child_prisms[(8*s + j)*6 .. +5] (x0,x1,y0,y1,z0,z1) for the
+ * s-th split cell in the order of Cell::child_cells (Mesh.cpp:1581-1598:
+ * j = 4i + 2j' + k over x, y, z halves), so the later columns shift by 7 per
+ * earlier split -- the column order of the refined leaf list
+ * (Mesh.cpp:1934-1941).  Only the child columns are computed (on the GPU, in
+ * the context's storage type); Nm grows by 7 * n_split.  Regularisation
+ * operators and the resident Gauss-Newton state refer to the old Nm and must
+ * be set again.
+ */
+gra_status gra_expand_cols(gra_ctx *ctx, const int64_t *split_cols, int64_t n_split,
+                           const double *child_prisms);
+
+/*
  * Column 2-norms of the stored G: norms[j] = ||G(:, j)||_2, j < Nm.  Replaces
  * the G.col(i).norm() reads of InversionBase::set_depth_weighting(beta, 1)
  * (InversionBase.cpp:519-526), whose weight is sqrt(norm / Nd).

@@ -1,0 +1,71 @@
+"""Octree refinement (host): the reference's leaf order and 2:1 face balance
+(Mesh.cpp:1462-1976), pinned by construction invariants."""
+import numpy as np
+
+from graada3d_amd import mesh as M
+from graada3d_amd.octree import OctreeMesh
+
+
+def _volume(p):
+    return float(np.sum((p[:, 1] - p[:, 0]) * (p[:, 3] - p[:, 2]) * (p[:, 5] - p[:, 4])))
+
+
+def test_level0_is_the_regular_mesh():
+    o = OctreeMesh((0, 400), 4, (0, 300), 3, (0, 200), 2)
+    assert np.array_equal(o.prisms(), M.regular_mesh((0, 400), 4, (0, 300), 3, (0, 200), 2))
+
+
+def test_split_replaces_parent_in_place_with_children_in_xyz_order():
+    o = OctreeMesh((0, 400), 4, (0, 300), 3, (0, 200), 2)
+    before = o.prisms()
+    split, kids = o.refine([5])
+    assert list(split) == [5] and kids.shape == (8, 6)
+    after = o.prisms()
+    assert after.shape[0] == before.shape[0] + 7
+    assert np.array_equal(after[:5], before[:5]) and np.array_equal(after[13:], before[6:])
+    p = before[5]
+    xm, ym, zm = 0.5 * (p[1] + p[0]), 0.5 * (p[3] + p[2]), 0.5 * (p[5] + p[4])
+    for c, child in enumerate(after[5:13]):
+        i, j, k = c >> 2, (c >> 1) & 1, c & 1
+        assert child[0] == (p[0], xm)[i] and child[1] == (xm, p[1])[i]
+        assert child[2] == (p[2], ym)[j] and child[3] == (ym, p[3])[j]
+        assert child[4] == (p[4], zm)[k] and child[5] == (zm, p[5])[k]
+    assert np.array_equal(kids, after[5:13])
+
+
+def test_balance_splits_coarser_face_neighbours_first():
+    o = OctreeMesh((0, 400), 4, (0, 400), 4, (0, 400), 4)
+    o.refine([0])                               # cell (0,0,0) -> 8 children
+    # child 7 of cell 0 (upper x,y,z octant) touches level-0 cells (1,0,0),
+    # (0,1,0), (0,0,1); splitting it forces those three to split
+    n0 = o.n_cells
+    split, kids = o.refine([7])
+    assert o.is_balanced()
+    assert len(split) == 4 and kids.shape == (32, 6)
+    assert 7 in split and o.n_cells == n0 + 28
+    # cells (0,0,1), (0,1,0), (1,0,0) sit at old indices 8, 11, 23 (level-0 id + 7)
+    assert sorted(split) == [7, 8, 11, 23]
+
+
+def test_random_refinement_keeps_volume_balance_and_disjointness():
+    rng = np.random.default_rng(4)
+    o = OctreeMesh((0, 800), 4, (0, 800), 4, (0, 400), 2)
+    vol = 800.0 * 800.0 * 400.0
+    for _ in range(4):
+        targets = rng.choice(o.n_cells, size=max(1, o.n_cells // 10), replace=False)
+        prev = o.prisms()
+        split, kids = o.refine(targets)
+        assert np.all(np.diff(split) > 0)
+        p = o.prisms()
+        assert p.shape[0] == prev.shape[0] + 7 * split.size
+        assert abs(_volume(p) - vol) < 1e-6 * vol
+        assert o.is_balanced()
+        # children of each split cell tile their parent exactly
+        for s, col in enumerate(split):
+            assert abs(_volume(kids[8 * s:8 * s + 8]) - _volume(prev[col:col + 1])) < 1e-9 * vol
+    # no two leaves overlap: cell centres are distinct and inside exactly one leaf
+    c = 0.5 * (p[:, 0::2] + p[:, 1::2])
+    inside = ((c[:, None, 0] > p[None, :, 0]) & (c[:, None, 0] < p[None, :, 1]) &
+              (c[:, None, 1] > p[None, :, 2]) & (c[:, None, 1] < p[None, :, 3]) &
+              (c[:, None, 2] > p[None, :, 4]) & (c[:, None, 2] < p[None, :, 5]))
+    assert np.array_equal(inside.sum(axis=1), np.ones(p.shape[0]))
