@@ -115,6 +115,66 @@ class OctreeMesh:
         self.leaves = new
         return np.array(split, dtype=np.int64), np.array(kids, dtype=float).reshape(-1, 6)
 
+    def _finer_across(self, lf: _Leaf, axis: int, side: int):
+        """The 4 children (level L+1) adjacent to lf's face when the cell
+        across it is split, else None."""
+        L = lf.level
+        pos = list(lf.key)
+        pos[axis] += 1 if side else -1
+        if pos[axis] < 0 or pos[axis] >= self.base[axis] << L:
+            return None
+        if (L,) + tuple(pos) in self._index:
+            return None
+        other = [a for a in range(3) if a != axis]
+        out = []
+        for u in range(2):
+            for v in range(2):
+                key = [0, 0, 0]
+                key[axis] = 2 * pos[axis] + (0 if side else 1)
+                key[other[0]] = 2 * pos[other[0]] + u
+                key[other[1]] = 2 * pos[other[1]] + v
+                c = self._index.get((L + 1,) + tuple(key))
+                if c is None:
+                    return None
+                out.append(c)
+        return out
+
+    def indicator(self, m) -> np.ndarray:
+        """AdaptiveInversion::indicator_calculator (AdaptiveInversion.cpp:108-204):
+        for each leaf, the sum over its face neighbours of (m_nb - m_c)^2 times
+        the shared face area (a split neighbour contributes each of its 4 child
+        faces); faces on the mesh boundary contribute nothing."""
+        m = np.asarray(m, float)
+        idx = {id(lf): i for i, lf in enumerate(self.leaves)}
+        out = np.zeros(len(self.leaves))
+
+        def area(b, axis):
+            d = (b[1] - b[0], b[3] - b[2], b[5] - b[4])
+            return d[(axis + 1) % 3] * d[(axis + 2) % 3]
+
+        for i, lf in enumerate(self.leaves):
+            mc = m[i]
+            acc = 0.0
+            for side in (0, 1):
+                for axis in range(3):
+                    fine = self._finer_across(lf, axis, side)
+                    if fine is not None:
+                        for nb in fine:
+                            d = m[idx[id(nb)]] - mc
+                            acc += d * d * area(nb.bounds, axis)
+                        continue
+                    pos = list(lf.key)
+                    pos[axis] += 1 if side else -1
+                    if pos[axis] < 0 or pos[axis] >= self.base[axis] << lf.level:
+                        continue
+                    nb = self._index.get((lf.level,) + tuple(pos))
+                    if nb is None:
+                        nb = self._coarser_neighbour(lf, axis, side)
+                    d = m[idx[id(nb)]] - mc
+                    acc += d * d * area(lf.bounds, axis)
+            out[i] = acc
+        return out
+
     def is_balanced(self) -> bool:
         """Every face neighbour of every leaf is at most one level coarser."""
         for lf in self.leaves:

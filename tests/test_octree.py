@@ -69,3 +69,28 @@ def test_random_refinement_keeps_volume_balance_and_disjointness():
               (c[:, None, 1] > p[None, :, 2]) & (c[:, None, 1] < p[None, :, 3]) &
               (c[:, None, 2] > p[None, :, 4]) & (c[:, None, 2] < p[None, :, 5]))
     assert np.array_equal(inside.sum(axis=1), np.ones(p.shape[0]))
+
+
+def test_indicator_matches_brute_force_face_overlaps():
+    """indicator_calculator: sum over face-adjacent leaf pairs of
+    (m_n - m_c)^2 * overlap area, found here by brute force geometry."""
+    rng = np.random.default_rng(6)
+    o = OctreeMesh((0, 800), 4, (0, 800), 4, (0, 400), 2)
+    for _ in range(3):
+        o.refine(rng.choice(o.n_cells, size=o.n_cells // 6, replace=False))
+    p = o.prisms()
+    m = rng.standard_normal(p.shape[0])
+    got = o.indicator(m)
+    ref = np.zeros(p.shape[0])
+    for a in range(3):
+        o1, o2 = [b for b in range(3) if b != a]
+        lo, hi = p[:, 2 * a], p[:, 2 * a + 1]
+        touch = np.isclose(hi[:, None], lo[None, :], rtol=0, atol=1e-9)      # c below n along a
+        ox = np.minimum(p[:, 2 * o1 + 1][:, None], p[:, 2 * o1 + 1][None, :]) - \
+            np.maximum(p[:, 2 * o1][:, None], p[:, 2 * o1][None, :])
+        oy = np.minimum(p[:, 2 * o2 + 1][:, None], p[:, 2 * o2 + 1][None, :]) - \
+            np.maximum(p[:, 2 * o2][:, None], p[:, 2 * o2][None, :])
+        A = np.where(touch & (ox > 0) & (oy > 0), np.clip(ox, 0, None) * np.clip(oy, 0, None), 0.0)
+        d2 = (m[:, None] - m[None, :]) ** 2
+        ref += (A * d2).sum(axis=1) + (A * d2).sum(axis=0)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-9 * ref.max())
