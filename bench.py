@@ -25,13 +25,19 @@ measured for every storage type.  Kernel rooflines use HIP events on the
 engine's stream; the CPU baseline is the single-threaded C oracle on a
 bounded sample.
 
+* batch: 64 uneven instances (2:1 octree meshes of 2k-60k cells refined at
+  seeded box faces, 400-10,000 points, random component subsets; 1.4e10 G
+  entries, 114 GB in FP64).  A step assembles every instance's G: once in a
+  single batched launch (gra_assemble_batch, `value` = G entries/s) and once
+  as 64 separate launches, reported beside it.
 * serving: an open-loop replay of a seeded Poisson request stream (default
   200 requests/s for 2 s): 80 % matrix-free forward requests (5k-130k cells,
   1k-20k points, 1-9 components, FP32 out), 20 % 20-iteration CGLS solves on
   resident dense G (LRU admission / eviction), one request at a time on the
   GPU (graada3d_amd/serving.py).  `value` is the p99 request latency.
 
-    python bench.py [--workload inversion_small|tensor_large|serving] [--steps K] [--warmup W]
+    python bench.py [--workload inversion_small|tensor_large|batch|serving] [--steps K]
+                    [--warmup W]
                     [--stores f64,f32,bf16] [--no-fused] [--no-cpu] [--gpus 1]
                     [--rate R] [--duration S]
 """
@@ -202,6 +208,73 @@ def cpu_baseline(args, inv, pb, dobs, Wd):
                       f"{os.cpu_count()} logical CPUs"}
 
 
+def run_batch(args):
+    from graada3d_amd.engine import TIME_ASSEMBLE, Context, Fwd, assemble_batch
+    from graada3d_amd.workloads import batch_instances
+    steps = args.steps or 2
+    warm = 1 if args.warmup is None else args.warmup
+    t0 = time.perf_counter()
+    inst = batch_instances()
+    print(f"[bench] batch: {len(inst)} instances generated in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    fwds = [Fwd(b.prisms, b.points, b.flag, ctx=Context(0)) for b in inst]
+    entries = float(sum(f.Nd * f.Nm for f in fwds))
+    for _ in range(warm):
+        assemble_batch(fwds)
+    t_batch, t_seq = [], []
+    ctx0 = fwds[0].ctx
+    for s in range(steps):
+        ctx0.timing(True)
+        ctx0.timing_reset()
+        c0 = time.perf_counter()
+        assemble_batch(fwds)
+        t_batch.append(time.perf_counter() - c0)
+        k_ms, _ = ctx0.timing_get(TIME_ASSEMBLE)
+        ctx0.timing(False)
+        c0 = time.perf_counter()
+        for f in fwds:
+            f.compute_G("f64")
+        t_seq.append(time.perf_counter() - c0)
+        print(f"[bench] batch step {s}: batched {t_batch[-1]:.3f} s (kernel {k_ms / 1e3:.3f} s), "
+              f"sequential {t_seq[-1]:.3f} s", file=sys.stderr, flush=True)
+    tb, ts = min(t_batch), min(t_seq)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        b = inst[0]
+        n = max(1, int(2e6 // b.prisms.shape[0]))
+        c0 = time.perf_counter()
+        oracle.assemble(b.prisms, b.points[:n], b.flag)
+        c1 = time.perf_counter()
+        e = n * b.prisms.shape[0] * bin(b.flag).count("1")
+        cpu = {"value": e / (c1 - c0), "unit": "G entries/s", "cores": 1, "kind": "port",
+               "sample": f"single-threaded C oracle assembly of instance 0's G on {n} of its "
+                         f"{b.points.shape[0]} points ({b.prisms.shape[0]} cells, "
+                         f"{bin(b.flag).count('1')} components; {c1 - c0:.1f} s)"}
+    sizes = np.array([[f.Nm, f.N_obs, f.Nd // f.N_obs] for f in fwds])
+    out = {
+        "metric": METRIC, "value": entries / tb, "unit": "G entries/s", "n_gpus": 1,
+        "steps": steps, "warmup": warm, "ms_per_step": tb * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batch: 64 uneven instances, 2:1 octree meshes (2k-60k cells "
+                               "refined at seeded box faces), 400-10000 points at 0.5-100 m, "
+                               "random component subsets; FP64 G of every instance assembled by "
+                               "one batched launch", "entries": entries,
+                   "cells": [int(sizes[:, 0].min()), int(sizes[:, 0].max())],
+                   "points": [int(sizes[:, 1].min()), int(sizes[:, 1].max())]},
+        "batched_s": tb, "sequential_s": ts, "sequential_entries_per_s": entries / ts,
+        "speedup_batched_vs_sequential": ts / tb,
+        "roofline": {"bound": "fp64-valu", "achieved": None, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": None, "traffic": None,
+                     "note": "transcendental-bound FP64 evaluation; HBM writes are "
+                             f"{entries * 8 / tb / 1e9:.0f} GB/s of 8000"},
+        "cpu_baseline": cpu,
+    }
+    for f in fwds:
+        f.ctx.close()
+    print(json.dumps(out))
+
+
 def run_serving(args):
     from graada3d_amd import serving as S
     pool = S.default_cg_pool()
@@ -255,7 +328,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="inversion_small",
-                    choices=["inversion_small", "tensor_large", "serving"])
+                    choices=["inversion_small", "tensor_large", "batch", "serving"])
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: arrival window, s")
     ap.add_argument("--seed", type=int, default=20240605, help="serving: stream seed")
@@ -277,6 +350,9 @@ def main():
         sys.exit(2)
     if args.workload == "serving":
         run_serving(args)
+        return
+    if args.workload == "batch":
+        run_batch(args)
         return
     small = args.workload == "inversion_small"
     if args.steps is None:

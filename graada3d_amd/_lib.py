@@ -25,7 +25,7 @@ DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
 EXPORTS = [
     "gra_status_string", "gra_last_error", "gra_ctx_create", "gra_ctx_destroy", "gra_sync",
     "gra_assemble", "gra_get_dims", "gra_copy_G", "gra_G_vec_mul", "gra_GT_vec_mul",
-    "gra_col_norms", "gra_expand_cols",
+    "gra_col_norms", "gra_expand_cols", "gra_assemble_batch",
     "gra_forward", "gra_forward_rectilinear", "gra_set_regularization", "gra_solve_cg", "gra_barrier_P",
     "gra_barrier_update", "gra_misfit", "gra_irls_weights", "gra_gn_setup_problem",
     "gra_gn_step", "gra_gn_get_model", "gra_timing_enable", "gra_timing_reset",
@@ -45,6 +45,11 @@ class GraError(RuntimeError):
 class GraCsr(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("row_ptr", c_i64p),
                 ("col_idx", c_i32p), ("val", c_dp)]
+
+
+class GraBatchItem(ctypes.Structure):
+    _fields_ = [("prisms", c_dp), ("Nm", ctypes.c_int64), ("pts", c_dp), ("N_obs", ctypes.c_int64),
+                ("field_flag", ctypes.c_uint32)]
 
 
 class GraCgArgs(ctypes.Structure):
@@ -90,6 +95,8 @@ def load() -> ctypes.CDLL:
         "gra_GT_vec_mul": ([vp, c_dp, c_dp], ctypes.c_int),
         "gra_col_norms": ([vp, c_dp], ctypes.c_int),
         "gra_expand_cols": ([vp, c_i64p, ctypes.c_int64, c_dp], ctypes.c_int),
+        "gra_assemble_batch": ([ctypes.POINTER(vp), ctypes.c_int64,
+                                ctypes.POINTER(GraBatchItem), ctypes.c_int], ctypes.c_int),
         "gra_forward": ([vp, c_dp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64, ctypes.c_uint32,
                          c_dp], ctypes.c_int),
         "gra_forward_rectilinear": ([vp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64, c_dp,

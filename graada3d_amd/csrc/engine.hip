@@ -265,12 +265,13 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
     const int nc = ncomp(field_flag);
     const int64_t Nd = (int64_t)nc * N_obs;
     const int64_t ld = (Nm + 63) / 64 * 64;
-    c.G.release();
-    c.Nd = c.Nm = c.N_obs = c.ld = 0;
+    c.Nd = c.Nm = c.N_obs = c.ld = 0;          // G is invalid until the kernel has run
     c.gn_ready = false;
     c.partn_holds_gn_model = false;
-    HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
-    if (ld > Nm) HIPCHK(hipMemsetAsync(c.G.p, 0, c.G.bytes, c.stream), "clear G padding");
+    HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");  // reuses
+    if (ld > Nm)
+        HIPCHK(hipMemsetAsync(c.G.p, 0, dtype_size(store) * (size_t)Nd * (size_t)ld, c.stream),
+               "clear G padding");
     DevBuf dpr;
     DevBuf &dpt = c.pts_dev;
     HIPCHK(put(dpr, prisms, (size_t)(6 * Nm), c.stream), "upload prisms");
@@ -361,6 +362,108 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
                           c.stream),
            "download");
     HIPCHK(hipStreamSynchronize(c.stream), "gemv");
+    return GRA_OK;
+}
+
+gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item *items,
+                              gra_dtype store)
+{
+    if (!ctxs || !items || n <= 0 || n > INT32_MAX) return GRA_BAD_ARG;
+    std::vector<gra_ctx *> order(ctxs, ctxs + n);
+    for (gra_ctx *x : order)
+        if (!x) return GRA_BAD_ARG;
+    std::sort(order.begin(), order.end());
+    if (std::adjacent_find(order.begin(), order.end()) != order.end()) return GRA_BAD_ARG;
+    std::vector<std::unique_lock<std::mutex>> locks;
+    locks.reserve(order.size());
+    for (gra_ctx *x : order) locks.emplace_back(x->c.mu);   // address order: no lock cycles
+    Ctx &c = ctxs[0]->c;
+    c.last_error.clear();
+    for (int64_t b = 0; b < n; ++b)
+        if (ctxs[b]->c.device != c.device)
+            return set_error(c, GRA_BAD_ARG, "batched contexts must share one device");
+    if (hipSetDevice(c.device) != hipSuccess) return set_error(c, GRA_HIP_ERROR, "hipSetDevice");
+    if (store != GRA_F64 && store != GRA_F32 && store != GRA_BF16)
+        return set_error(c, GRA_BAD_ARG, "unknown storage type");
+    int64_t tot_prisms = 0;
+    for (int64_t b = 0; b < n; ++b) {
+        const gra_batch_item &it = items[b];
+        const uint32_t f = it.field_flag;
+        if (!it.prisms || !it.pts || it.Nm <= 0 || it.N_obs <= 0 || it.Nm > INT32_MAX ||
+            (f & 1u) || (f & ~0x3FFu) || !(f & 0x3FEu) || !prisms_valid(it.prisms, it.Nm) ||
+            !points_valid(it.pts, it.N_obs))
+            return set_error(c, GRA_BAD_ARG,
+                             "batch item " + std::to_string(b) + ": invalid mesh, points or flag");
+        tot_prisms += it.Nm;
+    }
+    std::vector<double> prisms((size_t)(6 * tot_prisms));
+    std::vector<BatchItem> tab((size_t)n);
+    int64_t off = 0, blocks = 0;
+    for (int64_t b = 0; b < n; ++b) {
+        const gra_batch_item &it = items[b];
+        Ctx &cb = ctxs[b]->c;
+        const int64_t Nd = (int64_t)ncomp(it.field_flag) * it.N_obs, ld = (it.Nm + 63) / 64 * 64;
+        cb.Nd = cb.Nm = cb.N_obs = cb.ld = 0;
+        cb.gn_ready = false;
+        cb.have_crg_ops = false;
+        cb.partn_holds_gn_model = false;
+        HIPCHK(cb.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
+        if (ld > it.Nm)
+            HIPCHK(hipMemsetAsync(cb.G.p, 0, dtype_size(store) * (size_t)Nd * (size_t)ld, c.stream),
+                   "clear G padding");
+        HIPCHK(put(cb.pts_dev, it.pts, (size_t)(3 * it.N_obs), c.stream), "upload points");
+        std::copy(it.prisms, it.prisms + 6 * it.Nm, prisms.begin() + 6 * off);
+        BatchItem &t = tab[(size_t)b];
+        t.prisms = nullptr;     // set after the arena upload
+        t.pts = cb.pts_dev.as<double>();
+        t.G = cb.G.p;
+        t.Nm = it.Nm;
+        t.N_obs = it.N_obs;
+        t.ld = ld;
+        t.bpr = (it.Nm + 255) / 256;
+        t.block0 = blocks;
+        t.flag = it.field_flag;
+        blocks += t.bpr * it.N_obs;
+        off += it.Nm;
+    }
+    if (blocks > (int64_t)INT32_MAX) return set_error(c, GRA_BAD_ARG, "batch too large for one launch");
+    HIPCHK(put(c.batch_prisms, prisms.data(), prisms.size(), c.stream), "upload prisms");
+    off = 0;
+    for (int64_t b = 0; b < n; ++b) {
+        tab[(size_t)b].prisms = c.batch_prisms.as<double>() + 6 * off;
+        off += items[b].Nm;
+    }
+    HIPCHK(c.batch_items.reserve(sizeof(BatchItem) * (size_t)n), "alloc item table");
+    HIPCHK(hipMemcpyAsync(c.batch_items.p, tab.data(), sizeof(BatchItem) * (size_t)n,
+                          hipMemcpyHostToDevice, c.stream),
+           "upload item table");
+    HIPCHK(c.batch_bad.reserve(sizeof(int) * (size_t)n), "alloc status");
+    HIPCHK(hipMemsetAsync(c.batch_bad.p, 0, sizeof(int) * (size_t)n, c.stream), "clear status");
+    {
+        TimedRegion tr(c, kTimeAssemble);
+        HIPCHK(launch_assemble_batch(c.batch_items.as<BatchItem>(), (int)n, blocks, store,
+                                     c.batch_bad.as<int>(), c.stream),
+               "batched assembly");
+    }
+    std::vector<int> bad((size_t)n, 0);
+    HIPCHK(hipMemcpyAsync(bad.data(), c.batch_bad.p, sizeof(int) * (size_t)n,
+                          hipMemcpyDeviceToHost, c.stream),
+           "status");
+    HIPCHK(hipStreamSynchronize(c.stream), "batched assembly");
+    for (int64_t b = 0; b < n; ++b) {
+        Ctx &cb = ctxs[b]->c;
+        cb.Nd = (int64_t)ncomp(items[b].field_flag) * items[b].N_obs;
+        cb.Nm = items[b].Nm;
+        cb.N_obs = items[b].N_obs;
+        cb.ld = tab[(size_t)b].ld;
+        cb.flag = items[b].field_flag;
+        cb.store = store;
+    }
+    for (int64_t b = 0; b < n; ++b)
+        if (bad[(size_t)b])
+            return set_error(c, GRA_DEGENERATE_GEOMETRY,
+                             "batch item " + std::to_string(b) +
+                                 ": an observation point lies on a prism edge (Integral.cpp:77)");
     return GRA_OK;
 }
 

@@ -108,6 +108,7 @@ struct Ctx {
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
     DevBuf pts_dev;                                  // the assembled G's points (expansion)
+    DevBuf batch_prisms, batch_items, batch_bad;     // gra_assemble_batch (first context)
     DevBuf red;                // block partial sums for reductions
     DevBuf scal;               // CgScalars
     DevBuf status;             // int flags written by kernels
@@ -154,6 +155,18 @@ hipError_t launch_assemble(const double *d_prisms, int64_t Nm, const double *d_p
                            uint32_t flag, gra_dtype store, void *G, int64_t ld, int *d_bad,
                            hipStream_t st);
 // Matrix-free forward: out[i*nc+k].
+// One item of a batched assembly (device-side table, sorted by block0).
+struct BatchItem {
+    const double *prisms, *pts;   // device
+    void *G;                      // device, Nd x ld of the item's storage type
+    int64_t Nm, N_obs, ld;
+    int64_t bpr;                  // blocks per point row: ceil(Nm / 256)
+    int64_t block0;               // first block of the item in the launch
+    uint32_t flag;
+};
+hipError_t launch_assemble_batch(const BatchItem *d_items, int n_items, int64_t total_blocks,
+                                 gra_dtype store, int *d_bad, hipStream_t st);
+
 // Node-sum forward on a rectilinear mesh (rectilinear_kernels.hip); w holds
 // (nx+1)(ny+1)(nz+1) node weights.
 hipError_t launch_node_forward(const double *xs, const double *ys, const double *zs, int64_t nx,

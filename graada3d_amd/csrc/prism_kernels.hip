@@ -110,6 +110,47 @@ forward_kernel(const double *__restrict__ prisms, const double *__restrict__ rho
     }
 }
 
+// Many G assemblies of uneven sizes in one launch: block b belongs to the
+// last item whose first block is <= b (items sorted by block0); inside an
+// item the mapping is assemble_kernel's, with the item's runtime flag.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+assemble_batch_kernel(const BatchItem *__restrict__ items, int n_items, int *__restrict__ bad)
+{
+    __shared__ int s_item;
+    const int64_t b = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = n_items - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (items[mid].block0 <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        s_item = lo;
+    }
+    __syncthreads();
+    const BatchItem it = items[s_item];
+    const int64_t lb = b - it.block0;
+    const int64_t i = lb / it.bpr;
+    const int64_t j = (lb - i * it.bpr) * kBlock + threadIdx.x;
+    if (i >= it.N_obs || j >= it.Nm) return;
+    double pt[3] = {it.pts[3 * i], it.pts[3 * i + 1], it.pts[3 * i + 2]};
+    double pr[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) pr[q] = it.prisms[6 * j + q];
+    double out[10];
+    if (prism_fields(pt, pr, 1.0, it.flag, out)) atomicOr(bad + s_item, 1);
+    T *G = static_cast<T *>(it.G);
+    int k = 0;
+#pragma unroll
+    for (int bit = 1; bit < 10; ++bit) {
+        if (it.flag & (1u << bit)) {
+            G[((int64_t)k * it.N_obs + i) * it.ld + j] = cvt<T>(out[bit]);
+            ++k;
+        }
+    }
+}
+
 template <typename T>
 hipError_t assemble_dispatch(const double *prisms, int64_t Nm, const double *pts, int64_t N_obs,
                              uint32_t flag, T *G, int64_t ld, int *bad, hipStream_t st)
@@ -160,6 +201,28 @@ hipError_t launch_assemble(const double *d_prisms, int64_t Nm, const double *d_p
         return assemble_dispatch<__hip_bfloat16>(d_prisms, Nm, d_pts, N_obs, flag,
                                                  (__hip_bfloat16 *)G, ld, d_bad, st);
     }
+}
+
+hipError_t launch_assemble_batch(const BatchItem *d_items, int n_items, int64_t total_blocks,
+                                 gra_dtype store, int *d_bad, hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    dim3 grid((unsigned)total_blocks), block(kBlock);
+    switch (store) {
+    case GRA_F64:
+        hipLaunchKernelGGL(assemble_batch_kernel<double>, grid, block, 0, st, d_items, n_items,
+                           d_bad);
+        break;
+    case GRA_F32:
+        hipLaunchKernelGGL(assemble_batch_kernel<float>, grid, block, 0, st, d_items, n_items,
+                           d_bad);
+        break;
+    default:
+        hipLaunchKernelGGL(assemble_batch_kernel<__hip_bfloat16>, grid, block, 0, st, d_items,
+                           n_items, d_bad);
+        break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_forward(const double *d_prisms, const double *d_rho, int64_t Nm,

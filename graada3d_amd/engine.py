@@ -331,6 +331,23 @@ class GaussNewtonInversion(Fwd):
         return m
 
 
+def assemble_batch(fwds, store: str = "f64"):
+    """Assemble the G of every Fwd in `fwds` (each with its own context, one
+    device) in a single batched launch (gra_assemble_batch)."""
+    n = len(fwds)
+    if n == 0:
+        return
+    L = fwds[0].ctx.L
+    hs = (ctypes.c_void_p * n)(*[f.ctx.h.value for f in fwds])
+    items = (_lib.GraBatchItem * n)()
+    for b, f in enumerate(fwds):
+        items[b] = _lib.GraBatchItem(_p(f.prisms), f.Nm, _p(f.points), f.N_obs, f.field_flag)
+    st = L.gra_assemble_batch(hs, n, items, _lib.DTYPES[store])
+    fwds[0].ctx.check(st, "assemble_batch")
+    for f in fwds:
+        f._assembled = True
+
+
 def forward_rectilinear(xs, ys, zs, rho, points, field_flag: int, ctx: Context | None = None):
     """Forward of a rectilinear mesh (node coordinates xs, ys, zs; cell order
     i*ny*nz + j*nz + k) by the node-sum form: (N_obs, n_comp) array.  Points

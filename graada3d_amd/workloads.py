@@ -106,3 +106,75 @@ def tensor_problem_large(seed: int = 20240602) -> ForwardProblem:
     rho = M.model_from_boxes(prisms, boxes)
     pts = M.grid_points(100, 0.0, 4000.0, 100, 0.0, 4000.0, -50.0)
     return ForwardProblem(prisms, rho, pts, ALL9, (64, 64, 32), seed)
+
+
+@dataclass
+class BatchInstance:
+    mesh: object            # octree.OctreeMesh
+    prisms: np.ndarray
+    rho: np.ndarray
+    points: np.ndarray
+    flag: int
+    lp: int                 # 0 (L0) or 2 (L2)
+    cross_gradient: bool
+    seed: int
+
+
+def _crosses_boundary(prisms: np.ndarray, box) -> np.ndarray:
+    """Cells that overlap the box without lying inside it (they straddle a
+    box face)."""
+    x0, x1, y0, y1, z0, z1 = box
+    overlap = ((prisms[:, 0] < x1) & (prisms[:, 1] > x0) & (prisms[:, 2] < y1) &
+               (prisms[:, 3] > y0) & (prisms[:, 4] < z1) & (prisms[:, 5] > z0))
+    inside = ((prisms[:, 0] >= x0) & (prisms[:, 1] <= x1) & (prisms[:, 2] >= y0) &
+              (prisms[:, 3] <= y1) & (prisms[:, 4] >= z0) & (prisms[:, 5] <= z1))
+    return overlap & ~inside
+
+
+def batch_instances(n: int = 64, seed: int = 20240604, extent=(4000.0, 4000.0, 2000.0)):
+    """The uneven batch: per instance, a 2:1 octree (cell count log-uniform in
+    [2k, 60k]: a regular base of about 40 % of the target, refined up to two
+    levels at the faces of two seeded boxes), a square point grid of a
+    log-uniform count in [400, 10000] at a height uniform in [0.5, 100] m, a
+    random non-empty component subset, L0 or L2 regularisation and, for half
+    of the instances, a cross-gradient term."""
+    from .octree import OctreeMesh
+    rng = np.random.default_rng(seed)
+    X, Y, Z = extent
+    out = []
+    for b in range(n):
+        target = float(np.exp(rng.uniform(np.log(2e3), np.log(6e4))))
+        nz = max(2, int(round((0.4 * target / 4.0) ** (1.0 / 3.0))))
+        o = OctreeMesh((0.0, X), 2 * nz, (0.0, Y), 2 * nz, (0.0, Z), nz)
+        boxes = []
+        for _ in range(2):
+            s = np.exp(rng.uniform(np.log(300.0), np.log(1500.0), 3))
+            s[2] = min(s[2], 0.8 * Z)
+            lo = rng.uniform([0, 0, 0], [X - s[0], Y - s[1], Z - s[2]])
+            boxes.append((lo[0], lo[0] + s[0], lo[1], lo[1] + s[1], lo[2], lo[2] + s[2],
+                          float(rng.uniform(-500.0, 500.0))))
+        for level in range(2):
+            if o.n_cells >= 0.8 * target:
+                break
+            p = o.prisms()
+            hit = np.zeros(p.shape[0], bool)
+            for bx in boxes:
+                hit |= _crosses_boundary(p, bx[:6])
+            hit &= o.levels() == level
+            idx = np.flatnonzero(hit)
+            room = int((1.3 * target - o.n_cells) // 7)
+            if idx.size == 0 or room <= 0:
+                break
+            if idx.size > room:
+                idx = rng.choice(idx, room, replace=False)
+            o.refine(idx)
+        prisms = o.prisms()
+        npts = float(np.exp(rng.uniform(np.log(400.0), np.log(1e4))))
+        m = max(20, min(100, int(round(np.sqrt(npts)))))
+        pts = M.grid_points(m, 0.0, X, m, 0.0, Y, -float(rng.uniform(0.5, 100.0)))
+        ncomp = int(rng.integers(1, 10))
+        bits = rng.choice(np.arange(1, 10), ncomp, replace=False)
+        flag = int(sum(1 << int(q) for q in bits))
+        out.append(BatchInstance(o, prisms, M.model_from_boxes(prisms, boxes), pts, flag,
+                                 int(rng.choice([0, 2])), bool(b % 2), seed * 1000 + b))
+    return out

@@ -84,6 +84,25 @@ gra_status gra_sync(gra_ctx *ctx);
 gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const double *pts,
                         int64_t N_obs, uint32_t field_flag, gra_dtype store);
 
+/*
+ * Batched Fwd::compute_G over uneven instances (no counterpart in the
+ * reference, which assembles one G per inversion): ctxs[b] (distinct, one
+ * device) receives the G of items[b] -- its own mesh, points and component
+ * set -- exactly as gra_assemble would compute it, but all n instances are
+ * evaluated by one kernel launch (on ctxs[0]'s stream) whose blocks are
+ * distributed over the instances in proportion to their sizes.  Errors are
+ * reported on ctxs[0] (gra_last_error names the instance).
+ */
+typedef struct {
+    const double *prisms;  /* Nm x 6 */
+    int64_t Nm;
+    const double *pts;     /* N_obs x 3 */
+    int64_t N_obs;
+    uint32_t field_flag;
+} gra_batch_item;
+gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item *items,
+                              gra_dtype store);
+
 /* Shape of the context's G: Nd = n_comp * N_obs rows, Nm columns. */
 gra_status gra_get_dims(const gra_ctx *ctx, int64_t *Nd, int64_t *Nm, int64_t *N_obs,
                         uint32_t *field_flag, int *store);
