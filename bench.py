@@ -56,6 +56,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "G entries/s, CG iter/s (mixed vs FP64 plain) at stated error; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+# FP64 flop per G entry of the assembly kernels (gz+Tzz / all nine components),
+# from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters
+FLOP_PER_ENTRY = {True: 2368.0, False: 1568.0}
 BYTES = {"f64": 8, "f32": 4, "bf16": 2}
 
 
@@ -416,7 +419,12 @@ def main():
         "kernels": {"gemv_n": ref["gemv_n"], "gemv_t": ref["gemv_t"]},
         "g_assembly": {"entries_per_s": ref["entries_per_s"], "avg_ms": ref["asm_ms"],
                        "entries": nd * nm, "store": "f64",
-                       "kernel": "face/edge FP64 evaluation, " + ("gz+Tzz" if small else "9 comps")},
+                       "kernel": "face/edge FP64 evaluation, " + ("gz+Tzz" if small else "9 comps"),
+                       # FP64 flop per entry from PMC instruction counters
+                       "roofline": {"bound": "fp64-valu", "flop_per_entry": FLOP_PER_ENTRY[small],
+                                    "achieved": ref["entries_per_s"] * FLOP_PER_ENTRY[small] / 1e12,
+                                    "peak": 78.6, "unit": "TFLOP/s",
+                                    "frac": ref["entries_per_s"] * FLOP_PER_ENTRY[small] / 78.6e12}},
         "final_misfit": ref["misfit"],
         "mixed": mixed,
         "mixed_error_definition": "||m - m_f64|| / ||m_f64|| of the final model and |misfit - "

@@ -11,6 +11,14 @@ Runs a fixed, short sequence on GPU 0 for ``rocprofv3 --pmc`` passes:
 
     rocprofv3 --pmc FETCH_SIZE -d <dir> -o probe -- python3 -m graada3d_amd.probe
     rocprofv3 --pmc WRITE_SIZE -d <dir> -o probe -- python3 -m graada3d_amd.probe
+
+With ``--assemble`` it instead assembles G once for the same survey (gz+Tzz,
+1.6e8 entries), once with all nine components on a 20x20x10 mesh and a 40x40
+grid (1.44e7 entries) and runs one node-sum forward (40x40x20 mesh, 50x50
+points, 9 components), for FP64 instruction counters:
+
+    rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 \
+        SQ_INSTS_VALU_TRANS_F64 -d <dir> -o asm -- python3 -m graada3d_amd.probe --assemble
 """
 from __future__ import annotations
 
@@ -39,9 +47,29 @@ def device_copy(nbytes: int) -> None:
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--all-dtypes", action="store_true")
+    ap.add_argument("--assemble", action="store_true")
     args = ap.parse_args(argv)
-    from .engine import Context, Fwd
+    from .engine import Context, Fwd, forward_rectilinear
     from .workloads import forward_problem_small
+    from . import mesh as M
+
+    if args.assemble:
+        ctx = Context(0)
+        fp = forward_problem_small()
+        f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
+        f.compute_G("f64")
+        print(f"[probe] assembled {f.Nd} x {f.Nm} (gz+Tzz)", flush=True)
+        pr = M.regular_mesh((0.0, 2000.0), 20, (0.0, 2000.0), 20, (0.0, 1000.0), 10)
+        pts = M.grid_points(40, 0.0, 2000.0, 40, 0.0, 2000.0, -1.0)
+        g = Fwd(pr, pts, 0x3FE, ctx=ctx)
+        g.compute_G("f64")
+        print(f"[probe] assembled {g.Nd} x {g.Nm} (9 components)", flush=True)
+        xs = M.linspaced(41, 0.0, 2000.0)
+        zs = M.linspaced(21, 0.0, 1000.0)
+        forward_rectilinear(xs, xs, zs, fp.rho, fp.points, 0x3FE, ctx=ctx)
+        print("[probe] node-sum forward 40x40x20 mesh, 2500 points, 9 components", flush=True)
+        ctx.close()
+        return
 
     device_copy(1 << 30)
     print("[probe] 1 GiB device copy done", flush=True)
