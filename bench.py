@@ -175,22 +175,41 @@ def run(args):
     return pb, nd, nm, results, cpu
 
 
+def all_cores(fn, unit_count, unit):
+    """Re-times fn() with the oracle's loops on every host core."""
+    import oracle
+    n = oracle.max_threads()
+    oracle.set_threads(n)
+    try:
+        c0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - c0
+    finally:
+        oracle.set_threads(1)
+    return {"value": unit_count / dt, "unit": unit, "cores": n, "seconds": dt}
+
+
 def cpu_baseline(args, inv, pb, dobs, Wd):
     import oracle
     if args.workload == "inversion_small":
         inv.compute_G("f64")
         G = inv.get_G()
         P = oracle.barrier_P(pb["m_ini"], pb["m_min"], pb["m_max"])
+
+        def solve():
+            oracle.solve_cg(G, 0.0, args.cpu_iters, pb["lambdas"][0], pb["m_ini"], pb["mref"], Wd,
+                            dobs, P, *pb["W"])
+
         c0 = time.perf_counter()
-        oracle.solve_cg(G, 0.0, args.cpu_iters, pb["lambdas"][0], pb["m_ini"], pb["mref"], Wd, dobs,
-                        P, *pb["W"])
+        solve()
         c1 = time.perf_counter()
         return {"value": args.cpu_iters / (c1 - c0), "unit": "CG iter/s", "cores": 1,
                 "kind": "port",
                 "sample": f"one solve_cg of {args.cpu_iters} CGLS iterations (plus its setup "
                           f"products) on the same 5000x32000 FP64 G, single-threaded C oracle "
                           f"(sequential GEMV, as the reference's Eigen GEMV runs); {c1 - c0:.1f} s; "
-                          f"host has {os.cpu_count()} logical CPUs"}
+                          f"host has {os.cpu_count()} logical CPUs",
+                "all_cores": all_cores(solve, args.cpu_iters, "CG iter/s")}
     # tensor_large: G (94 GB) is not copied to the host; the per-iteration CPU
     # cost is extrapolated from single-threaded oracle products on 2,000 rows.
     fp = pb["fp"]
@@ -199,16 +218,21 @@ def cpu_baseline(args, inv, pb, dobs, Wd):
     Gs = rng.standard_normal((rows, fp.prisms.shape[0]))
     x = rng.standard_normal(fp.prisms.shape[0])
     y = rng.standard_normal(rows)
+    def products():
+        for _ in range(3):
+            oracle.gemv(Gs, x)
+            oracle.gemv(Gs, y, trans=True)
+
     c0 = time.perf_counter()
-    for _ in range(3):
-        oracle.gemv(Gs, x)
-        oracle.gemv(Gs, y, trans=True)
+    products()
     c1 = time.perf_counter()
     per_iter = (c1 - c0) / 3 * (inv.Nd / rows)
+    ac = all_cores(products, 3 * rows / inv.Nd, "CG iter/s")
     return {"value": 1.0 / per_iter, "unit": "CG iter/s", "cores": 1, "kind": "port",
             "sample": f"single-threaded C oracle G x and G^T w on {rows} rows x {inv.Nm} columns "
                       f"(3 repetitions, {c1 - c0:.1f} s), extrapolated to {inv.Nd} rows; host has "
-                      f"{os.cpu_count()} logical CPUs"}
+                      f"{os.cpu_count()} logical CPUs",
+            "all_cores": ac}
 
 
 def run_batch(args):
@@ -250,10 +274,11 @@ def run_batch(args):
         oracle.assemble(b.prisms, b.points[:n], b.flag)
         c1 = time.perf_counter()
         e = n * b.prisms.shape[0] * bin(b.flag).count("1")
+        ac = all_cores(lambda: oracle.assemble(b.prisms, b.points[:n], b.flag), e, "G entries/s")
         cpu = {"value": e / (c1 - c0), "unit": "G entries/s", "cores": 1, "kind": "port",
                "sample": f"single-threaded C oracle assembly of instance 0's G on {n} of its "
                          f"{b.points.shape[0]} points ({b.prisms.shape[0]} cells, "
-                         f"{bin(b.flag).count('1')} components; {c1 - c0:.1f} s)"}
+                         f"{bin(b.flag).count('1')} components; {c1 - c0:.1f} s)", "all_cores": ac}
     sizes = np.array([[f.Nm, f.N_obs, f.Nd // f.N_obs] for f in fwds])
     out = {
         "metric": METRIC, "value": entries / tb, "unit": "G entries/s", "n_gpus": 1,
@@ -298,12 +323,14 @@ def run_serving(args):
         oracle.forward(prisms, rho, pts[:n], r.params["flag"])
         c1 = time.perf_counter()
         per_req = (c1 - c0) * pts.shape[0] / n
+        ac = all_cores(lambda: oracle.forward(prisms, rho, pts[:n], r.params["flag"]),
+                       n / pts.shape[0], "forward requests/s")
         cpu = {"value": 1.0 / per_req, "unit": "forward requests/s", "cores": 1, "kind": "port",
                "sample": f"single-threaded C oracle forward of request {r.rid} ({prisms.shape[0]} "
                          f"cells, {bin(r.params['flag']).count('1')} components) on {n} of its "
                          f"{pts.shape[0]} points ({c1 - c0:.1f} s), extrapolated to the whole "
                          f"request; GPU: {res['by_kind']['forward']['service_ms']['mean']:.1f} ms "
-                         f"mean forward service time"}
+                         f"mean forward service time", "all_cores": ac}
     out = {
         "metric": METRIC,
         "value": res["latency_ms"]["p99"],

@@ -60,6 +60,16 @@ def lib():
     return _lib
 
 
+def set_threads(n: int) -> None:
+    """Threads of the oracle's point / row / column loops (results do not
+    depend on it; 1 = the reference's single-threaded GEMV)."""
+    lib().ora_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().ora_max_threads())
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_PATH)
 

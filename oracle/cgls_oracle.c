@@ -40,8 +40,11 @@ typedef struct {
 } ora_csr;
 
 /* y = G x, G row-major nd x nm (Fwd::G_vec_mul, Fwd.cpp:145) */
+extern int ora_threads;   /* prism_oracle.c */
+
 void ora_gemv_n(const double *G, int64_t nd, int64_t nm, const double *x, double *y)
 {
+#pragma omp parallel for num_threads(ora_threads) schedule(static)
     for (int64_t i = 0; i < nd; i++) {
         const double *row = G + i * nm;
         double acc = 0.0;
@@ -53,11 +56,17 @@ void ora_gemv_n(const double *G, int64_t nd, int64_t nm, const double *x, double
 /* y = G^T x (Fwd::GT_vec_mul, Fwd.cpp:111) */
 void ora_gemv_t(const double *G, int64_t nd, int64_t nm, const double *x, double *y)
 {
-    for (int64_t j = 0; j < nm; j++) y[j] = 0.0;
-    for (int64_t i = 0; i < nd; i++) {
-        const double *row = G + i * nm;
-        double xi = x[i];
-        for (int64_t j = 0; j < nm; j++) y[j] += row[j] * xi;
+    /* column blocks per thread; each column accumulates rows in order */
+    const int64_t nb = (nm + 511) / 512;
+#pragma omp parallel for num_threads(ora_threads) schedule(static)
+    for (int64_t b = 0; b < nb; b++) {
+        const int64_t j0 = b * 512, j1 = j0 + 512 < nm ? j0 + 512 : nm;
+        for (int64_t j = j0; j < j1; j++) y[j] = 0.0;
+        for (int64_t i = 0; i < nd; i++) {
+            const double *row = G + i * nm;
+            double xi = x[i];
+            for (int64_t j = j0; j < j1; j++) y[j] += row[j] * xi;
+        }
     }
 }
 

@@ -28,6 +28,7 @@
  * aborting and the value follows the reference's formula.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stddef.h>
 
@@ -231,20 +232,34 @@ int ora_prism_field(const double *point, const double *prism, double rho,
  * field bit), point i, prism j; only point rows row0..row0+nrows-1 are filled
  * (the caller's buffer holds n_comp*N_obs rows).  Returns 0 or 1 (degenerate).
  */
+/*
+ * Threads used by the loops below (points for assembly / forward, rows of
+ * G x, column blocks of G^T x).  Each thread computes whole output entries in
+ * the serial order, so results are identical for every thread count; 1 (the
+ * default) reproduces the reference's single-threaded Eigen GEMV timing.
+ */
+int ora_threads = 1;
+void ora_set_threads(int n) { ora_threads = n > 0 ? n : 1; }
+int ora_max_threads(void) { return omp_get_max_threads(); }
+
 int ora_assemble(const double *prisms, int64_t Nm, const double *pts, int64_t N_obs,
                  uint32_t flag, int64_t row0, int64_t nrows, double *G)
 {
     int bits[10];
     int nc = comp_bits(flag, bits);
     int st = 0;
-    double f[10];
-    for (int64_t i = row0; i < row0 + nrows && i < N_obs; i++) {
+    int64_t i1 = row0 + nrows < N_obs ? row0 + nrows : N_obs;
+#pragma omp parallel for num_threads(ora_threads) schedule(dynamic, 4) reduction(| : st)
+    for (int64_t i = row0; i < i1; i++) {
+        double f[10];
+        int sti = 0;
         v3 ob = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
         for (int64_t j = 0; j < Nm; j++) {
-            prism_fields(ob, prisms + 6 * j, 1.0, flag, f, &st);
+            prism_fields(ob, prisms + 6 * j, 1.0, flag, f, &sti);
             for (int k = 0; k < nc; k++)
                 G[((int64_t)k * N_obs + i) * Nm + j] = f[bits[k]];
         }
+        st |= sti;
     }
     return st;
 }
@@ -259,14 +274,17 @@ int ora_forward(const double *prisms, const double *rho, int64_t Nm, const doubl
     int bits[10];
     int nc = comp_bits(flag, bits);
     int st = 0;
-    double f[10];
+#pragma omp parallel for num_threads(ora_threads) schedule(dynamic, 4) reduction(| : st)
     for (int64_t i = 0; i < N_obs; i++) {
+        double f[10];
+        int sti = 0;
         v3 ob = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
         for (int k = 0; k < nc; k++) out[i * nc + k] = 0.0;
         for (int64_t j = 0; j < Nm; j++) {
-            prism_fields(ob, prisms + 6 * j, rho[j], flag, f, &st);
+            prism_fields(ob, prisms + 6 * j, rho[j], flag, f, &sti);
             for (int k = 0; k < nc; k++) out[i * nc + k] += f[bits[k]];
         }
+        st |= sti;
     }
     return st;
 }

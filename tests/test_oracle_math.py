@@ -170,3 +170,24 @@ def test_point_on_edge_is_degenerate_and_face_plane_is_not():
     assert st == 0 and np.all(np.isfinite(v[1:]))
     v, st = oracle.prism_field(np.array([25.0, 25.0, 0.0]), pr, 1.0, 0x2)    # on the top face
     assert st == 0 and np.isfinite(v[1])
+
+
+def test_threaded_oracle_is_bit_identical_to_single_threaded():
+    import numpy as np
+    import oracle
+    from graada3d_amd import mesh as M
+    prisms = M.regular_mesh((0.0, 300.0), 6, (0.0, 200.0), 5, (0.0, 100.0), 3)
+    pts = M.grid_points(7, -20.0, 320.0, 6, -20.0, 220.0, -3.0)
+    rng = np.random.default_rng(2)
+    rho = rng.uniform(-1, 1, prisms.shape[0])
+    out = {}
+    for t in (1, 4):
+        oracle.set_threads(t)
+        G, _ = oracle.assemble(prisms, pts, 0x3FE)
+        x = np.linspace(-1, 1, G.shape[1])
+        y = np.linspace(1, 2, G.shape[0])
+        out[t] = (G, oracle.forward(prisms, rho, pts, 0x3FE)[0], oracle.gemv(G, x),
+                  oracle.gemv(G, y, trans=True))
+    oracle.set_threads(1)
+    for a, b in zip(out[1], out[4]):
+        assert np.array_equal(a, b)
