@@ -523,7 +523,8 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     const int chunks = shape.t_chunks;
     hipError_t e;
     FusedPlan plan;
-    bool fused = a.fused && fused_plan(c.store, nd, ld, c.device, c.fused_wgs, &plan);
+    bool fused = a.fused && fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth,
+                                        &plan);
     int *fused_err = c.status.as<int>() + 1;
     if (fused) {
         if ((e = c.fused_scratch.reserve(fused_scratch_bytes(plan, ld))) != hipSuccess)

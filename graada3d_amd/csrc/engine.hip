@@ -979,6 +979,12 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
         if (value < 0 || value > 65536) return set_error(c, GRA_BAD_ARG, "fused workgroups: 0..65536");
         c.fused_wgs = (int)value;
         break;
+    case GRA_TUNE_FUSED_DEPTH:
+        if (value < 0 || value > 7)
+            return set_error(c, GRA_BAD_ARG, "fused lag variant: 0 (default) or 1..7");
+        c.fused_depth = (int)value;
+        break;
+
     default:
         return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
     }
@@ -1105,6 +1111,7 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_T_UNROLL: *value = s.t_unroll; break;
     case GRA_TUNE_NONTEMPORAL: *value = s.nontemporal; break;
     case GRA_TUNE_FUSED_WGS: *value = c.fused_wgs; break;
+    case GRA_TUNE_FUSED_DEPTH: *value = c.fused_depth; break;
     default: return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
     }
     return GRA_OK;

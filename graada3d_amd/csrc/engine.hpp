@@ -82,6 +82,8 @@ struct FusedPlan {
     int64_t nrb_group = 0;        // row blocks per group
     int RB = 0;                   // rows per block
     int threads = 0;
+    uint64_t wait_ticks = 0;      // bound of the kernel's waits (wall-clock ticks)
+    int L = 0;                    // coefficient lag (blocks)
     const void *kernel = nullptr;
 };
 
@@ -130,6 +132,7 @@ struct Ctx {
     // launch-shape overrides (0 = automatic; nontemporal -1 = automatic)
     GemvShape tune_override{0, 0, 0, 0, 0, -1};
     int fused_wgs = 0;            // 0: two per CU
+    int fused_depth = 0;          // 0: default (lag, prefetch) variant
 
     // timing
     bool timing = false;
@@ -202,7 +205,8 @@ hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, 
 // Fused product (fused_kernels.hip).  fused_plan returns false when the
 // cooperative grid cannot be co-resident on `device`; target_wgs <= 0 means
 // two workgroups per CU, larger values are capped at the co-resident limit.
-bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, FusedPlan *out);
+bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, int depth,
+                FusedPlan *out);
 size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld);
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
                         const double *x, const double *Wd, double *coef, double *tdat,

@@ -19,18 +19,26 @@
 //    accumulates q_tile += sum_rows c * G[row, tile] from the same registers.
 //    These waves synchronise only through LDS, so their outstanding G loads
 //    are never drained by a memory fence.
-//  * wave 4 (sync): for each block, copies the posted partials to global
-//    memory, takes an atomic ticket; the last of the NT tiles to arrive sums
-//    the NT partials of each row in tile order, writes t_d = Wd y, c = Wd^2 y
-//    and a partial of ||t_d||^2, and publishes a flag; later the sync wave
-//    waits for the flag of a block and hands its coefficients to the compute
-//    waves through LDS.
+//  * wave 4 (sync), step s: hands block s-L's coefficients to the compute
+//    waves (loaded one step earlier), stores block s's partial dots to global
+//    memory, and -- in the group's tile (s-D) mod NT -- reduces block s-D: the
+//    NT partials of each row in tile order, t_d = Wd y, c = Wd^2 y and a
+//    partial of ||t_d||^2.
+//
+// Cross-workgroup data (partials, coefficients) is published without fences
+// or counters: both buffers are filled with a signalling-NaN sentinel before
+// the launch, producers only store (agent-scope atomic stores: device-coherent,
+// single-copy atomic for 8 bytes), and consumers poll the values themselves
+// until none is the sentinel.  A computed value is never that bit pattern
+// (arithmetic NaNs are quiet).  Every wait points to a strictly earlier step
+// of some workgroup (1 <= D < L), so with all workgroups co-resident the grid
+// cannot deadlock.
 //
 // At the end the four compute waves' q partials are summed in wave order and
 // the workgroup writes its tile of q for its row group (the consumer sums the
 // groups in order), so results are bit-for-bit repeatable.
 //
-// Ring-slot reuse is safe by construction: LDS slots are reused after D = L+1
+// Ring-slot reuse is safe by construction: LDS slots are reused after DS = L+1
 // blocks and register slots after L+1+PF blocks; each reuse is preceded by a
 // wait that implies the previous occupant was consumed (see the comments at
 // the waits).  Every wait is bounded: on time-out the kernel sets *err, every
@@ -46,10 +54,9 @@ namespace gra {
 namespace {
 
 constexpr int kCompute = 4;
-constexpr int kThreads = 64 * (kCompute + 1);
-// Every cross-wave wait gives up after ~2 s of wall-clock time (the constant
-// 100 MHz counter) or as soon as another wave has reported a failure.
-constexpr uint64_t kWaitTicks = 200000000ull;
+constexpr int kThreads = 64 * (kCompute + 2);   // + a sync wave and a reduce wave
+// Sentinel of unpublished partials and coefficients (a signalling NaN).
+constexpr uint64_t kSentinel = 0x7FF4DEADBEEF0001ull;
 
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 
@@ -95,13 +102,8 @@ __device__ inline void lds_store(unsigned *p, unsigned v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Cross-workgroup traffic (row partials, coefficients, counters, flags) uses
-// agent-scope relaxed atomics only: they compile to device-coherent loads and
-// stores (sc1), which do not hit another XCD's stale L2 copy, so no
-// release/acquire fence -- an L2 writeback plus invalidate on this device --
-// is needed per row block.  Ordering: a producer waits until its stores have
-// completed before it bumps a counter or sets a flag; a consumer issues its
-// data loads only after the flag load has returned the set value.
+// Agent-scope relaxed atomics: device-coherent loads and stores (sc1) that do
+// not hit another XCD's stale L2 copy.
 template <typename U> __device__ inline void st_agent(U *p, U v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -110,12 +112,13 @@ template <typename U> __device__ inline U ld_agent(const U *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void stores_complete() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ inline bool is_sentinel(double v) { return __double_as_longlong(v) == (long long)kSentinel; }
 
 // Bounded-wait bookkeeping of one wave.  `failed` is sticky: after the first
 // failure the wave skips all further waits and runs to the end.
 struct Waiter {
     const int *err;
+    uint64_t limit;          // wall-clock ticks (about 2 s)
     bool failed = false;
     unsigned spins = 0;
     uint64_t t0 = 0;
@@ -128,7 +131,7 @@ struct Waiter {
             uint64_t t = wall_clock64();
             if (t0 == 0) t0 = t;
             bool other = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-            if (other || t - t0 > kWaitTicks) {
+            if (other || t - t0 > limit) {
                 if (!other && (threadIdx.x & 63) == 0) atomicOr(err_w, code);
                 failed = true;
                 return false;
@@ -147,15 +150,14 @@ struct FusedArgs {
     int64_t nrb_group;    // row blocks per group
     const double *x;      // [ld]
     const double *Wd;     // [Nd] (null = 1)
-    double *partial;      // [S*nrb_group][NT][RB]
-    double *coef;         // [Nd]   Wd^2 * y
+    double *partial;      // [S*nrb_group][NT][RB], sentinel on entry
+    double *coef;         // [Nd]   Wd^2 * y, sentinel on entry
     double *tdat;         // [Nd]   Wd * y
     double *t2part;       // [S*nrb_group]
-    unsigned *cnt;        // [S*nrb_group], zero on entry
-    unsigned *flag;       // [S*nrb_group], zero on entry
     double *zpart;        // [S][ld]
     int *err;
     const int *done;
+    uint64_t wait_ticks;  // bound of every wait, wall-clock ticks
 };
 
 template <typename T, int CW, int RW, int L, int PF>
@@ -194,7 +196,7 @@ fused_normal_kernel(FusedArgs a)
     if (wave < kCompute) {
         // ---------------------------------------------------- compute waves
         const T *G = static_cast<const T *>(a.G);
-        Waiter wt{a.err};
+        Waiter wt{a.err, a.wait_ticks};
         bool col_ok[CW];
         double xr[CW][V];
         double zacc[CW][V];
@@ -304,9 +306,41 @@ fused_normal_kernel(FusedArgs a)
 #pragma unroll
             for (int q = 0; q < V; ++q) z_lds[wave][cw * 64 * V + lane * V + q] = zacc[cw][q];
     } else {
-        // ------------------------------------------------------- sync wave
-        constexpr int L1 = 1;   // coefficient fetch trails the posts by L1 blocks
-        Waiter wt{a.err};
+        // ------------------------------------------- sync wave and reduce wave
+        Waiter wt{a.err, a.wait_ticks};
+        const int NT = a.NT;
+        const bool row_lane = lane < RB;
+        // coefficient prefetch: the load for the block fetched at step s is
+        // issued at step s - PQ into register slot s % PQ
+        constexpr int PQ = L > 3 ? 2 : 1;
+        double pre[PQ];
+#pragma unroll
+        for (int u = 0; u < PQ; ++u) pre[u] = 0.0;
+        auto coef_row = [&](int64_t f, int64_t &row) {
+            row = g_row0 + f * RB + lane;
+            return row_lane && row < g_row1;
+        };
+        auto issue_prefetch = [&](int64_t f, double &slot) {
+            int64_t row;
+            slot = coef_row(f, row) ? ld_agent(a.coef + row) : 0.0;
+        };
+        auto fetch = [&](int64_t f, double prefetched) {
+            int64_t row;
+            const bool mine = coef_row(f, row);
+            double v = prefetched;
+            wt.begin();
+            while (!wt.failed && !__all(!(mine && is_sentinel(v)))) {
+                if (!wt.keep_waiting(8, a.err)) break;
+                v = mine ? ld_agent(a.coef + row) : 0.0;
+            }
+            const int sl = (int)(f % DS);
+            if (row_lane) c_lds[sl][lane] = v;
+            compiler_fence();
+            // c_lds[sl] previously held block f-DS: every compute wave has
+            // posted block f+L-1 (the post of the previous step), hence
+            // finished phase B of block f-2 >= f-DS.
+            if (lane == 0) lds_store(&cready[sl], (unsigned)(f + 1));
+        };
         auto post = [&](int64_t j) {
             const int sl = (int)(j % DS);
             const unsigned want = (unsigned)(kCompute * (j / DS + 1));
@@ -314,70 +348,79 @@ fused_normal_kernel(FusedArgs a)
             while (!wt.failed && lds_load(&pcount[sl]) < want)
                 if (!wt.keep_waiting(4, a.err)) break;
             compiler_fence();
-            const int64_t rb = rb_base + j;
-            double *part = a.partial + (rb * a.NT + tile) * RB;
-            if (lane < RB) st_agent(part + lane, p_lds[sl][lane]);
-            stores_complete();
-            unsigned old = 0;
-            if (lane == 0)
-                old = __hip_atomic_fetch_add(&a.cnt[rb], 1u, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-            old = __shfl(old, 0, 64);
-            if (old != (unsigned)(a.NT - 1)) return;
-            // last tile of this block: every other tile's partials completed
-            // before its ticket.  Reduce the NT partials of each row in tile
-            // order (lane l: row l % RB, tiles l / RB + m * (64 / RB)), then
-            // combine the lanes of a row with a fixed butterfly.
+            // the slot's previous block (j - DS) was copied at step j - DS
+            const double v = row_lane ? p_lds[sl][lane] : 0.0;
+            if (row_lane) st_agent(a.partial + ((rb_base + j) * NT + tile) * RB + lane, v);
+        };
+        auto reduce = [&](int64_t r) {
+            // lane l: row l % RB, tiles l / RB + m * (64 / RB), in tile order;
+            // then the lanes of a row combine with a fixed butterfly
             constexpr int PER = 64 / RB;
-            const int r = lane % RB, t0 = lane / RB;
-            const double *pb = a.partial + rb * a.NT * RB;
+            constexpr int BATCH = 16;
+            const int rr = lane % RB, t0 = lane / RB;
+            const int64_t rb = rb_base + r;
+            const double *pb = a.partial + rb * NT * RB;
             double y = 0.0;
-            constexpr int BATCH = 16;          // loads in flight per lane
-            for (int tb = t0; tb < a.NT; tb += PER * BATCH) {
+            for (int tb = t0; tb - t0 < NT; tb += PER * BATCH) {
                 double v[BATCH];
+                bool need[BATCH];
 #pragma unroll
-                for (int b = 0; b < BATCH; ++b) {
-                    const int tt = tb + b * PER;
-                    v[b] = ld_agent(pb + (int64_t)(tt < a.NT ? tt : t0) * RB + r);
+                for (int q = 0; q < BATCH; ++q) {
+                    const int tt = tb + q * PER;
+                    need[q] = tt < NT;
+                    v[q] = ld_agent(pb + (int64_t)(need[q] ? tt : 0) * RB + rr);
+                }
+                wt.begin();
+                for (;;) {
+                    bool ok = true;
+#pragma unroll
+                    for (int q = 0; q < BATCH; ++q) ok = ok && !(need[q] && is_sentinel(v[q]));
+                    if (wt.failed || __all(ok)) break;
+                    if (!wt.keep_waiting(16, a.err)) break;
+#pragma unroll
+                    for (int q = 0; q < BATCH; ++q)
+                        if (need[q] && is_sentinel(v[q]))
+                            v[q] = ld_agent(pb + (int64_t)(tb + q * PER) * RB + rr);
                 }
 #pragma unroll
-                for (int b = 0; b < BATCH; ++b)
-                    if (tb + b * PER < a.NT) y += v[b];
+                for (int q = 0; q < BATCH; ++q)
+                    if (need[q]) y += v[q];
             }
 #pragma unroll
             for (int off = RB; off < 64; off <<= 1) y += __shfl_xor(y, off, 64);
-            int64_t row = g_row0 + j * RB + r;
+            const int64_t row = g_row0 + r * RB + rr;
             double t2 = 0.0;
-            if (lane < RB && row < g_row1) {
-                double wd = a.Wd ? a.Wd[row] : 1.0;
-                double td = wd * y;
-                a.tdat[row] = td;              // read by later kernels only
+            if (row_lane && row < g_row1) {
+                const double wd = a.Wd ? a.Wd[row] : 1.0;
+                const double td = wd * y;
+                a.tdat[row] = td;               // read by later kernels only
                 st_agent(a.coef + row, wd * td);
                 t2 = td * td;
             }
 #pragma unroll
             for (int off = 1; off < RB; off <<= 1) t2 += __shfl_xor(t2, off, 64);
             if (lane == 0) a.t2part[rb] = t2;
-            stores_complete();
-            if (lane == 0) st_agent(a.flag + rb, 1u);
         };
-        auto fetch = [&](int64_t j) {
-            const int64_t rb = rb_base + j;
-            wt.begin();
-            while (!wt.failed && ld_agent(a.flag + rb) == 0u)
-                if (!wt.keep_waiting(8, a.err)) break;
-            compiler_fence();
-            const int sl = (int)(j % DS);
-            int64_t row = g_row0 + j * RB + lane;
-            if (lane < RB) c_lds[sl][lane] = row < g_row1 ? ld_agent(a.coef + row) : 0.0;
-            compiler_fence();
-            // c_lds[sl] previously held block j-DS: every compute wave has
-            // posted block j+L1, hence finished phase B of block j+L1-1-L >= j-DS.
-            if (lane == 0) lds_store(&cready[sl], (unsigned)(j + 1));
-        };
-        for (int64_t j = 0; j < nrb + L1; ++j) {
-            if (j < nrb) post(j);
-            if (j >= L1 && j - L1 < nrb) fetch(j - L1);
+        if (wave == kCompute) {
+            // sync wave: step s hands block s-L's coefficients to the compute
+            // waves and posts block s's partial dots
+            for (int64_t s0 = 0; s0 < nrb + L; s0 += PQ) {
+#pragma unroll
+                for (int u = 0; u < PQ; ++u) {
+                    const int64_t s = s0 + u;
+                    if (s >= nrb + L) break;
+                    const int64_t f = s - L;
+                    if (f >= 0 && f < nrb) fetch(f, pre[u]);   // issued at step s - PQ
+                    if (s < nrb) post(s);
+                    const int64_t fn = s + PQ - L;              // fetched at step s + PQ
+                    if (fn >= 0 && fn < nrb) issue_prefetch(fn, pre[u]);
+                }
+            }
+        } else {
+            // reduce wave: the blocks r with r mod NT == tile, in order; each
+            // waits only for the posts of block r, so reductions never hold
+            // up this workgroup's posts
+            for (int64_t r = tile; r < nrb; r += NT) reduce(r);
         }
     }
     __syncthreads();
@@ -391,20 +434,37 @@ fused_normal_kernel(FusedArgs a)
     }
 }
 
-__global__ void zero_u32(unsigned *p, int64_t n)
+__global__ void fill_u64(uint64_t *p, int64_t n, uint64_t v)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+    if (i < n) p[i] = v;
 }
 
-template <typename T, int CW, int RW, int L, int PF> struct FusedCfg {
+// Lag depths L (blocks between posting a block's partial dots and applying
+// its coefficients) with their prefetch depths PF: the register ring holds
+// L+1+PF blocks.
+constexpr int kNumDepths = 7;
+constexpr int kDepthL[kNumDepths] = {3, 4, 5, 6, 6, 6, 8};   // prefetch 2 2 1 1 2 3 2
+
+template <typename T, int CW, int RW> struct FusedCfg {
     static constexpr int V = VecN<T>::n;
     static constexpr int TW = 64 * V * CW;
     static constexpr int RB = kCompute * RW;
-    static const void *kernel() { return (const void *)fused_normal_kernel<T, CW, RW, L, PF>; }
+    static const void *kernel(int depth_index)
+    {
+        switch (depth_index) {
+        case 0: return (const void *)fused_normal_kernel<T, CW, RW, 3, 2>;
+        case 1: return (const void *)fused_normal_kernel<T, CW, RW, 4, 2>;
+        case 2: return (const void *)fused_normal_kernel<T, CW, RW, 5, 1>;
+        case 3: return (const void *)fused_normal_kernel<T, CW, RW, 6, 1>;
+        case 4: return (const void *)fused_normal_kernel<T, CW, RW, 6, 2>;
+        case 5: return (const void *)fused_normal_kernel<T, CW, RW, 6, 3>;
+        default: return (const void *)fused_normal_kernel<T, CW, RW, 8, 2>;
+        }
+    }
 };
 
-template <class C> FusedPlan make_plan(int64_t Nd, int64_t ld, int target_wgs)
+template <class C> FusedPlan make_plan(int64_t Nd, int64_t ld, int target_wgs, int depth_index)
 {
     FusedPlan p;
     p.NT = (int)((ld + C::TW - 1) / C::TW);
@@ -417,31 +477,26 @@ template <class C> FusedPlan make_plan(int64_t Nd, int64_t ld, int target_wgs)
     p.nrb_group = p.rows_per_group / C::RB;
     p.RB = C::RB;
     p.threads = kThreads;
-    p.kernel = C::kernel();
+    p.kernel = C::kernel(depth_index);
     return p;
 }
 
 // Per storage type: columns per lane (in 16-byte vectors), rows per compute
-// wave, coefficient lag L (blocks), prefetch depth PF (blocks).
-using CfgF64 = FusedCfg<double, 2, 2, 3, 2>;
-using CfgF32 = FusedCfg<float, 1, 2, 3, 2>;
-using CfgBf16 = FusedCfg<__hip_bfloat16, 1, 2, 3, 2>;
+// wave.
+using CfgF64 = FusedCfg<double, 2, 2>;
+using CfgF32 = FusedCfg<float, 1, 2>;
+using CfgBf16 = FusedCfg<__hip_bfloat16, 1, 2>;
 
 }  // namespace
 
-bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, FusedPlan *out)
+static bool plan_variant(gra_dtype dt, int64_t Nd, int64_t ld, int cus, int target_wgs, int di,
+                         FusedPlan *out)
 {
-    int cus = 0, coop = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
-        return false;
-    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess ||
-        !coop)
-        return false;
     const void *kern;
     switch (dt) {
-    case GRA_F64: kern = CfgF64::kernel(); break;
-    case GRA_F32: kern = CfgF32::kernel(); break;
-    default: kern = CfgBf16::kernel(); break;
+    case GRA_F64: kern = CfgF64::kernel(di); break;
+    case GRA_F32: kern = CfgF32::kernel(di); break;
+    default: kern = CfgBf16::kernel(di); break;
     }
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, 0) != hipSuccess)
@@ -451,11 +506,39 @@ bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs
     if (target > limit) target = limit;
     FusedPlan p;
     switch (dt) {
-    case GRA_F64: p = make_plan<CfgF64>(Nd, ld, target); break;
-    case GRA_F32: p = make_plan<CfgF32>(Nd, ld, target); break;
-    default: p = make_plan<CfgBf16>(Nd, ld, target); break;
+    case GRA_F64: p = make_plan<CfgF64>(Nd, ld, target, di); break;
+    case GRA_F32: p = make_plan<CfgF32>(Nd, ld, target, di); break;
+    default: p = make_plan<CfgBf16>(Nd, ld, target, di); break;
     }
     if ((int64_t)p.NT * p.S > (int64_t)limit) return false;
+    p.L = kDepthL[di];
+    *out = p;
+    return true;
+}
+
+bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, int depth,
+                FusedPlan *out)
+{
+    int cus = 0, coop = 0, clock_khz = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return false;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess ||
+        !coop)
+        return false;
+    if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+        clock_khz <= 0)
+        return false;
+    // depth: 0 = the storage type's default (fastest in a sweep on the
+    // 5,000 x 32,000 G), else an index + 1 into the (L, PF) table; if the grid
+    // cannot be co-resident (wide G, many column tiles) the leaner (5, 1)
+    // variant is tried before giving up
+    const int def = dt == GRA_F64 ? 6 : (dt == GRA_F32 ? 2 : 4);
+    const int di = depth >= 1 && depth <= kNumDepths ? depth - 1 : def;
+    FusedPlan p;
+    if (!plan_variant(dt, Nd, ld, cus, target_wgs, di, &p) &&
+        !(depth == 0 && plan_variant(dt, Nd, ld, cus, target_wgs, 2, &p)))
+        return false;
+    p.wait_ticks = (uint64_t)clock_khz * 2000ull;   // 2 s
     *out = p;
     return true;
 }
@@ -464,8 +547,7 @@ size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld)
 {
     int64_t nrb = (int64_t)p.S * p.nrb_group;
     return sizeof(double) * (size_t)(nrb * p.NT * p.RB + nrb) +   // partial, t2part
-           sizeof(double) * (size_t)p.S * (size_t)ld +              // zpart
-           sizeof(unsigned) * (size_t)(2 * nrb);                    // cnt, flag
+           sizeof(double) * (size_t)p.S * (size_t)ld;               // zpart
 }
 
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
@@ -490,15 +572,16 @@ hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t l
     a.t2part = reinterpret_cast<double *>(b);
     b += sizeof(double) * (size_t)nrb;
     a.zpart = reinterpret_cast<double *>(b);
-    b += sizeof(double) * (size_t)p.S * (size_t)ld;
-    a.cnt = reinterpret_cast<unsigned *>(b);
-    a.flag = a.cnt + nrb;
     a.coef = coef;
     a.tdat = tdat;
     a.err = err;
     a.done = done;
-    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((2 * nrb + 255) / 256)), dim3(256), 0, st, a.cnt,
-                       2 * nrb);
+    a.wait_ticks = p.wait_ticks;
+    const int64_t npart = nrb * p.NT * p.RB;
+    hipLaunchKernelGGL(fill_u64, dim3((unsigned)((npart + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<uint64_t *>(a.partial), npart, kSentinel);
+    hipLaunchKernelGGL(fill_u64, dim3((unsigned)((Nd + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<uint64_t *>(coef), Nd, kSentinel);
     void *args[] = {&a};
     hipError_t e = hipLaunchCooperativeKernel(p.kernel, dim3((unsigned)(p.NT * p.S)),
                                               dim3(p.threads), args, 0, st);

@@ -283,6 +283,8 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
 #define GRA_TUNE_NONTEMPORAL 5  /* non-temporal loads of G, mask: 1 G*x, 2 G^T*w */
 #define GRA_TUNE_FUSED_WGS 6    /* workgroups of the fused CG product: 1..65536,
                                    capped at what can be co-resident      */
+#define GRA_TUNE_FUSED_DEPTH 7  /* fused: (lag L, prefetch PF) variant 1..7 =
+                                   (3,2) (4,2) (5,1) (6,1) (6,2) (6,3) (8,2) */
 gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
 /*
