@@ -175,6 +175,56 @@ class OctreeMesh:
             out[i] = acc
         return out
 
+    def difference_matrices(self):
+        """(D_x1, D_y1, D_z1) of InversionBase::set_difference_matrix
+        (InversionBase.cpp:189-434) on this octree, CSR in leaf order.  Row c,
+        axis a: a forward difference to the + neighbour over
+        h = 0.5*(d_c + d_n) (a same-size or coarser neighbour; for a split +
+        face the mean of its 4 finer neighbours, weight 0.25 each); at the
+        + boundary the backward difference to the - neighbour(s) with the same
+        rule; a z row is empty when the cell has no z neighbour at all."""
+        import scipy.sparse as sp
+        n = len(self.leaves)
+        idx = {id(lf): i for i, lf in enumerate(self.leaves)}
+        out = []
+        for axis in range(3):
+            rows, cols, vals = [], [], []
+            for i, lf in enumerate(self.leaves):
+                d0 = lf.bounds[2 * axis + 1] - lf.bounds[2 * axis]
+
+                def neighbours(side):
+                    fine = self._finer_across(lf, axis, side)
+                    if fine is not None:
+                        return fine
+                    pos = list(lf.key)
+                    pos[axis] += 1 if side else -1
+                    if pos[axis] < 0 or pos[axis] >= self.base[axis] << lf.level:
+                        return None
+                    nb = self._index.get((lf.level,) + tuple(pos))
+                    return [nb if nb is not None else self._coarser_neighbour(lf, axis, side)]
+
+                nbs, sgn = neighbours(1), 1.0
+                if nbs is None:
+                    nbs, sgn = neighbours(0), -1.0
+                    if nbs is None:
+                        if axis == 2:
+                            continue
+                        raise ValueError("x/y differences need two cells along the axis")
+                d1 = nbs[0].bounds[2 * axis + 1] - nbs[0].bounds[2 * axis]
+                h = 1.0 / (0.5 * (d0 + d1))
+                rows.append(i)
+                cols.append(i)
+                vals.append(-sgn * h)
+                w = sgn * h * 0.25 if len(nbs) == 4 else sgn * h
+                for nb in nbs:
+                    rows.append(i)
+                    cols.append(idx[id(nb)])
+                    vals.append(w)
+            D = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+            D.sort_indices()
+            out.append(D)
+        return tuple(out)
+
     def is_balanced(self) -> bool:
         """Every face neighbour of every leaf is at most one level coarser."""
         for lf in self.leaves:

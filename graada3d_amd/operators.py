@@ -1,4 +1,5 @@
-"""Regularisation operators on a regular (level-0) prism mesh, as CSR.
+"""Regularisation operators as CSR (regular meshes here; octrees supply their
+own difference matrices through the ``*_from`` builders).
 
 Host-side builders of the inputs that ``solve_cg`` consumes, following
 ``InversionBase`` on an unrefined mesh:
@@ -80,7 +81,14 @@ def depth_weights_offset(prisms: np.ndarray, z0: float, beta: float) -> np.ndarr
 
 def weighting_operators(prisms, nx, ny, nz, Z, a_s=1e-3, a_x=1.0, a_y=1.0, a_z=1.0):
     """(Ws, Wx, Wy, Wz) with W = a * V * D * Z, entries ((a*V_i)*D_ij)*Z_j."""
-    Dx, Dy, Dz = difference_matrices(prisms, nx, ny, nz)
+    return weighting_operators_from(prisms, difference_matrices(prisms, nx, ny, nz), Z, a_s, a_x,
+                                    a_y, a_z)
+
+
+def weighting_operators_from(prisms, D, Z, a_s=1e-3, a_x=1.0, a_y=1.0, a_z=1.0):
+    """weighting_operators for any mesh, given its (D_x1, D_y1, D_z1) -- e.g.
+    an octree's (octree.OctreeMesh.difference_matrices)."""
+    Dx, Dy, Dz = D
     n = prisms.shape[0]
     V = np.sqrt(cell_volumes(prisms))
     Ds = sp.identity(n, format="csr")
@@ -98,7 +106,12 @@ def weighting_operators(prisms, nx, ny, nz, Z, a_s=1e-3, a_x=1.0, a_y=1.0, a_z=1
 def cross_gradient_operators(prisms, nx, ny, nz, m0_s):
     """(T_x, T_y, T_z) of InversionBase::set_Tmatrix for the reference model
     m0_s."""
-    Dx, Dy, Dz = difference_matrices(prisms, nx, ny, nz)
+    return cross_gradient_operators_from(prisms, difference_matrices(prisms, nx, ny, nz), m0_s)
+
+
+def cross_gradient_operators_from(prisms, D, m0_s):
+    """cross_gradient_operators for any mesh, given its (D_x1, D_y1, D_z1)."""
+    Dx, Dy, Dz = D
     V = sp.diags(np.sqrt(cell_volumes(prisms)))
     Dxs, Dys, Dzs = (sp.diags(D @ m0_s) for D in (Dx, Dy, Dz))
     Tz = (V @ (Dys @ Dx - Dxs @ Dy)).tocsr()
@@ -110,5 +123,18 @@ def cross_gradient_operators(prisms, nx, ny, nz, m0_s):
     return Tx, Ty, Tz
 
 
-__all__ = ["difference_matrices", "depth_weights_reference", "depth_weights_offset",
-           "weighting_operators", "cross_gradient_operators", "cell_centres"]
+def cross_gradient_mask(prisms, x_lim, y_lim, z_lim) -> np.ndarray:
+    """S_crg of InversionBase::update_S_crg (InversionBase.cpp:710-728): 1 for
+    the cells whose centre lies in the constraint box (its faces included to
+    within 1e-10), else 0."""
+    c = cell_centres(prisms)
+    s = np.ones(prisms.shape[0])
+    for a, (lo, hi) in enumerate((x_lim, y_lim, z_lim)):
+        v = c[:, a]
+        s *= (((v < hi) | (np.abs(v - hi) < 1e-10)) & ((v > lo) | (np.abs(v - lo) < 1e-10)))
+    return s
+
+
+__all__ = ["cross_gradient_mask", "difference_matrices", "depth_weights_reference", "depth_weights_offset",
+           "depth_weights_sensitivity", "weighting_operators", "weighting_operators_from",
+           "cross_gradient_operators", "cross_gradient_operators_from", "cell_centres"]

@@ -94,3 +94,49 @@ def test_indicator_matches_brute_force_face_overlaps():
         d2 = (m[:, None] - m[None, :]) ** 2
         ref += (A * d2).sum(axis=1) + (A * d2).sum(axis=0)
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-9 * ref.max())
+
+
+def test_difference_matrices_level0_equal_regular_operators():
+    from graada3d_amd import operators as O
+    o = OctreeMesh((0, 400), 5, (0, 300), 4, (0, 200), 3)
+    for a, b in zip(o.difference_matrices(), O.difference_matrices(o.prisms(), 5, 4, 3)):
+        assert np.array_equal(a.toarray(), b.toarray())
+
+
+def test_difference_matrices_on_refined_octree():
+    """Every row differentiates a linear field exactly (neighbour centres lie
+    0.5*(d0+d1) away along the axis; 4 finer neighbours average to the point
+    across the face), annihilates constants, and matches a brute-force
+    geometric construction of the reference's stencil rules."""
+    rng = np.random.default_rng(8)
+    o = OctreeMesh((0, 800), 4, (0, 800), 4, (0, 400), 2)
+    for _ in range(3):
+        o.refine(rng.choice(o.n_cells, size=o.n_cells // 5, replace=False))
+    p = o.prisms()
+    c = 0.5 * (p[:, 0::2] + p[:, 1::2])
+    d = p[:, 1::2] - p[:, 0::2]
+    D = o.difference_matrices()
+    n = p.shape[0]
+    for a in range(3):
+        assert np.allclose(D[a] @ c[:, a], 1.0, rtol=0, atol=1e-12)
+        assert np.allclose(D[a] @ np.ones(n), 0.0, atol=1e-15)
+        # brute force: + face neighbours (touching planes, positive overlap)
+        o1, o2 = [b for b in range(3) if b != a]
+        ref = np.zeros((n, n))
+        for i in range(n):
+            def across(side):
+                plane = p[i, 2 * a + 1] if side else p[i, 2 * a]
+                other = p[:, 2 * a] if side else p[:, 2 * a + 1]
+                ov1 = np.minimum(p[i, 2 * o1 + 1], p[:, 2 * o1 + 1]) - np.maximum(p[i, 2 * o1], p[:, 2 * o1])
+                ov2 = np.minimum(p[i, 2 * o2 + 1], p[:, 2 * o2 + 1]) - np.maximum(p[i, 2 * o2], p[:, 2 * o2])
+                return np.flatnonzero(np.isclose(other, plane, atol=1e-9, rtol=0) & (ov1 > 0) & (ov2 > 0))
+            nb, sgn = across(1), 1.0
+            if nb.size == 0:
+                nb, sgn = across(0), -1.0
+            if nb.size == 0:
+                continue
+            finer = d[nb[0], a] < d[i, a]
+            h = 1.0 / (0.5 * (d[i, a] + d[nb[0], a]))
+            ref[i, i] = -sgn * h
+            ref[i, nb] = sgn * h * (0.25 if finer else 1.0)
+        assert np.array_equal(D[a].toarray(), ref), a

@@ -64,3 +64,16 @@ def test_cross_gradient_of_field_with_itself_vanishes():
     for T in (Tx, Ty, Tz):
         assert np.abs(T @ m0).max() < 1e-12 * np.abs(m0).max()
         assert T.nnz > 0
+
+
+def test_cross_gradient_mask_box_with_tolerant_faces():
+    from graada3d_amd import mesh as M
+    from graada3d_amd import operators as O
+    pr = M.regular_mesh((0.0, 100.0), 10, (0.0, 100.0), 10, (0.0, 50.0), 5)
+    s = O.cross_gradient_mask(pr, (25.0, 55.0), (5.0, 95.0), (0.0, 50.0))
+    c = O.cell_centres(pr)
+    # centres at 5, 15, ..., 95; x in [25, 55] -> 25, 35, 45, 55 (faces included)
+    want = (np.isin(c[:, 0], [25.0, 35.0, 45.0, 55.0]) & (c[:, 1] >= 5.0) & (c[:, 1] <= 95.0))
+    assert np.array_equal(s, want.astype(float))
+    assert np.array_equal(O.cross_gradient_mask(pr, (25.0 + 1e-11, 55.0), (0, 100), (0, 50)),
+                          O.cross_gradient_mask(pr, (25.0, 55.0), (0, 100), (0, 50)))
