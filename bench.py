@@ -107,6 +107,7 @@ def run(args):
     if args.fused:
         variants += [(s, True) for s in args.stores]
     asm = {}
+    skipped = []
     for store, fused in variants:
         key = store + ("_fused" if fused else "")
         # the context holds one G: assemble it in this variant's storage type
@@ -164,7 +165,13 @@ def run(args):
             gemv_n=kern(gn_ms, gn_c), gemv_t=kern(gt_ms, gt_c), fused_product=kern(fu_ms, fu_c),
             tuning=ctx.tuning(), autotune_s=tune_s)
         if fused and fu_c == 0:
-            raise RuntimeError("the fused product did not run (grid not co-resident?)")
+            # the fused grid cannot be co-resident for this G: the solver ran
+            # the two-pass iteration; report that instead of a fused figure
+            print(f"[bench] {key}: fused grid not co-resident for this G; two-pass ran",
+                  file=sys.stderr, flush=True)
+            del results[key]
+            skipped.append(key)
+            continue
         print(f"[bench] {key}: {results[key]['iter_s']:.1f} CG iter/s, assembly "
               f"{results[key]['entries_per_s']:.3e} entries/s", file=sys.stderr, flush=True)
 
@@ -172,6 +179,8 @@ def run(args):
     if not args.no_cpu:
         cpu = cpu_baseline(args, inv, pb, dobs, Wd)
     ctx.close()
+    if skipped:
+        results["_skipped"] = skipped
     return pb, nd, nm, results, cpu
 
 
@@ -395,6 +404,7 @@ def main():
     args.stores = ["f64"] + [s for s in args.stores if s != "f64"]
 
     pb, nd, nm, res, cpu = run(args)
+    skipped = res.pop("_skipped", [])
     ref = res["f64"]
     mixed = {}
     for s, r in res.items():
@@ -454,6 +464,8 @@ def main():
                                     "frac": ref["entries_per_s"] * FLOP_PER_ENTRY[small] / 78.6e12}},
         "final_misfit": ref["misfit"],
         "mixed": mixed,
+        "mixed_not_run": {k: "fused grid not co-resident for this G (two-pass iteration ran)"
+                          for k in skipped},
         "mixed_error_definition": "||m - m_f64|| / ||m_f64|| of the final model and |misfit - "
                                   "misfit_f64| / misfit_f64 after the same steps, against the "
                                   "plain FP64 GPU run (itself checked against the CPU oracle)",

@@ -64,7 +64,8 @@ class GraCgArgs(ctypes.Structure):
 class GraGnSetup(ctypes.Structure):
     _fields_ = [("dobs", c_dp), ("Wd", c_dp), ("m_min", c_dp), ("m_max", c_dp),
                 ("mref", c_dp), ("m_ini", c_dp), ("lp", ctypes.c_int), ("eps2", ctypes.c_double),
-                ("fused", ctypes.c_int)]
+                ("fused", ctypes.c_int), ("use_cross_gradient", ctypes.c_int),
+                ("a_crg", ctypes.c_double), ("S_crg", c_dp)]
 
 
 _lib = None

@@ -848,6 +848,17 @@ gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
     c.gn_lp = s->lp;
     c.gn_eps2 = s->eps2;
     c.gn_fused = s->fused;
+    c.gn_crg = s->use_cross_gradient != 0;
+    c.gn_a_crg = s->a_crg;
+    c.gn_have_S = false;
+    if (c.gn_crg) {
+        if (!c.have_crg_ops || c.Tx0.n != nm)
+            return set_error(c, GRA_BAD_ARG, "cross-gradient requested but no T operators for this Nm");
+        if (s->S_crg) {
+            HIPCHK(put(c.gn_S, s->S_crg, (size_t)nm, c.stream), "S_crg");
+            c.gn_have_S = true;
+        }
+    }
     c.partn_holds_gn_model = false;
     c.gn_have_wl = false;
     if (s->lp != 2) HIPCHK(c.gn_WL.reserve(sizeof(double) * 4 * (size_t)nm), "alloc WL");
@@ -882,8 +893,10 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
     a.lambda = lambda;
     a.quirk_mode = 1;
     a.fused = c.gn_fused;
+    a.use_cross_gradient = c.gn_crg ? 1 : 0;
+    a.a_crg = c.gn_a_crg;
     CgDiag dg{c.gn_Wd.as<double>(), c.gn_P.as<double>(), nullptr, nullptr, nullptr, nullptr,
-              nullptr, nullptr, nullptr, nullptr};
+              nullptr, nullptr, nullptr, c.gn_have_S ? c.gn_S.as<double>() : nullptr};
     if (c.gn_have_wl) {
         double *wl = c.gn_WL.as<double>();
         dg.WLs = wl;

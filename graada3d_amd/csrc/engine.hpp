@@ -118,6 +118,9 @@ struct Ctx {
 
     // resident Gauss-Newton state
     bool gn_ready = false;
+    bool gn_crg = false, gn_have_S = false;   // resident loop with the cross-gradient term
+    double gn_a_crg = 0.0;
+    DevBuf gn_S;
     DevBuf gn_dobs, gn_Wd, gn_mmin, gn_mmax, gn_mref, gn_m, gn_mtrial, gn_P, gn_dx, gn_one;
     DevBuf gn_WL;              // 4 x Nm: WL_s, WL_x, WL_y, WL_z (IRLS)
     // part_n holds the column-split partials of G * gn_m (left by the last

@@ -137,7 +137,12 @@ struct Waiter {
                 return false;
             }
         }
-        __builtin_amdgcn_s_sleep(1);
+        // back off: many waves may poll one line (every tile of a row group
+        // waits for the same coefficients), so the polls must not starve
+        // the store they wait for
+        if (spins < 16) __builtin_amdgcn_s_sleep(1);
+        else if (spins < 64) __builtin_amdgcn_s_sleep(8);
+        else __builtin_amdgcn_s_sleep(32);
         return true;
     }
 };
@@ -501,7 +506,12 @@ static bool plan_variant(gra_dtype dt, int64_t Nd, int64_t ld, int cus, int targ
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, 0) != hipSuccess)
         return false;
-    const int limit = per_cu * cus;
+    // Stay below the reported co-residency limit: two six-wave workgroups
+    // share a CU only when their waves balance over the four SIMDs, so a grid
+    // that fills every slot may not be fully resident -- and every row block
+    // waits for all tiles of its group.  Such problems use the two-pass
+    // iteration instead.
+    const int limit = per_cu * cus - cus / 32;
     int target = target_wgs > 0 ? target_wgs : 2 * cus;
     if (target > limit) target = limit;
     FusedPlan p;

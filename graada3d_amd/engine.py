@@ -309,11 +309,18 @@ class GaussNewtonInversion(Fwd):
 
     # resident Gauss-Newton loop (model and CG state stay on the GPU)
     def gn_setup(self, dobs, Wd, m_min, m_max, mref, m_ini, lp: int = 2, eps2: float = 0.0,
-                 fused: bool = False):
+                 fused: bool = False, cross_gradient: bool = False, a_crg: float = 0.0,
+                 S_crg=None):
+        """Resident Gauss-Newton state; with cross_gradient the T operators of
+        set_regularization enter every CG (weight a_crg, mask S_crg)."""
         nm, nd = self.Nm, self.Nd
         self._gn_keep = [_f64(dobs, nd), _f64(Wd, nd), _f64(m_min, nm), _f64(m_max, nm),
                          _f64(mref, nm), _f64(m_ini, nm)]
-        s = GraGnSetup(*[_p(a) for a in self._gn_keep], int(lp), float(eps2), 1 if fused else 0)
+        S = _f64(S_crg, nm, "S_crg") if S_crg is not None else None
+        self._gn_keep.append(S)
+        s = GraGnSetup(*[_p(a) for a in self._gn_keep[:6]], int(lp), float(eps2),
+                       1 if fused else 0, 1 if cross_gradient else 0, float(a_crg),
+                       _p(S) if S is not None else None)
         self.ctx.check(self.ctx.L.gra_gn_setup_problem(self.ctx.h, ctypes.byref(s)), "gn_setup")
 
     def gn_step(self, lam, cg_tol, cg_maxit):

@@ -259,6 +259,10 @@ typedef struct gra_gn_setup {
     int lp;                /* Lp exponent of the model norms (2 = L2, no IRLS) */
     double eps2;           /* IRLS epsilon^2 (GaussNewtonInversion.h:57)      */
     int fused;             /* CG iteration as gra_cg_args.fused               */
+    int use_cross_gradient;/* 1: the cross-gradient rows with the T operators of
+                              gra_set_regularization (GN.cpp:1477-1484)       */
+    double a_crg;          /* cross-gradient weight a (GaussNewtonInversion.h) */
+    const double *S_crg;   /* [Nm] cross-gradient mask (update_S_crg), or null */
 } gra_gn_setup;
 
 gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *setup);
