@@ -245,15 +245,15 @@ def cpu_baseline(args, inv, pb, dobs, Wd):
 
 
 def run_batch(args):
-    from graada3d_amd.engine import TIME_ASSEMBLE, Context, Fwd, assemble_batch
-    from graada3d_amd.workloads import batch_instances
+    from graada3d_amd.engine import TIME_ASSEMBLE, Context, GaussNewtonInversion, assemble_batch
+    from graada3d_amd.workloads import add_noise, batch_instances, batch_regularisation
     steps = args.steps or 2
     warm = 1 if args.warmup is None else args.warmup
     t0 = time.perf_counter()
     inst = batch_instances()
     print(f"[bench] batch: {len(inst)} instances generated in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
-    fwds = [Fwd(b.prisms, b.points, b.flag, ctx=Context(0)) for b in inst]
+    fwds = [GaussNewtonInversion(b.prisms, b.points, b.flag, ctx=Context(0)) for b in inst]
     entries = float(sum(f.Nd * f.Nm for f in fwds))
     for _ in range(warm):
         assemble_batch(fwds)
@@ -274,6 +274,35 @@ def run_batch(args):
         print(f"[bench] batch step {s}: batched {t_batch[-1]:.3f} s (kernel {k_ms / 1e3:.3f} s), "
               f"sequential {t_seq[-1]:.3f} s", file=sys.stderr, flush=True)
     tb, ts = min(t_batch), min(t_seq)
+    # batched inversion: per instance, L0 (IRLS, eps^2 = 1e-2) or L2 Gauss-Newton
+    # on its octree operators, half of them with the cross-gradient term; the
+    # host-side operator build and uploads are not timed
+    gn_steps, cg_its = 3, 20
+    c0 = time.perf_counter()
+    for b, f in zip(inst, fwds):
+        W, T = batch_regularisation(b)
+        f.set_regularization(*W, *(T or ()))
+        ncomp = f.Nd // f.N_obs
+        dobs, sigma = add_noise(f.G_vec_mul(b.rho), f.N_obs, ncomp, b.seed)
+        nm = f.Nm
+        f.gn_setup(dobs, np.repeat(1.0 / sigma, f.N_obs), np.full(nm, -1000.0),
+                   np.full(nm, 1000.0), np.zeros(nm), np.zeros(nm), lp=b.lp,
+                   eps2=1e-2 if b.lp != 2 else 0.0, cross_gradient=b.cross_gradient, a_crg=1.0)
+    setup_s = time.perf_counter() - c0
+    for f in fwds:
+        f.ctx.sync()
+    c0 = time.perf_counter()
+    total_its, mis_drop = 0, []
+    for b, f in zip(inst, fwds):
+        first = None
+        for k in range(gn_steps):
+            mis, err, its = f.gn_step(100.0 * 0.5 ** k, 0.0, cg_its)
+            total_its += its
+            first = mis if first is None else first
+        mis_drop.append(mis / first)
+    t_inv = time.perf_counter() - c0
+    print(f"[bench] batch inversion: {total_its} CG iterations in {t_inv:.2f} s "
+          f"(operator setup {setup_s:.1f} s)", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -301,6 +330,15 @@ def run_batch(args):
                    "points": [int(sizes[:, 1].min()), int(sizes[:, 1].max())]},
         "batched_s": tb, "sequential_s": ts, "sequential_entries_per_s": entries / ts,
         "speedup_batched_vs_sequential": ts / tb,
+        "batch_inversion": {"instances": len(inst), "gn_steps": gn_steps, "cg_iters_per_step": cg_its,
+                            "cg_iterations": total_its, "seconds": t_inv,
+                            "cg_iter_per_s": total_its / t_inv,
+                            "G_bytes_per_s": 2.0 * 8.0 * sum(f.Nd * f.Nm for f in fwds) *
+                            gn_steps * cg_its / t_inv,
+                            "L0": int(sum(b.lp == 0 for b in inst)),
+                            "cross_gradient": int(sum(b.cross_gradient for b in inst)),
+                            "misfit_ratio_last_first_median": float(np.median(mis_drop)),
+                            "host_setup_s": setup_s},
         "roofline": {"bound": "fp64-valu", "achieved": None, "peak": 78.6, "unit": "TFLOP/s",
                      "frac": None, "traffic": None,
                      "note": "transcendental-bound FP64 evaluation; HBM writes are "

@@ -178,3 +178,28 @@ def batch_instances(n: int = 64, seed: int = 20240604, extent=(4000.0, 4000.0, 2
         out.append(BatchInstance(o, prisms, M.model_from_boxes(prisms, boxes), pts, flag,
                                  int(rng.choice([0, 2])), bool(b % 2), seed * 1000 + b))
     return out
+
+
+def smooth_random_field(centres: np.ndarray, length: float, seed: int, modes: int = 32):
+    """A smooth Gaussian-like random field (a sum of random Fourier modes with
+    wavelengths around 2*pi*length), unit variance, at the given points."""
+    rng = np.random.default_rng(seed)
+    k = rng.standard_normal((modes, 3)) / length
+    phase = rng.uniform(0.0, 2.0 * np.pi, modes)
+    f = np.cos(centres @ k.T + phase).sum(axis=1)
+    return (f - f.mean()) / (f.std() + 1e-300)
+
+
+def batch_regularisation(b: BatchInstance, z0: float = 50.0, beta: float = 3.0):
+    """Operators of one batch instance on its octree: (W, T or None), with the
+    depth weighting (z + z0)^(-beta/2) and, for the cross-gradient instances,
+    T built from a smooth random reference model m0_s (length scale 300 m)."""
+    from .operators import (cross_gradient_operators_from, depth_weights_offset,
+                            weighting_operators_from)
+    D = b.mesh.difference_matrices()
+    W = weighting_operators_from(b.prisms, D, depth_weights_offset(b.prisms, z0, beta))
+    T = None
+    if b.cross_gradient:
+        m0s = smooth_random_field(M.cell_centres(b.prisms), 300.0, b.seed)
+        T = cross_gradient_operators_from(b.prisms, D, m0s)
+    return W, T
