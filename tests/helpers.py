@@ -31,3 +31,111 @@ def model_from_fixture(fx):
     prisms = M.regular_mesh((x0, x1), int(nx), (y0, y1), int(ny), (z0, z1), int(nz))
     rho = M.model_from_boxes(prisms, fx["boxes"])
     return prisms, rho
+
+
+class OracleGN:
+    """The oracle behind the interface of engine.GaussNewtonInversion
+    (compute_G, expand_G, G_vec_mul, set_regularization, solve_cg, barrier_P,
+    barrier_update, misfit, irls_weights), so the Gauss-Newton driver can run
+    on the CPU as a checker."""
+
+    def __init__(self, prisms, points, field_flag):
+        self.prisms = np.asarray(prisms, float).copy()
+        self.points = np.asarray(points, float).copy()
+        self.flag = int(field_flag)
+        self.G = None
+        self.W = None
+
+    @property
+    def Nm(self):
+        return self.prisms.shape[0]
+
+    def compute_G(self, store="f64"):
+        import oracle
+        self.G, st = oracle.assemble(self.prisms, self.points, self.flag)
+        assert st == 0, st
+
+    def expand_G(self, split_cols, child_prisms):
+        import oracle
+        parts_p, parts_g = [], []
+        kids = np.asarray(child_prisms, float).reshape(-1, 8, 6)
+        Gk, st = oracle.assemble(kids.reshape(-1, 6), self.points, self.flag)
+        assert st == 0, st
+        pos = {int(c): k for k, c in enumerate(split_cols)}
+        for j in range(self.Nm):
+            if j in pos:
+                k = pos[j]
+                parts_p.append(kids[k])
+                parts_g.append(Gk[:, 8 * k:8 * k + 8])
+            else:
+                parts_p.append(self.prisms[j:j + 1])
+                parts_g.append(self.G[:, j:j + 1])
+        self.prisms = np.concatenate(parts_p)
+        self.G = np.ascontiguousarray(np.concatenate(parts_g, axis=1))
+
+    def G_vec_mul(self, m):
+        import oracle
+        return oracle.gemv(self.G, m)
+
+    def set_regularization(self, Ws, Wx, Wy, Wz, Tx=None, Ty=None, Tz=None):
+        self.W = (Ws, Wx, Wy, Wz)
+
+    def solve_cg(self, tol, maxit, lam, mk, mref, Wd, dobs, P, WL_s=None, WL_x=None, WL_y=None,
+                 WL_z=None, quirks=True, **_):
+        import oracle
+        WL = {k: v for k, v in dict(s=WL_s, x=WL_x, y=WL_y, z=WL_z).items() if v is not None}
+        return oracle.solve_cg(self.G, tol, maxit, lam, mk, mref, Wd, dobs, P, *self.W, WL=WL,
+                               quirks=quirks)
+
+    def barrier_P(self, m, m_min, m_max):
+        import oracle
+        return oracle.barrier_P(m, m_min, m_max)
+
+    def barrier_update(self, m, dx, m_min, m_max, m_trial):
+        import oracle
+        return oracle.barrier_update(m, dx, m_min, m_max, m_trial)
+
+    def misfit(self, m, Wd, dobs):
+        import oracle
+        return oracle.misfit(self.G, m, Wd, dobs)
+
+    def irls_weights(self, W, m, p, eps2):
+        import oracle
+        return oracle.irls_weights(W, m, p, eps2)
+
+
+KARST_FOLDERS = {"raw": "invert_raw_data_pivetta2015",
+                 "interpolated": "invert_interpolated_data_Braitenberg2015"}
+
+
+def stage_karst(key, root):
+    """Lay out one reference cave-inversion example under `root` as the
+    reference ships it (configuration files in <folder>/, the data file in
+    data_preprocessing/); returns (config path, fixture dict)."""
+    import shutil
+    fx = load(os.path.join("karst", f"{key}.npz"))
+    folder = KARST_FOLDERS[key]
+    dst = os.path.join(str(root), folder)
+    os.makedirs(dst, exist_ok=True)
+    for name in ("config", "config_data", "config_model", "config_inversion"):
+        shutil.copyfile(os.path.join(GOLDEN, "karst", folder, name), os.path.join(dst, name))
+    os.makedirs(os.path.join(str(root), "data_preprocessing"), exist_ok=True)
+    np.savetxt(os.path.join(str(root), "data_preprocessing", str(fx["data_name"])), fx["data"],
+               fmt="%.17g")
+    return os.path.join(dst, "config"), fx
+
+
+def check_against_run(run, fx, out_dir, misfit_rtol=1e-3, final_rtol=1e-6, model_rtol=1e-5):
+    """Compare a finished driver run with a reference run's logs and outputs."""
+    lm = np.loadtxt(os.path.join(out_dir, "lambda_misfit_GN"), skiprows=1, ndmin=2)
+    it = np.loadtxt(os.path.join(out_dir, "Iteration_misfit_GN"), skiprows=1, ndmin=2)
+    assert lm.shape[0] == fx["lam"].shape[0], (lm.shape, fx["lam"].shape)
+    np.testing.assert_allclose(lm[:, 0], fx["lam"], rtol=1e-13)
+    np.testing.assert_allclose(lm[:, 1], fx["misfit"], rtol=misfit_rtol)
+    np.testing.assert_array_equal(it[:, :2], fx["iters"][:, :2])
+    np.testing.assert_allclose(it[:, 2], fx["iters"][:, 2], rtol=final_rtol)
+    cells = run.mesh.prisms()
+    assert cells.shape == fx["cells"].shape
+    np.testing.assert_array_equal(cells, fx["cells"])
+    assert rel_norm(run.m, fx["model"]) < model_rtol
+    assert rel_norm(run.engine.G_vec_mul(run.m), fx["dpred"]) < final_rtol
