@@ -36,7 +36,12 @@ bounded sample.
   resident dense G (LRU admission / eviction), one request at a time on the
   GPU (graada3d_amd/serving.py).  `value` is the p99 request latency.
 
-    python bench.py [--workload inversion_small|tensor_large|batch|serving] [--steps K]
+* karst: the reference's own cave-inversion example (Examples/Karstic_cave,
+  interpolated data: 11,374 points, octree refined to 10,490 cells) run end
+  to end by the inversion driver; `value` is the wall time of one whole
+  inversion, beside the run time the reference logged for it.
+
+    python bench.py [--workload inversion_small|tensor_large|batch|serving|karst] [--steps K]
                     [--warmup W]
                     [--stores f64,f32,bf16] [--no-fused] [--no-cpu] [--gpus 1]
                     [--rate R] [--duration S]
@@ -401,11 +406,134 @@ def run_serving(args):
     print(json.dumps(out))
 
 
+def stage_karst(root, key="interpolated"):
+    """The reference's cave-inversion example (configuration files and data
+    file, kept as fixtures under tests/golden/karst) laid out under root."""
+    import shutil
+    folder = {"raw": "invert_raw_data_pivetta2015",
+              "interpolated": "invert_interpolated_data_Braitenberg2015"}[key]
+    src = os.path.join(ROOT, "tests", "golden", "karst")
+    fx = dict(np.load(os.path.join(src, f"{key}.npz"), allow_pickle=False))
+    os.makedirs(os.path.join(root, folder), exist_ok=True)
+    for name in ("config", "config_data", "config_model", "config_inversion"):
+        shutil.copyfile(os.path.join(src, folder, name), os.path.join(root, folder, name))
+    os.makedirs(os.path.join(root, "data_preprocessing"), exist_ok=True)
+    np.savetxt(os.path.join(root, "data_preprocessing", str(fx["data_name"])), fx["data"],
+               fmt="%.17g")
+    return os.path.join(root, folder, "config"), fx
+
+
+def run_karst(args):
+    """The whole adaptive inversion of the reference's cave example (11,374
+    gz points, 16x13x12 initial octree refined to 10,490 cells, L1
+    smallness, 52 lambda trials of up to 100 CG iterations) through the
+    driver on the GPU engine, from reading the files to the final model."""
+    import tempfile
+    from graada3d_amd import driver
+    from graada3d_amd.engine import TIME_GEMV_N, TIME_GEMV_T, Context
+    steps = args.steps or 3
+    warm = 1 if args.warmup is None else args.warmup
+    ctx = Context(0)
+    tmp = tempfile.mkdtemp(prefix="karst_")
+    cfg, fx = stage_karst(tmp)
+    times, runs = [], []
+    for k in range(warm + steps):
+        d, m, inv = driver.read_config(cfg)
+        ctx.timing(k >= warm)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        run = driver.AdaptiveInversion(d, m, inv, ctx=ctx, out_dir=tmp)
+        res = run.run()
+        ctx.sync()
+        el = time.perf_counter() - t0
+        if k >= warm:
+            gn, gt = ctx.timing_get(TIME_GEMV_N), ctx.timing_get(TIME_GEMV_T)
+            times.append(el)
+            runs.append((run, res, gn, gt))
+        print(f"[bench] karst run {k}: {el:.3f} s, misfit {res['misfit']:.9g}, Nm {res['Nm']}",
+              file=sys.stderr, flush=True)
+        ctx.timing(False)
+    run, res, gn, gt = runs[-1]
+    nd = run.n_obs
+    # G-product bytes: every solve's products at that trial's Nm; the launches
+    # beyond the CG iterations (set-up products, misfits) are spread evenly
+    trials = run.trials
+    its = np.array([t["cg_iterations"] for t in trials], float)
+    nms = np.array([t["Nm"] for t in trials], float)
+
+    def rate(tm):
+        avg_ms, n = tm
+        extra = (n - its.sum()) / len(trials)
+        byt = float(((its + extra) * nd * nms * 8).sum())
+        return byt / (avg_ms * n * 1e-3) / 1e9, avg_ms * n
+    gbn, msn = rate(gn)
+    gbt, mst = rate(gt)
+    frac = (gbn * msn + gbt * mst) / (msn + mst) / HBM_PEAK_GBS
+    cpu = None
+    if not args.no_cpu:
+        cpu = karst_cpu_baseline(cfg, trials, res["Nm"])
+    sec = float(np.mean(times))
+    out = {
+        "metric": "whole adaptive inversion wall time (reference's cave example)",
+        "value": sec, "unit": "s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "reference example data (Filtered_anomaly.dat)",
+        "config": {"workload": f"cave inversion: {nd} gz points, 16x13x12 octree -> {res['Nm']} "
+                               f"cells, {len(trials)} lambda trials, "
+                               f"{int(its.sum())} CG iterations, final misfit {res['misfit']:.9g}"},
+        "roofline": {"bound": "hbm", "achieved": (gbn * msn + gbt * mst) / (msn + mst),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": frac, "traffic": None},
+        "gemv_GBps": {"n": gbn, "t": gbt}, "gemv_share_of_wall": (msn + mst) * 1e-3 / sec,
+        "reference_logged_s": float(fx["seconds"]),
+        "misfit_vs_reference_run": abs(res["misfit"] - float(fx["iters"][-1, 2]))
+        / float(fx["iters"][-1, 2]),
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out))
+
+
+def karst_cpu_baseline(cfg, trials, final_nm, n_trials=2):
+    """The same driver with the oracle behind it, for the first n_trials
+    lambda trials (all at the initial 2,496 cells), scaled to the whole run by
+    CG work (iterations x Nm per trial)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import tempfile
+    import oracle
+    from helpers import OracleGN
+    from graada3d_amd import driver
+    n = oracle.max_threads()
+    oracle.set_threads(n)
+    try:
+        d, m, inv = driver.read_config(cfg)
+        c0 = time.perf_counter()
+        run = driver.AdaptiveInversion(d, m, inv, out_dir=tempfile.mkdtemp(prefix="karst_cpu_"),
+                                       engine_factory=OracleGN)
+        c1 = time.perf_counter()
+        run.run(max_trials=n_trials)
+        c2 = time.perf_counter()
+    finally:
+        oracle.set_threads(1)
+
+    def work(ts):
+        return sum((t["cg_iterations"] + 2) * t["Nm"] for t in ts)
+    # columns assembled over the run: the initial mesh plus 8 per split cell
+    nm0, nm1 = trials[0]["Nm"], final_nm
+    cols = nm0 + 8 * (nm1 - nm0) / 7
+    est = (c1 - c0) * cols / nm0 + (c2 - c1) * work(trials) / work(run.trials)
+    return {"value": est, "unit": "s (whole inversion, extrapolated)", "cores": n, "kind": "port",
+            "sample": f"driver on the C oracle (OpenMP, {n} threads): set-up with the "
+                      f"{run.n_obs}x{nm0} G assembly ({c1 - c0:.1f} s, scaled to the run's "
+                      f"{cols:.0f} assembled columns) and the first {n_trials} lambda trials "
+                      f"({c2 - c1:.1f} s, scaled by CG work = iterations x cells to the GPU run's "
+                      f"{len(trials)} trials)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="inversion_small",
-                    choices=["inversion_small", "tensor_large", "batch", "serving"])
+                    choices=["inversion_small", "tensor_large", "batch", "serving", "karst"])
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: arrival window, s")
     ap.add_argument("--seed", type=int, default=20240605, help="serving: stream seed")
@@ -430,6 +558,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "karst":
+        run_karst(args)
         return
     small = args.workload == "inversion_small"
     if args.steps is None:

@@ -274,8 +274,12 @@ class AdaptiveInversion:
         self._build_operators()
         return int(split.size)
 
-    def run(self):
+    def run(self, max_trials: int | None = None, verbose: bool = False):
+        """The Gauss-Newton loop.  `max_trials` stops after that many lambda
+        trials in total (a bounded sample of the run); `verbose` prints the
+        reference's per-trial line."""
         c, e = self.cfg, self.engine
+        self.trials = []
         lam_log = open(os.path.join(self.out_dir, "lambda_misfit_GN"), "w")
         it_log = open(os.path.join(self.out_dir, "Iteration_misfit_GN"), "w")
         lam_log.write(f"{'lambda':>25}{'misfit':>25}\n")
@@ -307,17 +311,24 @@ class AdaptiveInversion:
             j = 0
             while j < c.n_lambda:
                 kw = {f"WL_{k}": v for k, v in WL.items()}
-                dx, _, _ = e.solve_cg(c.cg_tol, cg_maxit, lam, self.m, self.m0, self.Wd, self.dobs,
-                                      P, quirks=True, **kw)
+                dx, cg_err, cg_its = e.solve_cg(c.cg_tol, cg_maxit, lam, self.m, self.m0, self.Wd,
+                                                self.dobs, P, quirks=True, **kw)
                 m_trial = e.barrier_update(self.m, dx, self.m_min, self.m_max, m_trial)
                 misfit_last2, misfit_last1 = misfit_last1, misfit
                 misfit = e.misfit(m_trial, self.Wd, self.dobs)
                 lam_log.write(f"{lam:25.14e}{misfit:25.14e}\n")
+                self.trials.append(dict(lam=lam, misfit=misfit, cg_iterations=int(cg_its),
+                                        cg_error=float(cg_err), Nm=nm))
+                if verbose:
+                    print(f"  #{j} lambda={lam:g}, misfit={misfit:g} (CG iterations: {cg_its}, "
+                          f"error: {cg_err:g})", flush=True)
                 if j == 0 or misfit < misfit_opt:
                     misfit_opt, m_opt = misfit, m_trial.copy()
                     if i > 0:
                         self.m = m_opt.copy()
                     lambda_opt = lam
+                if max_trials is not None and len(self.trials) >= max_trials:
+                    break
                 if j > 1 and (abs(misfit - misfit_last1) / min(misfit, misfit_last1) < c.stag_tol
                               and abs(misfit_last1 - misfit_last2) /
                               min(misfit_last1, misfit_last2) < c.stag_tol):
@@ -333,6 +344,8 @@ class AdaptiveInversion:
             tried = c.n_lambda if j == c.n_lambda else j + 1
             it_log.write(f"{i + 1:25d}{tried:25d}{misfit:25.14e}\n")
             history.append(dict(iteration=i + 1, lambdas=tried, misfit=misfit, Nm=nm))
+            if max_trials is not None and len(self.trials) >= max_trials:
+                break
             if misfit < c.target_misfit or i == c.gn_iterations - 1:
                 break
             flag1 = abs(misfit - misfit_last_iteration) / min(misfit, misfit_last_iteration) < \
@@ -343,9 +356,11 @@ class AdaptiveInversion:
             if c_ref < c.max_refinements and (i_between % c.refine_interval == 0 or flag1):
                 i_between = 0
                 c_ref += 1
-                if self._refine():
-                    WL = self._irls(self.m)
-                    m_opt = self.m.copy()
+                # W and the IRLS weights are rebuilt from m after every
+                # refinement attempt (GN.cpp:1137-1262)
+                self._refine()
+                WL = self._irls(self.m)
+                m_opt = self.m.copy()
         lam_log.close()
         it_log.close()
         self.misfit, self.history = misfit, history
