@@ -523,8 +523,8 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     const int chunks = shape.t_chunks;
     hipError_t e;
     FusedPlan plan;
-    bool fused = a.fused && fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth,
-                                        &plan);
+    bool fused = a.fused && !c.wav &&
+                 fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth, &plan);
     int *fused_err = c.status.as<int>() + 1;
     if (fused) {
         if ((e = c.fused_scratch.reserve(fused_scratch_bytes(plan, ld))) != hipSuccess)
@@ -634,15 +634,13 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     // b, s, first residual and A^T b
     if (!have_Gmk) {
         TimedRegion tr(c, kTimeGemvN);
-        if ((e = launch_gemv_n(c.G.p, c.store, nd, ld, shape, d_mk, c.part_n.as<double>(), nullptr,
-                               st)))
+        if ((e = launch_G_n(c, shape, d_mk, c.part_n.as<double>(), nullptr, st)))
             return hip_fail(c, e, "gemv_n");
     }
     hipLaunchKernelGGL(k_b_init, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
     {
         TimedRegion tr(c, kTimeGemvT);
-        if ((e = launch_gemv_t(c.G.p, c.store, nd, ld, shape, d.w, c.part_t.as<double>(), nullptr,
-                               st)))
+        if ((e = launch_G_t(c, shape, d.w, c.part_t.as<double>(), nullptr, st)))
             return hip_fail(c, e, "gemv_t");
     }
     hipLaunchKernelGGL(k_r_update<1>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
@@ -656,8 +654,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
             hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
             {
                 TimedRegion tr(c, kTimeGemvN);
-                e = launch_gemv_n(c.G.p, c.store, nd, ld, shape, d.Pp, c.part_n.as<double>(), done,
-                                  st);
+                e = launch_G_n(c, shape, d.Pp, c.part_n.as<double>(), done, st);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
             }
             hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
@@ -665,8 +662,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
             hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
             {
                 TimedRegion tr(c, kTimeGemvT);
-                e = launch_gemv_t(c.G.p, c.store, nd, ld, shape, d.w, c.part_t.as<double>(), done,
-                                  st);
+                e = launch_G_t(c, shape, d.w, c.part_t.as<double>(), done, st);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
             }
             hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st, d);

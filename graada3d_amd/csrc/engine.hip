@@ -132,6 +132,9 @@ int ncomp(uint32_t flag) { return __builtin_popcount(flag & 0x3FEu); }
 
 size_t dtype_size(gra_dtype d) { return d == GRA_F64 ? 8 : (d == GRA_F32 ? 4 : 2); }
 
+// A G (dense or wavelet-compressed) is assembled.
+bool have_G(const Ctx &c) { return c.Nd > 0 && (c.wav || c.G.p); }
+
 }  // namespace
 
 GemvShape gemv_shape(const Ctx &c)
@@ -144,6 +147,7 @@ GemvShape gemv_shape(const Ctx &c)
     if (o.t_chunks) s.t_chunks = o.t_chunks;
     if (o.t_unroll) s.t_unroll = o.t_unroll;
     if (o.nontemporal >= 0) s.nontemporal = o.nontemporal;
+    if (c.wav) s.n_splits = s.t_chunks = 1;   // the compressed products write whole sums
     return s;
 }
 
@@ -267,6 +271,7 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
     const int64_t ld = (Nm + 63) / 64 * 64;
     c.Nd = c.Nm = c.N_obs = c.ld = 0;          // G is invalid until the kernel has run
     c.gn_ready = false;
+    c.wav = false;
     c.partn_holds_gn_model = false;
     HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");  // reuses
     if (ld > Nm)
@@ -307,10 +312,74 @@ gra_status gra_get_dims(const gra_ctx *ctx, int64_t *Nd, int64_t *Nm, int64_t *N
     return GRA_OK;
 }
 
+gra_status gra_assemble_wavelet(gra_ctx *ctx, const double *prisms, int64_t Nm, const double *pts,
+                                int64_t N_obs, uint32_t field_flag, const int64_t *order,
+                                double relative_threshold)
+{
+    CTX_GUARD(ctx);
+    if (!prisms || !pts || !order || Nm <= 0 || N_obs <= 0)
+        return set_error(c, GRA_BAD_ARG, "empty mesh, point set or order");
+    if ((field_flag & 1u) || (field_flag & ~0x3FFu) || !(field_flag & 0x3FEu))
+        return set_error(c, GRA_BAD_ARG, "field_flag must select components among bits 1..9");
+    if (Nm > INT32_MAX / 2) return set_error(c, GRA_BAD_ARG, "Nm exceeds int32 column indices");
+    if (!(relative_threshold >= 0.0) || !std::isfinite(relative_threshold))
+        return set_error(c, GRA_BAD_ARG, "relative threshold must be finite and >= 0");
+    if (!prisms_valid(prisms, Nm))
+        return set_error(c, GRA_BAD_ARG, "prism bounds must be finite with x0<x1, y0<y1, z0<z1");
+    if (!points_valid(pts, N_obs)) return set_error(c, GRA_BAD_ARG, "non-finite point");
+    const int64_t Nd = (int64_t)ncomp(field_flag) * N_obs;
+    if (Nd > INT32_MAX) return set_error(c, GRA_BAD_ARG, "rows exceed int32 row indices");
+    std::vector<char> seen((size_t)Nm, 0);
+    std::vector<int32_t> ord((size_t)Nm);
+    std::vector<double> pro((size_t)(6 * Nm));
+    for (int64_t j = 0; j < Nm; ++j) {
+        const int64_t o = order[j];
+        if (o < 0 || o >= Nm || seen[(size_t)o])
+            return set_error(c, GRA_BAD_ARG, "order must be a permutation of 0..Nm-1");
+        seen[(size_t)o] = 1;
+        ord[(size_t)j] = (int32_t)o;
+        for (int q = 0; q < 6; ++q) pro[(size_t)(6 * j + q)] = prisms[6 * o + q];
+    }
+    c.Nd = c.Nm = c.N_obs = c.ld = 0;
+    c.gn_ready = false;
+    c.partn_holds_gn_model = false;
+    c.wav = false;
+    c.G.release();
+    DevBuf dpr;
+    HIPCHK(put(dpr, pro.data(), pro.size(), c.stream), "upload prisms");
+    HIPCHK(put(c.pts_dev, pts, (size_t)(3 * N_obs), c.stream), "upload points");
+    HIPCHK(put(c.wav_order, ord.data(), ord.size(), c.stream), "upload order");
+    int *bad = c.status.as<int>();
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), c.stream), "clear status");
+    gra_status st = wavelet_assemble(c, dpr.as<double>(), Nm, c.pts_dev.as<double>(), N_obs,
+                                     field_flag, relative_threshold, bad);
+    if (st != GRA_OK) return st;
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream), "status");
+    HIPCHK(hipStreamSynchronize(c.stream), "wavelet assembly");
+    c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = (Nm + 63) / 64 * 64; c.flag = field_flag;
+    c.store = GRA_F64;
+    c.wav = true;
+    if (hbad)
+        return set_error(c, GRA_DEGENERATE_GEOMETRY,
+                         "an observation point lies on a prism edge (s0*s1 <= 0, Integral.cpp:77)");
+    return GRA_OK;
+}
+
+gra_status gra_wavelet_info(gra_ctx *ctx, int64_t *n_dy, int64_t *nnz)
+{
+    CTX_GUARD(ctx);
+    if (!c.wav || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no wavelet-compressed G");
+    if (n_dy) *n_dy = c.wav_ndy;
+    if (nnz) *nnz = c.wav_nnz;
+    return GRA_OK;
+}
+
 gra_status gra_copy_G(gra_ctx *ctx, double *out, int col_major)
 {
     CTX_GUARD(ctx);
     if (!out) return set_error(c, GRA_BAD_ARG, "null output");
+    if (c.wav) return set_error(c, GRA_BAD_ARG, "not available for the wavelet-compressed G");
     if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
     DevBuf tmp;
     HIPCHK(tmp.reserve(sizeof(double) * (size_t)c.Nd * (size_t)c.Nm), "allocate copy");
@@ -327,7 +396,7 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
 {
     CTX_GUARD(ctx);
     if (!vec || !product) return set_error(c, GRA_BAD_ARG, "null vector");
-    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    if (!have_G(c)) return set_error(c, GRA_BAD_ARG, "no G assembled");
     const int64_t nin = trans ? c.Nd : c.Nm, nout = trans ? c.Nm : c.Nd;
     DevBuf &x = c.vec_nd[1], &y = c.vec_nd[2];
     if (trans) HIPCHK(put(x, vec, (size_t)nin, c.stream), "upload vec");
@@ -339,9 +408,8 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
         HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * c.Nd), "alloc part");
         {
             TimedRegion tr(c, kTimeGemvN);
-            HIPCHK(launch_gemv_n(c.G.p, c.store, c.Nd, c.ld, shape, x.as<double>(),
-                                 c.part_n.as<double>(), nullptr, c.stream),
-                   "gemv_n");
+            HIPCHK(launch_G_n(c, shape, x.as<double>(), c.part_n.as<double>(), nullptr, c.stream),
+                   "G x");
         }
         HIPCHK(launch_reduce_parts(c.part_n.as<double>(), shape.n_splits, c.Nd, c.Nd, nullptr,
                                    y.as<double>(), c.stream),
@@ -350,9 +418,8 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
         HIPCHK(c.part_t.reserve(sizeof(double) * (size_t)shape.t_chunks * c.ld), "alloc part");
         {
             TimedRegion tr(c, kTimeGemvT);
-            HIPCHK(launch_gemv_t(c.G.p, c.store, c.Nd, c.ld, shape, x.as<double>(),
-                                 c.part_t.as<double>(), nullptr, c.stream),
-                   "gemv_t");
+            HIPCHK(launch_G_t(c, shape, x.as<double>(), c.part_t.as<double>(), nullptr, c.stream),
+                   "G^T w");
         }
         HIPCHK(launch_reduce_parts(c.part_t.as<double>(), shape.t_chunks, c.ld, c.Nm, nullptr,
                                    y.as<double>(), c.stream),
@@ -405,6 +472,7 @@ gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_i
         const int64_t Nd = (int64_t)ncomp(it.field_flag) * it.N_obs, ld = (it.Nm + 63) / 64 * 64;
         cb.Nd = cb.Nm = cb.N_obs = cb.ld = 0;
         cb.gn_ready = false;
+        cb.wav = false;
         cb.have_crg_ops = false;
         cb.partn_holds_gn_model = false;
         HIPCHK(cb.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
@@ -471,6 +539,7 @@ gra_status gra_expand_cols(gra_ctx *ctx, const int64_t *split_cols, int64_t n_sp
                            const double *child_prisms)
 {
     CTX_GUARD(ctx);
+    if (c.wav) return set_error(c, GRA_BAD_ARG, "not available for the wavelet-compressed G");
     if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
     if (!split_cols || !child_prisms || n_split <= 0)
         return set_error(c, GRA_BAD_ARG, "empty split list");
@@ -534,6 +603,7 @@ gra_status gra_col_norms(gra_ctx *ctx, double *norms)
 {
     CTX_GUARD(ctx);
     if (!norms) return set_error(c, GRA_BAD_ARG, "null output");
+    if (c.wav) return set_error(c, GRA_BAD_ARG, "not available for the wavelet-compressed G");
     if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
     DevBuf &y = c.vec_nd[2];
     HIPCHK(y.reserve(sizeof(double) * (size_t)c.Nm), "alloc out");
@@ -678,7 +748,7 @@ gra_status gra_solve_cg(gra_ctx *ctx, const gra_cg_args *a, double *dx, double *
     CTX_GUARD(ctx);
     if (!a || !dx || !a->mk || !a->mref || !a->Wd || !a->dobs || !a->P)
         return set_error(c, GRA_BAD_ARG, "missing solve_cg argument");
-    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    if (!have_G(c)) return set_error(c, GRA_BAD_ARG, "no G assembled");
     if (!c.Ws.present || c.Ws.n != c.Nm)
         return set_error(c, GRA_BAD_ARG, "regularisation operators not uploaded for this Nm");
     if (a->maxit < 0 || !(a->lambda >= 0)) return set_error(c, GRA_BAD_ARG, "maxit/lambda");
@@ -771,7 +841,7 @@ gra_status gra_misfit(gra_ctx *ctx, const double *m, const double *Wd, const dou
 {
     CTX_GUARD(ctx);
     if (!m || !Wd || !dobs || !misfit) return set_error(c, GRA_BAD_ARG, "args");
-    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    if (!have_G(c)) return set_error(c, GRA_BAD_ARG, "no G assembled");
     const int64_t nd = c.Nd;
     DevBuf &x = c.vec_nm[10], &v = c.vec_nd[3];
     HIPCHK(put_padded(x, m, c.Nm, c.ld, c.stream), "m");
@@ -785,9 +855,7 @@ gra_status gra_misfit(gra_ctx *ctx, const double *m, const double *Wd, const dou
     HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * nd), "alloc");
     HIPCHK(c.red.reserve(sizeof(double) * (size_t)((nd + 255) / 256 + 1)), "alloc");
     HIPCHK(c.vec_nm[11].reserve(sizeof(double)), "alloc");
-    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, c.ld, shape, x.as<double>(), c.part_n.as<double>(),
-                         nullptr, c.stream),
-           "gemv_n");
+    HIPCHK(launch_G_n(c, shape, x.as<double>(), c.part_n.as<double>(), nullptr, c.stream), "G x");
     HIPCHK(launch_misfit(c.part_n.as<double>(), shape.n_splits, nd, v.as<double>(), v.as<double>() + nd,
                          c.red.as<double>(), c.vec_nm[11].as<double>(), c.stream),
            "misfit");
@@ -824,7 +892,7 @@ gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
     CTX_GUARD(ctx);
     if (!s || !s->dobs || !s->Wd || !s->m_min || !s->m_max || !s->mref || !s->m_ini)
         return set_error(c, GRA_BAD_ARG, "missing setup array");
-    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    if (!have_G(c)) return set_error(c, GRA_BAD_ARG, "no G assembled");
     const int64_t nd = c.Nd, nm = c.Nm, ld = c.ld;
     HIPCHK(put(c.gn_dobs, s->dobs, (size_t)nd, c.stream), "dobs");
     HIPCHK(put(c.gn_Wd, s->Wd, (size_t)nd, c.stream), "Wd");
@@ -914,9 +982,7 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
            "barrier_update");
     const GemvShape shape = gemv_shape(c);
     HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * nd), "alloc");
-    HIPCHK(launch_gemv_n(c.G.p, c.store, nd, ld, shape, mt, c.part_n.as<double>(), nullptr,
-                         c.stream),
-           "gemv_n");
+    HIPCHK(launch_G_n(c, shape, mt, c.part_n.as<double>(), nullptr, c.stream), "G x");
     HIPCHK(launch_misfit(c.part_n.as<double>(), shape.n_splits, nd, c.gn_Wd.as<double>(),
                          c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
            "misfit");
@@ -1049,6 +1115,7 @@ static hipError_t time_pair(Ctx &c, const GemvShape &s, int reps, const double *
 gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t)
 {
     CTX_GUARD(ctx);
+    if (c.wav) return set_error(c, GRA_BAD_ARG, "not available for the wavelet-compressed G");
     if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
     const double gbytes = (double)c.Nd * (double)c.Nm * (c.store == GRA_F64 ? 8 : c.store == GRA_F32 ? 4 : 2);
     if (reps <= 0) reps = gbytes < 8e9 ? 6 : 2;

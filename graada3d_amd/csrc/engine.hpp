@@ -116,6 +116,16 @@ struct Ctx {
     DevBuf status;             // int flags written by kernels
     DevBuf host_io;            // unused placeholder
 
+    // wavelet-compressed G (gra_assemble_wavelet): replaces the dense G while
+    // `wav` is set.  Rows: start / length into (column, value); columns:
+    // pointers into (row, value); n_dy-long work vectors.
+    bool wav = false;
+    int64_t wav_ndy = 0, wav_nnz = 0, wav_nlds = 0;
+    DevBuf wav_order;                    // int32, Nm: reordered position -> cell
+    DevBuf wav_rs, wav_rl, wav_ci, wav_v;
+    DevBuf wav_cp, wav_ri, wav_vt;
+    DevBuf wav_x, wav_z, wav_tmp;
+
     // resident Gauss-Newton state
     bool gn_ready = false;
     bool gn_crg = false, gn_have_S = false;   // resident loop with the cross-gradient term
@@ -200,6 +210,19 @@ hipError_t launch_col_sumsq(const void *G, gra_dtype dt, int64_t Nd, int64_t ld,
 // y[i] = scale_i * sum_s part[s*stride + i]  (scale may be null => 1)
 hipError_t launch_reduce_parts(const double *part, int nparts, int64_t stride, int64_t n,
                                const double *scale, double *y, hipStream_t st);
+
+// Products of the context's G, dense or wavelet-compressed (wavelet_kernels.hip).
+// The wavelet form writes y (Nd) / z (ld entries, zero beyond Nm) directly
+// (one split / chunk).
+hipError_t launch_G_n(Ctx &c, const GemvShape &s, const double *x, double *part, const int *done,
+                      hipStream_t st);
+hipError_t launch_G_t(Ctx &c, const GemvShape &s, const double *w, double *part, const int *done,
+                      hipStream_t st);
+int64_t wavelet_length(int64_t n);
+// compute_G_wavelet on the device: prisms already in reordered order,
+// c.wav_order uploaded; fills the wav_* buffers.
+gra_status wavelet_assemble(Ctx &c, const double *d_prisms_ro, int64_t Nm, const double *d_pts,
+                            int64_t N_obs, uint32_t flag, double rel, int *d_bad);
 
 // Copy G to a float64 host-layout device buffer (row- or column-major).
 hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,

@@ -32,9 +32,10 @@ configuration, its data and model files, and the adaptive Gauss-Newton loop.
   (InversionBase::set_dobs); depth weighting |mean(z_obs) - z_c|^(-beta/2)
   (set_depth_weighting flag 0).
 
-Supported subset: method 1 (own CG on a stored G); no padding; no
-cross-gradient or petrophysical model files (the reference interpolates them
-from gridded files with Linterp); anything else raises ValueError.
+Supported subset: methods 1 (own CG on a stored G) and 2 (own CG on the
+wavelet-compressed G); no padding; no cross-gradient or petrophysical model
+files (the reference interpolates them from gridded files with Linterp);
+anything else raises ValueError.
 """
 from __future__ import annotations
 
@@ -191,8 +192,9 @@ class AdaptiveInversion:
                  out_dir: str = ".", engine_factory=None):
         from . import operators as O
         from .octree import OctreeMesh
-        if inv.method != 1:
-            raise ValueError("only method 1 (own CG on a stored G) is supported")
+        if inv.method not in (1, 2):
+            raise ValueError("method must be 1 (stored G) or 2 (wavelet-compressed G); "
+                             "method 0 (Eigen's LSCG) is not supported")
         if any(model.padding[k] for k in (0, 2, 4)):
             raise ValueError("padding cells are not supported")
         if inv.use_crg or inv.use_petro:
@@ -211,13 +213,22 @@ class AdaptiveInversion:
             def engine_factory(prisms, points, flag):
                 return GaussNewtonInversion(prisms, points, flag, ctx=ctx)
         self.engine = engine_factory(self.mesh.prisms(), self.points, data.flag)
-        self.engine.compute_G("f64")
+        self._assemble()
         nm = self.mesh.n_cells
         self.m_min = np.full(nm, inv.m_min)
         self.m_max = np.full(nm, inv.m_max)
         self.m0 = np.zeros(nm)
         self.m = np.full(nm, inv.m_ini)
         self._build_operators()
+
+    def _assemble(self):
+        """compute_G, or compute_G_wavelet over the reordered cells for
+        method 2 (GaussNewtonInversion::invert, GaussNewtonInversion.cpp:2-29)."""
+        if self.cfg.method == 2:
+            self.wavelet = self.engine.compute_G_wavelet(self.cfg.threshold,
+                                                         self.mesh.reordered())
+        else:
+            self.engine.compute_G("f64")
 
     # -- operators (InversionBase::init_matrices; GN.cpp:711-718) ------------
     def _build_operators(self):
@@ -265,7 +276,13 @@ class AdaptiveInversion:
         if marked.size == 0:
             return 0
         split, kids = self.mesh.refine(marked)
-        self.engine.expand_G(split, kids)
+        if self.cfg.method == 2:
+            # the compressed G is built again over the new reordered cells
+            # (AdaptiveInversion.cpp:690-700)
+            self.engine.set_mesh(self.mesh.prisms())
+            self._assemble()
+        else:
+            self.engine.expand_G(split, kids)
         # children inherit the parent's values (Mesh.cpp:1591-1597)
         reps = np.ones(nm, np.int64)
         reps[split] = 8

@@ -163,6 +163,36 @@ class Fwd:
         c.check(st, "gra_assemble")
         self._assembled = True
 
+    def set_mesh(self, prisms):
+        """Fwd::set_mesh (Fwd.cpp:221-225): new cells; G must be assembled
+        again."""
+        self.prisms = _f64(prisms).reshape(-1, 6).copy()
+        self.Nm = self.prisms.shape[0]
+        self._assembled = False
+
+    def compute_G_wavelet(self, relative_threshold: float, order):
+        """Fwd::compute_G_wavelet (Fwd.cpp:148-219): G over the cells in
+        `order` (reordered position -> cell, e.g. OctreeMesh.reordered()),
+        D4-transformed per row and thresholded at relative_threshold times
+        the row's largest coefficient; the products use it until the next
+        compute_G.  Returns (n_dy, kept coefficients)."""
+        c = self.ctx
+        o = np.ascontiguousarray(order, dtype=np.int64)
+        if o.shape != (self.Nm,):
+            raise ValueError(f"order must have {self.Nm} entries")
+        st = c.L.gra_assemble_wavelet(c.h, _p(self.prisms), self.Nm, _p(self.points), self.N_obs,
+                                      self.field_flag, o.ctypes.data_as(_lib.c_i64p),
+                                      float(relative_threshold))
+        c.check(st, "gra_assemble_wavelet")
+        self._assembled = True
+        return self.wavelet_info()
+
+    def wavelet_info(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.check(self.ctx.L.gra_wavelet_info(self.ctx.h, ctypes.byref(a), ctypes.byref(b)),
+                       "wavelet_info")
+        return a.value, b.value
+
     def _need_G(self):
         if not self._assembled:
             raise GraError(1, "Fwd", "compute_G() has not been called")

@@ -10,6 +10,15 @@
   Mesh.cpp:1474-1556), so face neighbours never differ by more than one
   level.
 
+A second, "reordered" list of the leaves (Mesh::ro_leaf_cells) gives the
+column order of the wavelet-compressed G: on the regular mesh each (x, y)
+column of cells is walked down or up in z, the direction alternating from
+column to column in (x-outer, y-inner) order (Mesh.cpp:855-893); a split
+cell is replaced there by its children in the path 0, 2, 6, 4, 5, 7, 3, 1
+when its column ran down, and in the reverse path otherwise; children keep
+their parent's direction (Mesh.cpp:1605, :1946-1960).  `reordered()` maps a
+reordered position to the leaf index (Mesh::get_reordered_id).
+
 `refine(targets)` splits the given leaves (indices in the current leaf list)
 and the coarser neighbours their balance needs, and returns what
 `Fwd.expand_G` (AdaptiveInversion::expand_G) consumes: the ascending old
@@ -24,12 +33,18 @@ import numpy as np
 from .mesh import linspaced
 
 
-class _Leaf:
-    __slots__ = ("level", "key", "bounds", "orig", "children")
+# child paths of the reordered list (child index 4i + 2j + k)
+_PATH_DOWN = (0, 2, 6, 4, 5, 7, 3, 1)
+_PATH_UP = (1, 3, 7, 5, 4, 6, 2, 0)
 
-    def __init__(self, level, key, bounds, orig=None):
+
+class _Leaf:
+    __slots__ = ("level", "key", "bounds", "orig", "children", "down")
+
+    def __init__(self, level, key, bounds, orig=None, down=True):
         self.level, self.key, self.bounds, self.orig = level, key, bounds, orig
         self.children = None
+        self.down = down
 
 
 class OctreeMesh:
@@ -43,6 +58,15 @@ class OctreeMesh:
                     self.leaves.append(_Leaf(0, (i, j, k),
                                              (xs[i], xs[i + 1], ys[j], ys[j + 1], zs[k], zs[k + 1])))
         self._index = {(0,) + lf.key: lf for lf in self.leaves}
+        self.ro = []
+        down = True
+        for i in range(nx):
+            for j in range(ny):
+                col = self.leaves[(i * ny + j) * nz:(i * ny + j + 1) * nz]
+                for lf in col:
+                    lf.down = down
+                self.ro.extend(col if down else col[::-1])
+                down = not down
 
     @property
     def n_cells(self) -> int:
@@ -87,7 +111,8 @@ class OctreeMesh:
             for j in range(2):
                 for k in range(2):
                     key = (2 * lf.key[0] + i, 2 * lf.key[1] + j, 2 * lf.key[2] + k)
-                    c = _Leaf(L, key, (xb[i], xb[i + 1], yb[j], yb[j + 1], zb[k], zb[k + 1]))
+                    c = _Leaf(L, key, (xb[i], xb[i + 1], yb[j], yb[j + 1], zb[k], zb[k + 1]),
+                              down=lf.down)
                     kids.append(c)
                     self._index[(L,) + key] = c
         del self._index[(lf.level,) + lf.key]
@@ -113,7 +138,19 @@ class OctreeMesh:
             kids.extend(c.bounds for c in lf.children)
             new.extend(lf.children)
         self.leaves = new
+        ro = []
+        for lf in self.ro:
+            if lf.children is None:
+                ro.append(lf)
+            else:
+                ro.extend(lf.children[c] for c in (_PATH_DOWN if lf.down else _PATH_UP))
+        self.ro = ro
         return np.array(split, dtype=np.int64), np.array(kids, dtype=float).reshape(-1, 6)
+
+    def reordered(self) -> np.ndarray:
+        """Leaf index of each position of the reordered list."""
+        idx = {id(lf): i for i, lf in enumerate(self.leaves)}
+        return np.array([idx[id(lf)] for lf in self.ro], dtype=np.int64)
 
     def _finer_across(self, lf: _Leaf, axis: int, side: int):
         """The 4 children (level L+1) adjacent to lf's face when the cell

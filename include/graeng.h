@@ -103,6 +103,29 @@ typedef struct {
 gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item *items,
                               gra_dtype store);
 
+/*
+ * Replaces Fwd::compute_G_wavelet (src/Forward_modelling/Fwd.cpp:148-219,
+ * compress_vec :298-371; the reference's method 2).  order[j] is the cell
+ * at reordered position j (Mesh::get_reordered_id, a permutation of
+ * 0..Nm-1).  Each row of G over the cells order[0..Nm), zero-padded to
+ * n_dy = the next power of two >= Nm, is transformed by the periodic
+ * Daubechies-4 wavelet transform (GSL gsl_wavelet_daubechies, k = 4) and
+ * only its coefficients c with |c| > t or ||c| - t| < 1e-15,
+ * t = relative_threshold * max|c| of the row, are kept on the device.
+ * Until the next gra_assemble, gra_G_vec_mul, gra_GT_vec_mul, gra_misfit,
+ * gra_solve_cg and the resident Gauss-Newton loop use this compressed G
+ * (Fwd.cpp:80-147 wavelet branches); gra_copy_G, gra_col_norms,
+ * gra_expand_cols (re-assemble instead, as AdaptiveInversion::refine_mesh
+ * does) and gra_autotune return GRA_BAD_ARG, and the single-read CG
+ * iteration falls back to the two-pass one.  Any dense G is released.
+ */
+gra_status gra_assemble_wavelet(gra_ctx *ctx, const double *prisms, int64_t Nm, const double *pts,
+                                int64_t N_obs, uint32_t field_flag, const int64_t *order,
+                                double relative_threshold);
+/* n_dy and the number of kept coefficients of the compressed G
+ * (the reference prints both, Fwd.cpp:210-218). */
+gra_status gra_wavelet_info(gra_ctx *ctx, int64_t *n_dy, int64_t *nnz);
+
 /* Shape of the context's G: Nd = n_comp * N_obs rows, Nm columns. */
 gra_status gra_get_dims(const gra_ctx *ctx, int64_t *Nd, int64_t *Nm, int64_t *N_obs,
                         uint32_t *field_flag, int *store);

@@ -228,3 +228,88 @@ def irls_weights(W, m, p, eps2):
     fn.argtypes = [ctypes.POINTER(_CSR), _dp, ctypes.c_int, ctypes.c_double, _dp]
     fn(ctypes.byref(c), _ptr(_c(m)), int(p), float(eps2), _ptr(out))
     return out
+
+
+# ------------------------------------------------------------- wavelets ----
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+def d4_filters():
+    h, g = np.zeros(4), np.zeros(4)
+    lib().ora_d4_filters(_ptr(h), _ptr(g))
+    return h, g
+
+
+def d4_transform(a, forward=True):
+    """The full periodic D4 transform of a (length a power of two)."""
+    a = _c(a).copy()
+    fn = lib().ora_d4_transform
+    fn.argtypes = [_dp, ctypes.c_int64, ctypes.c_int]
+    if fn(_ptr(a), a.shape[0], 1 if forward else -1):
+        raise ValueError("length must be a power of two >= 2")
+    return a
+
+
+def wavelet_length(n):
+    fn = lib().ora_wavelet_length
+    fn.argtypes = [ctypes.c_int64]
+    fn.restype = ctypes.c_int64
+    return int(fn(int(n)))
+
+
+class WaveletG:
+    """compute_G_wavelet's compressed rows: CSR (rowptr, ci, v) over the
+    n_dy wavelet coefficients, with the cell order it was built for."""
+
+    def __init__(self, prisms, pts, flag, order, rel):
+        prisms, pts = _c(prisms), _c(pts)
+        self.order = np.ascontiguousarray(order, dtype=np.int64)
+        self.Nm = prisms.shape[0]
+        self.n_dy = wavelet_length(self.Nm)
+        self.n_rows = n_components(flag) * pts.shape[0]
+        fn = lib().ora_wavelet_rows
+        fn.argtypes = [_dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_uint32, _i64p,
+                       ctypes.c_double, ctypes.c_int64, _i64p, _i64p, _i32p, _dp]
+        counts = np.zeros(self.n_rows, np.int64)
+        args = (_ptr(prisms), self.Nm, _ptr(pts), pts.shape[0], flag,
+                self.order.ctypes.data_as(_i64p), float(rel), self.n_dy)
+        st = fn(*args, counts.ctypes.data_as(_i64p), None, None, None)
+        self.rowptr = np.zeros(self.n_rows + 1, np.int64)
+        np.cumsum(counts, out=self.rowptr[1:])
+        self.ci = np.zeros(self.rowptr[-1], np.int32)
+        self.v = np.zeros(self.rowptr[-1])
+        fn(*args, counts.ctypes.data_as(_i64p), self.rowptr.ctypes.data_as(_i64p),
+           self.ci.ctypes.data_as(_i32p), _ptr(self.v))
+        self.status = st
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+    def _prod(self, name, x, n_out):
+        fn = getattr(lib(), name)
+        fn.argtypes = [_i64p, _i32p, _dp, ctypes.c_int64, ctypes.c_int64, _i64p, ctypes.c_int64,
+                       _dp, _dp]
+        y = np.zeros(n_out)
+        fn(self.rowptr.ctypes.data_as(_i64p), self.ci.ctypes.data_as(_i32p), _ptr(self.v),
+           self.n_rows, self.n_dy, self.order.ctypes.data_as(_i64p), self.Nm, _ptr(_c(x)), _ptr(y))
+        return y
+
+    def G_vec_mul(self, x):
+        return self._prod("ora_wavelet_gemv_n", x, self.n_rows)
+
+    def GT_vec_mul(self, r):
+        return self._prod("ora_wavelet_gemv_t", r, self.Nm)
+
+    def dense(self):
+        """The operator as a dense (n_rows x Nm) matrix in the original cell
+        order: row i = W^T c_i mapped back through the cell order."""
+        out = np.zeros((self.n_rows, self.Nm))
+        for i in range(self.n_rows):
+            a = np.zeros(self.n_dy)
+            sl = slice(self.rowptr[i], self.rowptr[i + 1])
+            a[self.ci[sl]] = self.v[sl]
+            out[i, self.order] = d4_transform(a, forward=False)[:self.Nm]
+        return out

@@ -50,6 +50,19 @@ class OracleGN:
     def Nm(self):
         return self.prisms.shape[0]
 
+    def set_mesh(self, prisms):
+        self.prisms = np.asarray(prisms, float).copy()
+        self.G = None
+
+    def compute_G_wavelet(self, relative_threshold, order):
+        """The oracle's compressed G; the products and the CG then use the
+        operator it represents as a dense matrix."""
+        import oracle
+        self.wav = oracle.WaveletG(self.prisms, self.points, self.flag, order, relative_threshold)
+        assert self.wav.status == 0
+        self.G = self.wav.dense()
+        return self.wav.n_dy, self.wav.nnz
+
     def compute_G(self, store="f64"):
         import oracle
         self.G, st = oracle.assemble(self.prisms, self.points, self.flag)

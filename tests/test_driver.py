@@ -67,7 +67,7 @@ def test_unsupported_options(tmp_path):
     with pytest.raises(ValueError, match="padding"):
         driver.AdaptiveInversion(d, replace(m, padding=(2, 1.5, 0, 0, 0, 0)), inv,
                                  engine_factory=OracleGN)
-    bad = replace(inv, method=2)
+    bad = replace(inv, method=0)
     with pytest.raises(ValueError, match="method"):
         driver.AdaptiveInversion(d, m, bad, engine_factory=OracleGN)
 
@@ -111,3 +111,21 @@ def test_reference_run_pins_solve_cg_quirk(tmp_path):
         mis = np.array([t["misfit"] for t in run.trials])
         dev = np.abs(mis[1:5] / fx["misfit"][1:5] - 1)
         assert lo <= dev.min() and dev.max() <= hi, (factory.__name__, dev)
+
+
+def test_driver_method2_threshold0_matches_reference_run(tmp_path):
+    """Method 2 with a zero threshold keeps every wavelet coefficient, so the
+    compressed operator equals G and the run must replay the reference's
+    method-1 run (the compressed G is rebuilt over the reordered cells after
+    the refinement)."""
+    cfg, fx = stage_karst("raw", tmp_path)
+    d, m, inv = driver.read_config(cfg)
+    inv = replace(inv, method=2, threshold=0.0)
+    out = tmp_path / "out"
+    out.mkdir()
+    run = driver.AdaptiveInversion(d, m, inv, out_dir=str(out), engine_factory=OracleGN)
+    n_dy, nnz = run.wavelet
+    assert n_dy == 4096 and nnz == 75 * 4096
+    res = run.run()
+    assert res["Nm"] == 4239 and res["iterations"] == 3
+    check_against_run(run, fx, str(out))
