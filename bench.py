@@ -439,6 +439,9 @@ def run_karst(args):
     times, runs = [], []
     for k in range(warm + steps):
         d, m, inv = driver.read_config(cfg)
+        if args.karst_method == 2:
+            from dataclasses import replace
+            inv = replace(inv, method=2, threshold=args.karst_threshold)
         ctx.timing(k >= warm)
         ctx.timing_reset()
         t0 = time.perf_counter()
@@ -460,17 +463,21 @@ def run_karst(args):
     trials = run.trials
     its = np.array([t["cg_iterations"] for t in trials], float)
     nms = np.array([t["Nm"] for t in trials], float)
+    # bytes of one product: the dense G (8 B per entry) or the compressed
+    # rows (8 B value + 4 B index per kept coefficient)
+    per = (np.array([12.0 * t["nnz"] for t in trials]) if args.karst_method == 2
+           else nd * nms * 8)
 
     def rate(tm):
         avg_ms, n = tm
         extra = (n - its.sum()) / len(trials)
-        byt = float(((its + extra) * nd * nms * 8).sum())
+        byt = float(((its + extra) * per).sum())
         return byt / (avg_ms * n * 1e-3) / 1e9, avg_ms * n
     gbn, msn = rate(gn)
     gbt, mst = rate(gt)
     frac = (gbn * msn + gbt * mst) / (msn + mst) / HBM_PEAK_GBS
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.karst_method == 1:
         cpu = karst_cpu_baseline(cfg, trials, res["Nm"])
     sec = float(np.mean(times))
     out = {
@@ -480,7 +487,11 @@ def run_karst(args):
         "vs_baseline": None, "dtype": "f64", "data": "reference example data (Filtered_anomaly.dat)",
         "config": {"workload": f"cave inversion: {nd} gz points, 16x13x12 octree -> {res['Nm']} "
                                f"cells, {len(trials)} lambda trials, "
-                               f"{int(its.sum())} CG iterations, final misfit {res['misfit']:.9g}"},
+                               f"{int(its.sum())} CG iterations, final misfit {res['misfit']:.9g}"
+                               + (f", method 2 (wavelet-compressed G, threshold "
+                                  f"{args.karst_threshold:g}: {trials[-1]['nnz']} of "
+                                  f"{nd * run.wavelet[0]} coefficients kept)"
+                                  if args.karst_method == 2 else ", method 1 (dense G)")},
         "roofline": {"bound": "hbm", "achieved": (gbn * msn + gbt * mst) / (msn + mst),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": frac, "traffic": None},
         "gemv_GBps": {"n": gbn, "t": gbt}, "gemv_share_of_wall": (msn + mst) * 1e-3 / sec,
@@ -548,6 +559,10 @@ def main():
     ap.add_argument("--no-fused", dest="fused", action="store_false",
                     help="skip the single-read (fused) CG iteration variants")
     ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--karst-method", type=int, default=1, choices=[1, 2],
+                    help="karst: dense G (1) or wavelet-compressed G (2)")
+    ap.add_argument("--karst-threshold", type=float, default=5e-4,
+                    help="karst: relative threshold of the wavelet compression (method 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine's "

@@ -120,10 +120,12 @@ struct Ctx {
     // `wav` is set.  Rows: start / length into (column, value); columns:
     // pointers into (row, value); n_dy-long work vectors.
     bool wav = false;
-    int64_t wav_ndy = 0, wav_nnz = 0, wav_nlds = 0;
+    int64_t wav_ndy = 0, wav_nnz = 0, wav_nlds = 0, wav_cus = 256;
     DevBuf wav_order;                    // int32, Nm: reordered position -> cell
     DevBuf wav_rs, wav_rl, wav_ci, wav_v;
     DevBuf wav_cp, wav_ri, wav_vt;
+    DevBuf wav_seg_start, wav_seg_len, wav_col_seg, wav_part;   // CSC column segments
+    int64_t wav_nseg = 0;
     DevBuf wav_x, wav_z, wav_tmp;
 
     // resident Gauss-Newton state

@@ -38,7 +38,6 @@ constexpr int kBlk = 256;
 constexpr int kLdsThreads = 1024;
 constexpr int64_t kLdsMaxN = 16384;                     // LDS window cap (128 KiB)
 constexpr int kFwdPer = (int)(kLdsMaxN / 2 / kLdsThreads);   // (s, d) pairs per thread
-constexpr int kInvPer = (int)(kLdsMaxN / kLdsThreads);       // outputs per thread
 
 // Daubechies (1988) 4-tap orthonormal filters: h = ((1+sqrt3), (3+sqrt3),
 // (3-sqrt3), (1-sqrt3)) / (4 sqrt2); g[k] = (-1)^k h[3-k].
@@ -102,17 +101,21 @@ dwt_lds_kernel(const double *in, double *out, int64_t ld, int64_t n0,
         for (int64_t j = tid; j < n0; j += kLdsThreads) lds[j] = a[j];
     }
     __syncthreads();
+    // one thread per (s, d) pair: 16-byte LDS reads of the even-aligned
+    // sample pairs (2ii, 2ii+1) and (2ii+2, 2ii+3) forward; the inverse
+    // produces outputs 2t and 2t+1 from s[t], d[t], s[t-1], d[t-1]
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    v2 *lds2 = reinterpret_cast<v2 *>(lds);
     if (FWD) {
         for (int64_t n = n0; n >= 2; n >>= 1) {
-            const int64_t nh = n >> 1, mask = n - 1;
+            const int64_t nh = n >> 1;
             double s[kFwdPer], d[kFwdPer];
 #pragma unroll
             for (int q = 0; q < kFwdPer; ++q) {
                 const int64_t ii = tid + (int64_t)q * kLdsThreads;
                 if (ii < nh) {
-                    const int64_t i = 2 * ii;
-                    analysis(lds[i], lds[(i + 1) & mask], lds[(i + 2) & mask], lds[(i + 3) & mask],
-                             s[q], d[q]);
+                    const v2 a = lds2[ii], b = lds2[(ii + 1) & (nh - 1)];
+                    analysis(a.x, a.y, b.x, b.y, s[q], d[q]);
                 }
             }
             __syncthreads();
@@ -128,18 +131,27 @@ dwt_lds_kernel(const double *in, double *out, int64_t ld, int64_t n0,
         }
     } else {
         for (int64_t n = 2; n <= n0; n <<= 1) {
-            double o[kInvPer];
-            auto at = [&](int64_t p) { return lds[p]; };
+            const int64_t nh = n >> 1;
+            v2 o[kFwdPer];
 #pragma unroll
-            for (int q = 0; q < kInvPer; ++q) {
-                const int64_t j = tid + (int64_t)q * kLdsThreads;
-                if (j < n) o[q] = synthesis(j, n, at);
+            for (int q = 0; q < kFwdPer; ++q) {
+                const int64_t t = tid + (int64_t)q * kLdsThreads;
+                if (t < nh) {
+                    const int64_t tb = (t - 1) & (nh - 1);
+                    const double sa = lds[t], da = lds[t + nh], sb = lds[tb], db = lds[tb + nh];
+                    // output 2t: k = 0 at t, k = 2 at t-1; output 2t+1: k = 1, 3
+                    const double ta0 = kH0 * sa + kH3 * da, tb0 = kH2 * sb + kH1 * db;
+                    const double ta1 = kH1 * sa + (-kH2) * da, tb1 = kH3 * sb + (-kH0) * db;
+                    const bool b_first = tb < t;
+                    o[q].x = b_first ? tb0 + ta0 : ta0 + tb0;
+                    o[q].y = b_first ? tb1 + ta1 : ta1 + tb1;
+                }
             }
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < kInvPer; ++q) {
-                const int64_t j = tid + (int64_t)q * kLdsThreads;
-                if (j < n) lds[j] = o[q];
+            for (int q = 0; q < kFwdPer; ++q) {
+                const int64_t t = tid + (int64_t)q * kLdsThreads;
+                if (t < nh) lds2[t] = o[q];
             }
             __syncthreads();
         }
@@ -328,38 +340,109 @@ gather_csc_kernel(const int32_t *__restrict__ perm, const int32_t *__restrict__ 
 
 // ------------------------------------------------------------------- SpMV
 
-// y[r] = sum_k v[k] * xw[ci[k]] over row r's entries (one wave per row).
+// Sparse dot products of one wave with a gathered vector: sum over k in
+// [b, e) of val[k] * x[idx[k]].  Each lane keeps kSpU entries in flight (the
+// index loads, then the value loads and gathers, are issued together); the
+// kSpU partial sums are added in a fixed order, then the wave butterfly, so
+// the result is repeatable.  Indices past e are clamped (no conditional
+// loads) and their products dropped.
+constexpr int kSpU = 8;
+constexpr int64_t kSegLen = 2048;   // entries per column segment of the CSC product
+
+template <typename X>
+__device__ inline double wave_sparse_dot(const int32_t *__restrict__ idx,
+                                         const double *__restrict__ val, X x, int64_t b,
+                                         int64_t e, int l)
+{
+    double acc[kSpU];
+#pragma unroll
+    for (int u = 0; u < kSpU; ++u) acc[u] = 0.0;
+    const int64_t last = e - 1;
+    for (int64_t k0 = b + l; k0 < e; k0 += (int64_t)kWave * kSpU) {
+        int32_t c[kSpU];
+        double vv[kSpU], g[kSpU];
+#pragma unroll
+        for (int u = 0; u < kSpU; ++u) {
+            const int64_t k = k0 + (int64_t)kWave * u;
+            c[u] = idx[k < e ? k : last];
+        }
+#pragma unroll
+        for (int u = 0; u < kSpU; ++u) {
+            const int64_t k = k0 + (int64_t)kWave * u;
+            vv[u] = val[k < e ? k : last];
+            g[u] = x(c[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kSpU; ++u) {
+            const int64_t k = k0 + (int64_t)kWave * u;
+            if (k < e) acc[u] += vv[u] * g[u];
+        }
+    }
+    double s = acc[0];
+#pragma unroll
+    for (int u = 1; u < kSpU; ++u) s += acc[u];
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+    return s;
+}
+
+// The same products with the gathered vector (n entries) staged in LDS: one
+// workgroup of kSpWaves waves per `per_block` consecutive outputs; each
+// wave takes every kSpWaves-th of them.  Output r sums entries
+// [start[r], start[r] + len[r]).  Used when the vector fits the LDS window:
+// the gathers then hit LDS instead of scattered L2 lines.
+constexpr int kSpWaves = 16;
+
+__global__ void __launch_bounds__(kSpWaves * kWave)
+spmv_lds_kernel(const int64_t *__restrict__ start, const int32_t *__restrict__ len,
+                const int32_t *__restrict__ idx, const double *__restrict__ val,
+                const double *__restrict__ x, int64_t n, int64_t nout, int64_t per_block,
+                double *__restrict__ y, const int *done)
+{
+    extern __shared__ double xs[];
+    if (finished(done)) return;
+    for (int64_t j = threadIdx.x; j < n; j += kSpWaves * kWave) xs[j] = x[j];
+    __syncthreads();
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    const int64_t r0 = (int64_t)blockIdx.x * per_block;
+    const int64_t r1 = r0 + per_block < nout ? r0 + per_block : nout;
+    auto gx = [&](int32_t c) { return xs[c]; };
+    for (int64_t r = r0 + w; r < r1; r += kSpWaves) {
+        const int64_t b = start[r];
+        const double s = wave_sparse_dot(idx, val, gx, b, b + len[r], l);
+        if (l == 0) y[r] = s;
+    }
+}
+
+// Output r = sum of entries [start[r], start[r] + len[r]) with global
+// gathers (one wave per output).
 __global__ void __launch_bounds__(kBlk)
-spmv_rows_kernel(const int64_t *__restrict__ rs, const int32_t *__restrict__ rl,
-                 const int32_t *__restrict__ ci, const double *__restrict__ v,
-                 const double *__restrict__ xw, int64_t nrows, double *__restrict__ y,
+spmv_rows_kernel(const int64_t *__restrict__ start, const int32_t *__restrict__ len,
+                 const int32_t *__restrict__ idx, const double *__restrict__ val,
+                 const double *__restrict__ x, int64_t nout, double *__restrict__ y,
                  const int *done)
 {
     if (finished(done)) return;
     const int64_t r = (int64_t)blockIdx.x * (kBlk / kWave) + threadIdx.x / kWave;
     const int l = threadIdx.x % kWave;
-    if (r >= nrows) return;
-    const int64_t b = rs[r], e = b + rl[r];
-    double s = 0.0;
-    for (int64_t k = b + l; k < e; k += kWave) s += v[k] * xw[ci[k]];
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (r >= nout) return;
+    const int64_t b = start[r];
+    const double s = wave_sparse_dot(idx, val, [&](int32_t c) { return x[c]; }, b, b + len[r], l);
     if (l == 0) y[r] = s;
 }
 
-// z[c] = sum over column c's entries of vt * w[ri] (one wave per column).
+// z[c] = the column's segment sums in order (columns are split into
+// segments of at most kSegLen entries so that the long coarse-scale
+// columns spread over many waves).
 __global__ void __launch_bounds__(kBlk)
-spmv_cols_kernel(const int64_t *__restrict__ cp, const int32_t *__restrict__ ri,
-                 const double *__restrict__ vt, const double *__restrict__ w, int64_t ncols,
-                 double *__restrict__ z, const int *done)
+segment_sum_kernel(const int64_t *__restrict__ col_seg, const double *__restrict__ part,
+                   int64_t ncols, double *__restrict__ z, const int *done)
 {
     if (finished(done)) return;
-    const int64_t c = (int64_t)blockIdx.x * (kBlk / kWave) + threadIdx.x / kWave;
-    const int l = threadIdx.x % kWave;
+    const int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x;
     if (c >= ncols) return;
     double s = 0.0;
-    for (int64_t k = cp[c] + l; k < cp[c + 1]; k += kWave) s += vt[k] * w[ri[k]];
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
-    if (l == 0) z[c] = s;
+    for (int64_t q = col_seg[c]; q < col_seg[c + 1]; ++q) s += part[q];
+    z[c] = s;
 }
 
 unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
@@ -407,6 +490,8 @@ static int64_t lds_window(int device)
         if (hipFuncSetAttribute((const void *)dwt_lds_kernel<true>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, b) != hipSuccess ||
             hipFuncSetAttribute((const void *)dwt_lds_kernel<false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, b) != hipSuccess ||
+            hipFuncSetAttribute((const void *)spmv_lds_kernel,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, b) != hipSuccess) {
             (void)hipGetLastError();
             n = 65536 / sizeof(double);
@@ -555,6 +640,38 @@ gra_status wavelet_assemble(Ctx &c, const double *d_prisms_ro, int64_t Nm, const
     hipLaunchKernelGGL(col_ptr_kernel, dim3(blocks_for(n + 1, kBlk)), dim3(kBlk), 0, st,
                        keys_out.as<int32_t>(), nnz, n, c.wav_cp.as<int64_t>());
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "CSC");
+    // column segments of at most kSegLen entries for the G^T product
+    {
+        std::vector<int64_t> h_cp((size_t)n + 1), h_cs((size_t)n + 1), h_ss;
+        std::vector<int32_t> h_sl;
+        if ((e = hipMemcpyAsync(h_cp.data(), c.wav_cp.p, sizeof(int64_t) * (size_t)(n + 1),
+                                hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            return hip_fail(c, e, "column pointers");
+        for (int64_t col = 0; col < n; ++col) {
+            h_cs[(size_t)col] = (int64_t)h_ss.size();
+            for (int64_t b = h_cp[(size_t)col]; b < h_cp[(size_t)col + 1]; b += kSegLen) {
+                h_ss.push_back(b);
+                h_sl.push_back((int32_t)std::min<int64_t>(kSegLen, h_cp[(size_t)col + 1] - b));
+            }
+        }
+        h_cs[(size_t)n] = (int64_t)h_ss.size();
+        const size_t ns = std::max<size_t>(h_ss.size(), 1);
+        if ((e = c.wav_seg_start.reserve(sizeof(int64_t) * ns)) != hipSuccess ||
+            (e = c.wav_seg_len.reserve(sizeof(int32_t) * ns)) != hipSuccess ||
+            (e = c.wav_part.reserve(sizeof(double) * ns)) != hipSuccess ||
+            (e = c.wav_col_seg.reserve(sizeof(int64_t) * (size_t)(n + 1))) != hipSuccess ||
+            (!h_ss.empty() &&
+             ((e = hipMemcpyAsync(c.wav_seg_start.p, h_ss.data(), sizeof(int64_t) * h_ss.size(),
+                                  hipMemcpyHostToDevice, st)) != hipSuccess ||
+              (e = hipMemcpyAsync(c.wav_seg_len.p, h_sl.data(), sizeof(int32_t) * h_sl.size(),
+                                  hipMemcpyHostToDevice, st)) != hipSuccess)) ||
+            (e = hipMemcpyAsync(c.wav_col_seg.p, h_cs.data(), sizeof(int64_t) * (size_t)(n + 1),
+                                hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            return hip_fail(c, e, "column segments");
+        c.wav_nseg = (int64_t)h_ss.size();
+    }
     if ((e = c.wav_x.reserve(sizeof(double) * (size_t)n)) != hipSuccess ||
         (e = c.wav_z.reserve(sizeof(double) * (size_t)n)) != hipSuccess ||
         (e = c.wav_tmp.reserve(sizeof(double) * (size_t)n)) != hipSuccess)
@@ -563,6 +680,11 @@ gra_status wavelet_assemble(Ctx &c, const double *d_prisms_ro, int64_t Nm, const
     c.wav_ndy = n;
     c.wav_nnz = nnz;
     c.wav_nlds = nlds;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    c.wav_cus = cus;
     return GRA_OK;
 }
 
@@ -586,9 +708,17 @@ static hipError_t wavelet_n(Ctx &c, const double *x, double *y, const int *done,
         hipError_t e = dwt_rows_forward(xw, tmp, 1, n, n, nlds, done, st);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(spmv_rows_kernel, dim3(blocks_for(c.Nd, kBlk / kWave)), dim3(kBlk), 0, st,
-                       c.wav_rs.as<int64_t>(), c.wav_rl.as<int32_t>(), c.wav_ci.as<int32_t>(),
-                       c.wav_v.as<double>(), xw, c.Nd, y, done);
+    if (n <= nlds) {
+        const int64_t per = (c.Nd + c.wav_cus - 1) / c.wav_cus;
+        hipLaunchKernelGGL(spmv_lds_kernel, dim3(blocks_for(c.Nd, (int)per)),
+                           dim3(kSpWaves * kWave), (size_t)n * sizeof(double), st,
+                           c.wav_rs.as<int64_t>(), c.wav_rl.as<int32_t>(), c.wav_ci.as<int32_t>(),
+                           c.wav_v.as<double>(), xw, n, c.Nd, per, y, done);
+    } else {
+        hipLaunchKernelGGL(spmv_rows_kernel, dim3(blocks_for(c.Nd, kBlk / kWave)), dim3(kBlk), 0,
+                           st, c.wav_rs.as<int64_t>(), c.wav_rl.as<int32_t>(),
+                           c.wav_ci.as<int32_t>(), c.wav_v.as<double>(), xw, c.Nd, y, done);
+    }
     return hipGetLastError();
 }
 
@@ -599,9 +729,24 @@ static hipError_t wavelet_t(Ctx &c, const double *w, double *z, int64_t out_len,
     const int64_t n = c.wav_ndy, nlds = c.wav_nlds;
     double *zw = c.wav_z.as<double>(), *tmp = c.wav_tmp.as<double>();
     const int32_t *order = c.wav_order.as<int32_t>();
-    hipLaunchKernelGGL(spmv_cols_kernel, dim3(blocks_for(n, kBlk / kWave)), dim3(kBlk), 0, st,
-                       c.wav_cp.as<int64_t>(), c.wav_ri.as<int32_t>(), c.wav_vt.as<double>(), w,
-                       n, zw, done);
+    const int64_t nseg = c.wav_nseg;
+    double *part = c.wav_part.as<double>();
+    if (nseg == 0) {
+        // no coefficient kept: segment_sum writes zeros
+    } else if (c.Nd <= nlds) {
+        const int64_t per = (nseg + c.wav_cus - 1) / c.wav_cus;
+        hipLaunchKernelGGL(spmv_lds_kernel, dim3(blocks_for(nseg, (int)per)),
+                           dim3(kSpWaves * kWave), (size_t)c.Nd * sizeof(double), st,
+                           c.wav_seg_start.as<int64_t>(), c.wav_seg_len.as<int32_t>(),
+                           c.wav_ri.as<int32_t>(), c.wav_vt.as<double>(), w, c.Nd, nseg, per,
+                           part, done);
+    } else {
+        hipLaunchKernelGGL(spmv_rows_kernel, dim3(blocks_for(nseg, kBlk / kWave)), dim3(kBlk), 0,
+                           st, c.wav_seg_start.as<int64_t>(), c.wav_seg_len.as<int32_t>(),
+                           c.wav_ri.as<int32_t>(), c.wav_vt.as<double>(), w, nseg, part, done);
+    }
+    hipLaunchKernelGGL(segment_sum_kernel, dim3(blocks_for(n, kBlk)), dim3(kBlk), 0, st,
+                       c.wav_col_seg.as<int64_t>(), part, n, zw, done);
     if (n <= nlds) {
         if (n >= 2)
             hipLaunchKernelGGL((dwt_lds_kernel<false>), dim3(1), dim3(kLdsThreads),

@@ -335,7 +335,8 @@ class AdaptiveInversion:
                 misfit = e.misfit(m_trial, self.Wd, self.dobs)
                 lam_log.write(f"{lam:25.14e}{misfit:25.14e}\n")
                 self.trials.append(dict(lam=lam, misfit=misfit, cg_iterations=int(cg_its),
-                                        cg_error=float(cg_err), Nm=nm))
+                                        cg_error=float(cg_err), Nm=nm,
+                                        nnz=self.wavelet[1] if c.method == 2 else None))
                 if verbose:
                     print(f"  #{j} lambda={lam:g}, misfit={misfit:g} (CG iterations: {cg_its}, "
                           f"error: {cg_err:g})", flush=True)
