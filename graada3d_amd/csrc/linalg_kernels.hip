@@ -299,8 +299,13 @@ GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm)
     int64_t max_by_cols = Nm / (4 * V * kWave) > 0 ? Nm / (4 * V * kWave) : 1;
     sp = sp > max_by_cols ? max_by_cols : sp;
     s.n_splits = (int)(sp < 1 ? 1 : (sp > 64 ? 64 : sp));
+    // G^T w: about 224 workgroups (column blocks x row chunks), never just
+    // above a multiple of the 256 CUs -- a launch of 260 runs its last few
+    // blocks as a second wave and loses up to 25 % of the bandwidth (swept
+    // over Nd 2k-60k, Nm 2.5k-64k on MI355X)
     int64_t bx = (Nm + kBlk * V - 1) / (kBlk * V);
-    int64_t ch = (256 + bx - 1) / bx;
+    int64_t ch = 224 / bx > 0 ? 224 / bx : 1;
+    if (bx * ch < 160 && bx * (ch + 1) <= 256) ++ch;
     int64_t max_by_rows = Nd / 32 > 0 ? Nd / 32 : 1;
     ch = ch > max_by_rows ? max_by_rows : ch;
     s.t_chunks = (int)(ch < 1 ? 1 : (ch > 256 ? 256 : ch));
