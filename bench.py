@@ -361,7 +361,8 @@ def run_serving(args):
     reqs = S.request_stream(args.rate, args.duration, args.seed, pool)
     print(f"[bench] serving: {len(reqs)} requests over {args.duration} s at {args.rate}/s",
           file=sys.stderr, flush=True)
-    res = S.serve(reqs, pool, progress=50, forward_method=args.forward_method)
+    res = S.serve(reqs, pool, progress=50, forward_method=args.forward_method,
+                  workers=args.workers)
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -395,11 +396,13 @@ def run_serving(args):
                                f"(seed {args.seed}), 80% matrix-free forward (5k-130k cells, "
                                f"1k-20k points, 1-9 comps, FP32 out, {args.forward_method} "
                                f"form), 20% 20-iteration CGLS on "
-                               f"resident G (pool: " + ", ".join(p.name for p in pool) + ")"},
+                               f"resident G (pool: " + ", ".join(p.name for p in pool) + "), "
+                               f"{args.workers} server streams"},
         "latency_ms": res["latency_ms"], "service_ms": res["service_ms"],
         "by_kind": res["by_kind"], "cg_warm_service_ms": res["cg_warm_service_ms"],
         "offered_rate": res["offered_rate"], "achieved_rate": res["achieved_rate"],
         "sustainable_rate": res["sustainable_rate"], "utilization": res["utilization"],
+        "mean_in_service": res["mean_in_service"], "workers": res["workers"],
         "cache": res["cache"],
         "cpu_baseline": cpu,
     }
@@ -548,6 +551,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: arrival window, s")
     ap.add_argument("--seed", type=int, default=20240605, help="serving: stream seed")
+    ap.add_argument("--workers", type=int, default=3,
+                    help="serving: server threads, each with its own stream")
     ap.add_argument("--forward-method", default="nodes", choices=["nodes", "prisms"],
                     help="serving: node-sum (rectilinear) or per-prism face/edge forward")
     ap.add_argument("--steps", type=int, default=None)

@@ -35,6 +35,41 @@ namespace {
 constexpr int kBlk = 256;
 constexpr double kG0 = 6.673e-11;
 
+// Natural logarithm of a positive normal double by table reduction (Tang's
+// method with left table points): v = 2^e m, m in [1, 2); F_j = 1 + j/128
+// for the top 7 mantissa bits j; with G_j = 1/F_j (rounded) and
+// T_j = -ln G_j,
+//     ln v = e ln2 + T_j + ln(1 + r),  r = m G_j - 1 in (-2^-52, 2^-7),
+// ln(1 + r) by its Taylor series to r^8 (truncation < r^9/9 < 2^-66).
+// ln2 is split (fdlibm's ln2_hi / ln2_lo) so that e ln2_hi is exact.  About
+// a quarter of the instructions of the general library log; within ~1 ulp
+// (absolute error <= 1.2e-16 for v in [0.5, 2]).  Arguments are clamped to
+// [2^-1000, 2^1000]: here v = c + r with r > |c| and the points strictly
+// above the mesh, so v >= z^2 / (2 r) stays far inside that range.
+constexpr int kLogBits = 7;
+constexpr int kLogTable = 1 << kLogBits;
+
+__device__ inline double table_log(double v, const double *G, const double *T)
+{
+    v = fmin(fmax(v, 9.33263618503218879e-302), 1.07150860718626732e301);
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const int e = (int)(b >> 52) - 1023;
+    const int j = (int)((b >> (52 - kLogBits)) & (kLogTable - 1));
+    const double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) |
+                                                      0x3FF0000000000000ull));
+    const double r = fma(m, G[j], -1.0);
+    double p = 1.0 / 8.0;
+    p = fma(p, -r, 1.0 / 7.0);
+    p = fma(p, r, -1.0 / 6.0);
+    p = fma(p, r, 1.0 / 5.0);
+    p = fma(p, r, -0.25);
+    p = fma(p, r, 1.0 / 3.0);
+    p = fma(p, r, -0.5);
+    p = fma(p * r, r, r);
+    const double ed = (double)e;
+    return fma(ed, 6.93147180369123816490e-01, T[j]) + fma(ed, 1.90821492927058770002e-10, p);
+}
+
 // Node weights, stored level by level: w[k * (NX*NY) + i * NY + j], so the
 // threads of a block (consecutive node columns) read consecutive addresses.
 __global__ void node_weights_kernel(const double *__restrict__ rho, int64_t nx, int64_t ny,
@@ -63,17 +98,40 @@ __global__ void node_weights_kernel(const double *__restrict__ rho, int64_t nx, 
 }
 
 // One block per observation point; a thread takes node columns (i, j), with
-// x and y fixed, and walks their nz+1 nodes.
+// x and y fixed, and walks their nz+1 nodes.  The atan arguments use one
+// reciprocal per node (1/r) with 1/x, 1/y per column and 1/z per level (LDS);
+// when all three atans are needed the third follows from the corner identity
+//   atan(xy/(zr)) + atan(yz/(xr)) + atan(xz/(yr)) = (pi/2) sgn(x) sgn(y)
+// (z > 0; sgn(+0) = +1, which is also the limit the x = 0 / y = 0 / axis
+// cases take).
+constexpr int kMaxLevels = 1024;
+
 __global__ void __launch_bounds__(kBlk)
 node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys,
                     const double *__restrict__ zs, int64_t nx, int64_t ny, int64_t nz,
                     const double *__restrict__ w, const double *__restrict__ pts, int64_t N_obs,
                     uint32_t flag, double *__restrict__ out)
 {
+    __shared__ double zl[kMaxLevels], izl[kMaxLevels];
+    __shared__ double lg_G[kLogTable], lg_T[kLogTable];
     const int64_t ip = blockIdx.x;
     if (ip >= N_obs) return;
+    for (int t = threadIdx.x; t < kLogTable; t += kBlk) {
+        const double g = 1.0 / (1.0 + t * (1.0 / kLogTable));
+        lg_G[t] = g;
+        lg_T[t] = -log(g);
+    }
     const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
     const int NY = (int)ny + 1, NZ = (int)nz + 1, NCOL = ((int)nx + 1) * NY;
+    const bool lds_levels = NZ <= kMaxLevels;
+    if (lds_levels) {
+        for (int k = threadIdx.x; k < NZ; k += kBlk) {
+            const double z = zs[k] - pz;
+            zl[k] = z;
+            izl[k] = 1.0 / z;
+        }
+    }
+    __syncthreads();
     // which transcendental terms the requested components need (uniform)
     const bool need_lx = flag & ((1u << 1) | (1u << 3) | (1u << 6));
     const bool need_ly = flag & ((1u << 1) | (1u << 2) | (1u << 5));
@@ -81,6 +139,8 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
     const bool need_axy = flag & ((1u << 1) | (1u << 4));
     const bool need_ayz = flag & ((1u << 2) | (1u << 7));
     const bool need_axz = flag & ((1u << 3) | (1u << 9));
+    const bool all_atan = need_axy && need_ayz && need_axz;
+    const bool any_atan = need_axy || need_ayz || need_axz;
     double acc[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) acc[q] = 0.0;
@@ -89,16 +149,34 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
         const double x = xs[i] - px, y = ys[j] - py;
         const double xy2 = fma(x, x, y * y);
         const bool axis = (x == 0.0) && (y == 0.0);
+        const double xy = x * y, ix = 1.0 / x, iy = 1.0 / y;
+        // (pi/2) sgn(x) sgn(y) with the sign of a zero: the limits 1/(+-0)
+        // gives the atans it replaces
+        const double half_pi_s = copysign(1.5707963267948966, x) * copysign(1.0, y);
         for (int k = 0; k < NZ; ++k) {
             const double wn = w[(int64_t)k * NCOL + col];
-            const double z = zs[k] - pz;
+            const double z = lds_levels ? zl[k] : zs[k] - pz;
             const double r = sqrt(xy2 + z * z);
-            const double lx = need_lx ? log(x + r) : 0.0;
-            const double ly = need_ly ? log(y + r) : 0.0;
-            const double lz = need_lz ? log(z + r) : 0.0;
-            const double axy = need_axy ? atan((x * y) / (z * r)) : 0.0;
-            const double ayz = need_ayz ? (axis ? atan(z / r) : atan((y * z) / (x * r))) : 0.0;
-            const double axz = need_axz ? (axis ? atan(z / r) : atan((x * z) / (y * r))) : 0.0;
+            const double lx = need_lx ? table_log(x + r, lg_G, lg_T) : 0.0;
+            const double ly = need_ly ? table_log(y + r, lg_G, lg_T) : 0.0;
+            const double lz = need_lz ? table_log(z + r, lg_G, lg_T) : 0.0;
+            double axy = 0.0, ayz = 0.0, axz = 0.0;
+            if (any_atan) {
+                const double ir = 1.0 / r;
+                const double iz = lds_levels ? izl[k] : 1.0 / z;
+                if (need_axy) axy = atan(xy * (iz * ir));
+                if (axis) {
+                    const double a = atan(z * ir);
+                    ayz = a;
+                    axz = a;
+                } else if (all_atan) {
+                    ayz = atan((y * z) * (ix * ir));
+                    axz = half_pi_s - axy - ayz;
+                } else {
+                    if (need_ayz) ayz = atan((y * z) * (ix * ir));
+                    if (need_axz) axz = atan((x * z) * (iy * ir));
+                }
+            }
             if (flag & (1u << 1)) acc[1] += wn * (x * ly + y * lx - z * axy);
             if (flag & (1u << 2)) acc[2] += wn * (y * lz + z * ly - x * ayz);
             if (flag & (1u << 3)) acc[3] += wn * (z * lx + x * lz - y * axz);

@@ -23,8 +23,9 @@ Stream (seeded numpy PCG64):
   used problems until the budget holds.
 
 Request payloads are built ahead by a producer thread (the client's work);
-the server thread takes requests in arrival order, waiting for each arrival
-time, and runs one request at a time on the GPU.
+a dispatcher releases each request at its arrival time and `workers`
+server threads (default 3, each with its own context and stream) take them
+in that order, so up to `workers` requests are on the GPU at once.
 """
 from __future__ import annotations
 
@@ -122,29 +123,68 @@ class ResidentCache:
         self.used = 0.0
         self.admissions = 0
         self.evictions = 0
+        self.lock = threading.Lock()
+        self.cv = threading.Condition(self.lock)
 
     @staticmethod
     def g_bytes(prob: CgProblem) -> float:
         nx, ny, nz = prob.shape
         return 8.0 * nx * ny * nz * prob.n_pts ** 2 * bin(prob.flag).count("1")
 
-    def get(self, pid: int):
-        if pid in self.entries:
-            self.entries.move_to_end(pid)
-            return self.entries[pid]
-        prob = self.pool[pid]
-        need = self.g_bytes(prob)
-        if need > self.budget:
-            raise MemoryError(f"problem {prob.name} ({need / 1e9:.1f} GB) exceeds the budget")
-        while self.entries and self.used + need > self.budget:
-            _, old = self.entries.popitem(last=False)
-            old["ctx"].close()
-            self.used -= old["bytes"]
-            self.evictions += 1
-        self.entries[pid] = self._admit(prob, need)
-        self.used += need
-        self.admissions += 1
-        return self.entries[pid]
+    def get(self, pid: int, info: dict | None = None):
+        """The resident entry of problem pid, admitted if needed; the caller
+        holds it (it is not evicted) until release().  Safe to call from
+        several threads: one admits, the others wait for it.  info["admitted"]
+        tells whether this call admitted the problem."""
+        if info is not None:
+            info["admitted"] = False
+        with self.lock:
+            while True:
+                e = self.entries.get(pid)
+                if e is None:
+                    break
+                if e.get("ready"):
+                    self.entries.move_to_end(pid)
+                    e["busy"] += 1
+                    return e
+                self.cv.wait()
+            prob = self.pool[pid]
+            need = self.g_bytes(prob)
+            if need > self.budget:
+                raise MemoryError(f"problem {prob.name} ({need / 1e9:.1f} GB) exceeds the budget")
+            # evict idle least-recently-used problems; in-use ones stay (the
+            # budget may be exceeded until they are released)
+            for old_pid in list(self.entries):
+                if self.used + need <= self.budget:
+                    break
+                old = self.entries[old_pid]
+                if old.get("ready") and old["busy"] == 0:
+                    del self.entries[old_pid]
+                    old["ctx"].close()
+                    self.used -= old["bytes"]
+                    self.evictions += 1
+            placeholder = dict(ready=False, busy=1, bytes=need)
+            if info is not None:
+                info["admitted"] = True
+            self.entries[pid] = placeholder
+            self.used += need
+            self.admissions += 1
+        try:
+            e = self._admit(prob, need)
+        except BaseException:
+            with self.lock:
+                del self.entries[pid]
+                self.used -= need
+                self.cv.notify_all()
+            raise
+        with self.lock:
+            placeholder.update(e, ready=True)
+            self.cv.notify_all()
+        return placeholder
+
+    def release(self, e):
+        with self.lock:
+            e["busy"] -= 1
 
     def _admit(self, prob: CgProblem, need: float):
         nx, ny, nz = prob.shape
@@ -169,7 +209,8 @@ class ResidentCache:
 
     def close(self):
         for e in self.entries.values():
-            e["ctx"].close()
+            if e.get("ready"):
+                e["ctx"].close()
         self.entries.clear()
         self.used = 0.0
 
@@ -184,8 +225,16 @@ def _percentiles(x):
 
 
 def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahead: int = 32,
-          progress=None, forward_method: str = "nodes"):
-    """Replays `requests` in arrival order; returns a summary dict (times in ms)."""
+          progress=None, forward_method: str = "nodes", records: bool = False,
+          workers: int = 3):
+    """Replays `requests` in arrival order; returns a summary dict (times in ms;
+    with `records`, also the per-request arrival / start / finish times).
+
+    `workers` server threads, each with its own engine context (stream),
+    take the released requests in arrival order: while one request's kernel
+    runs, the next one's inputs are uploaded, and a short request can run
+    beside a long one instead of waiting for it to finish."""
+    import queue as _queue
     payloads, cv = {}, threading.Condition()
     stop = [False]
 
@@ -203,52 +252,81 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
                 payloads[r.rid] = item
                 cv.notify_all()
 
-    fctx = Context(device)
+    workers = max(1, int(workers))
+    fctxs = [Context(device) for _ in range(workers)]
     cache = ResidentCache(pool, budget_bytes, device)
     th = threading.Thread(target=producer, daemon=True)
     th.start()
-    rec = []
-    try:
-        # warm the forward path once (first-call module loading is not a request)
-        def run_forward(q, flag):
-            if forward_method == "nodes":
-                return forward_rectilinear(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"], flag,
-                                           ctx=fctx)
-            return forward(q["prisms"], q["rho"], q["pts"], flag, ctx=fctx)
+    rec, rec_lock = [], threading.Lock()
+    work = _queue.Queue()
+    errors = []
+    t_start = [0.0]
 
-        run_forward(forward_payload(dict(cells=64, npts=16, flag=0x3FE, height=10.0, seed=1)),
-                    0x3FE)
-        t_start = time.perf_counter()
-        for k, r in enumerate(requests):
-            now = time.perf_counter() - t_start
+    def run_forward(fctx, q, flag):
+        if forward_method == "nodes":
+            return forward_rectilinear(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"], flag,
+                                       ctx=fctx)
+        return forward(q["prisms"], q["rho"], q["pts"], flag, ctx=fctx)
+
+    def worker(fctx):
+        while True:
+            r = work.get()
+            if r is None:
+                return
+            try:
+                if r.kind == "forward":
+                    with cv:
+                        while r.rid not in payloads:
+                            cv.wait()
+                        q = payloads.pop(r.rid)
+                        cv.notify_all()
+                    t0 = time.perf_counter() - t_start[0]
+                    out = run_forward(fctx, q, r.params["flag"]).astype(np.float32)
+                    size = dict(cells=q["prisms"].shape[0], points=q["pts"].shape[0],
+                                comps=out.shape[1])
+                    cold = False
+                else:
+                    t0 = time.perf_counter() - t_start[0]
+                    info = {}
+                    e = cache.get(r.params["problem"], info)
+                    try:
+                        cold = info["admitted"]
+                        zero = np.zeros(e["nm"])
+                        e["inv"].solve_cg(0.0, 20, r.params["lam"], zero, zero, e["Wd"],
+                                          e["dobs"], e["P"])
+                    finally:
+                        cache.release(e)
+                    size = dict(problem=pool[r.params["problem"]].name)
+                t1 = time.perf_counter() - t_start[0]
+                with rec_lock:
+                    rec.append(dict(kind=r.kind, arrival=r.t_arrival, start=t0, finish=t1,
+                                    cold=cold, **size))
+                    n = len(rec)
+                if progress and n % progress == 0:
+                    print(f"[serve] {n}/{len(requests)} requests, t = {t1:.1f} s", flush=True)
+            except BaseException as ex:          # reported after the replay
+                errors.append(ex)
+
+    threads = []
+    try:
+        # warm each forward context once (first-call module loading is not a request)
+        warm = forward_payload(dict(cells=64, npts=16, flag=0x3FE, height=10.0, seed=1))
+        for fctx in fctxs:
+            run_forward(fctx, warm, 0x3FE)
+        t_start[0] = time.perf_counter()
+        threads = [threading.Thread(target=worker, args=(f,), daemon=True) for f in fctxs]
+        for t in threads:
+            t.start()
+        for r in requests:
+            now = time.perf_counter() - t_start[0]
             if now < r.t_arrival:
                 time.sleep(r.t_arrival - now)
-            if r.kind == "forward":
-                with cv:
-                    while r.rid not in payloads:
-                        cv.wait()
-                    q = payloads.pop(r.rid)
-                    cv.notify_all()
-                t0 = time.perf_counter() - t_start
-                out = run_forward(q, r.params["flag"]).astype(np.float32)
-                size = dict(cells=q["prisms"].shape[0], points=q["pts"].shape[0],
-                            comps=out.shape[1])
-                cold = False
-            else:
-                t0 = time.perf_counter() - t_start
-                n_adm = cache.admissions
-                e = cache.get(r.params["problem"])
-                cold = cache.admissions != n_adm
-                zero = np.zeros(e["nm"])
-                e["inv"].solve_cg(0.0, 20, r.params["lam"], zero, zero, e["Wd"], e["dobs"],
-                                  e["P"])
-                size = dict(problem=pool[r.params["problem"]].name)
-            t1 = time.perf_counter() - t_start
-            rec.append(dict(kind=r.kind, arrival=r.t_arrival, start=t0, finish=t1, cold=cold,
-                            **size))
-            if progress and (k + 1) % progress == 0:
-                print(f"[serve] {k + 1}/{len(requests)} requests, t = {t1:.1f} s", flush=True)
+            work.put(r)
     finally:
+        for _ in threads:
+            work.put(None)
+        for t in threads:
+            t.join()
         with cv:
             stop[0] = True
             cv.notify_all()
@@ -256,7 +334,11 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         cache_stats = dict(admissions=cache.admissions, evictions=cache.evictions,
                            resident_gb=cache.used / 1e9)
         cache.close()
-        fctx.close()
+        for fctx in fctxs:
+            fctx.close()
+    if errors:
+        raise errors[0]
+    rec.sort(key=lambda x: x["arrival"])
     lat = [1e3 * (x["finish"] - x["arrival"]) for x in rec]
     svc = [1e3 * (x["finish"] - x["start"]) for x in rec]
     makespan = max(x["finish"] for x in rec) if rec else 0.0
@@ -271,7 +353,10 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         requests=len(rec), makespan_s=makespan,
         offered_rate=len(rec) / requests[-1].t_arrival if rec else 0.0,
         achieved_rate=len(rec) / makespan if makespan else 0.0,
-        utilization=sum(svc) / 1e3 / makespan if makespan else 0.0,
-        sustainable_rate=1e3 / np.mean(svc) if svc else 0.0,
+        workers=workers,
+        mean_in_service=sum(svc) / 1e3 / makespan if makespan else 0.0,
+        utilization=sum(svc) / 1e3 / makespan / workers if makespan else 0.0,
+        sustainable_rate=(1e3 / np.mean(svc) if svc else 0.0) if workers == 1 else None,
         latency_ms=_percentiles(lat), service_ms=_percentiles(svc), by_kind=by,
-        cg_warm_service_ms=_percentiles(warm_cg), cache=cache_stats)
+        cg_warm_service_ms=_percentiles(warm_cg), cache=cache_stats,
+        **(dict(records=rec) if records else {}))

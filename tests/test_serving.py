@@ -61,16 +61,50 @@ def test_resident_cache_lru_admission_and_eviction():
     sizes = [S.ResidentCache.g_bytes(p) for p in pool]
     budget = sizes[0] + sizes[1] + 1.0           # room for problems 0 and 1 only
     c = Cache(pool, budget)
-    c.get(0)
-    c.get(1)
-    c.get(0)                                     # hit; 1 is now least recent
+    info = {}
+    c.release(c.get(0, info))
+    assert info["admitted"]
+    c.release(c.get(1))
+    c.release(c.get(0, info))                    # hit; 1 is now least recent
+    assert not info["admitted"]
     assert c.admissions == 2 and c.evictions == 0
-    c.get(2)                                     # evicts 1 (LRU), maybe 0 too
+    c.release(c.get(2))                          # evicts 1 (LRU), maybe 0 too
     assert closed[0] == pool[1].name
     assert c.used <= budget
     assert 2 in c.entries
     with pytest.raises(MemoryError):
         S.ResidentCache(pool, 1.0).get(0)
+
+
+def test_resident_cache_threads_share_one_admission_and_keep_busy_entries():
+    """Two threads asking for the same absent problem admit it once; an
+    entry in use is not evicted (the budget is exceeded until release)."""
+    import threading
+    import time
+    pool = S.default_cg_pool()
+    closed, admitted = [], []
+
+    class Cache(S.ResidentCache):
+        def _admit(self, prob, need):
+            admitted.append(prob.name)
+            time.sleep(0.05)
+            return dict(ctx=_FakeCtx(closed, prob.name), bytes=need)
+
+    sizes = [S.ResidentCache.g_bytes(p) for p in pool]
+    c = Cache(pool, max(sizes) + 1.0)            # room for one problem at a time
+    got = []
+    ts = [threading.Thread(target=lambda: got.append(c.get(0))) for _ in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert admitted == [pool[0].name] and got[0] is got[1] and got[0]["busy"] == 2
+    e1 = c.get(1)                                # 0 is busy: kept, budget exceeded
+    assert closed == [] and 0 in c.entries and 1 in c.entries
+    for e in got + [e1]:
+        c.release(e)
+    c.release(c.get(2))                          # now the idle 0 and 1 go
+    assert set(closed) == {pool[0].name, pool[1].name} and c.used <= c.budget
 
 
 @pytest.mark.gpu
