@@ -506,19 +506,31 @@ static bool plan_variant(gra_dtype dt, int64_t Nd, int64_t ld, int cus, int targ
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, 0) != hipSuccess)
         return false;
-    // Stay below the reported co-residency limit: two six-wave workgroups
-    // share a CU only when their waves balance over the four SIMDs, so a grid
-    // that fills every slot may not be fully resident -- and every row block
-    // waits for all tiles of its group.  Such problems use the two-pass
-    // iteration instead.
-    const int limit = per_cu * cus - cus / 32;
-    int target = target_wgs > 0 ? target_wgs : 2 * cus;
-    if (target > limit) target = limit;
-    FusedPlan p;
-    switch (dt) {
-    case GRA_F64: p = make_plan<CfgF64>(Nd, ld, target, di); break;
-    case GRA_F32: p = make_plan<CfgF32>(Nd, ld, target, di); break;
-    default: p = make_plan<CfgBf16>(Nd, ld, target, di); break;
+    // Two six-wave workgroups share a CU only when their waves balance over
+    // the four SIMDs, so a grid that fills every reported slot may not be
+    // fully resident, and every row block waits for all tiles of its group.
+    // A group of at most half the reported slots fills the device (on the
+    // 5,000 x 32,000 G: FP64 2 groups of 125 tiles at one workgroup per CU,
+    // BF16 16 groups of 32; stopping cus/32 workgroups short of the limit
+    // halves the groups there and runs 1.7x slower).  A wider group stays
+    // cus/32 workgroups below the limit (a 512-tile group at the limit
+    // exceeded the bounded wait) and, if that does not fit, the two-pass
+    // iteration is used.
+    const int full = per_cu * cus;
+    auto plan_for = [&](int lim) {
+        int target = target_wgs > 0 ? target_wgs : 2 * cus;
+        if (target > lim) target = lim;
+        switch (dt) {
+        case GRA_F64: return make_plan<CfgF64>(Nd, ld, target, di);
+        case GRA_F32: return make_plan<CfgF32>(Nd, ld, target, di);
+        default: return make_plan<CfgBf16>(Nd, ld, target, di);
+        }
+    };
+    FusedPlan p = plan_for(full);
+    int limit = full;
+    if (2 * p.NT > full) {
+        limit = full - cus / 32;
+        p = plan_for(limit);
     }
     if ((int64_t)p.NT * p.S > (int64_t)limit) return false;
     p.L = kDepthL[di];
