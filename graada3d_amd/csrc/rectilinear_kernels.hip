@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
+#include "fastlog.hpp"
 
 namespace gra {
 
@@ -35,40 +36,9 @@ namespace {
 constexpr int kBlk = 256;
 constexpr double kG0 = 6.673e-11;
 
-// Natural logarithm of a positive normal double by table reduction (Tang's
-// method with left table points): v = 2^e m, m in [1, 2); F_j = 1 + j/128
-// for the top 7 mantissa bits j; with G_j = 1/F_j (rounded) and
-// T_j = -ln G_j,
-//     ln v = e ln2 + T_j + ln(1 + r),  r = m G_j - 1 in (-2^-52, 2^-7),
-// ln(1 + r) by its Taylor series to r^8 (truncation < r^9/9 < 2^-66).
-// ln2 is split (fdlibm's ln2_hi / ln2_lo) so that e ln2_hi is exact.  About
-// a quarter of the instructions of the general library log; within ~1 ulp
-// (absolute error <= 1.2e-16 for v in [0.5, 2]).  Arguments are clamped to
-// [2^-1000, 2^1000]: here v = c + r with r > |c| and the points strictly
-// above the mesh, so v >= z^2 / (2 r) stays far inside that range.
-constexpr int kLogBits = 7;
-constexpr int kLogTable = 1 << kLogBits;
-
-__device__ inline double table_log(double v, const double *G, const double *T)
-{
-    v = fmin(fmax(v, 9.33263618503218879e-302), 1.07150860718626732e301);
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    const int e = (int)(b >> 52) - 1023;
-    const int j = (int)((b >> (52 - kLogBits)) & (kLogTable - 1));
-    const double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) |
-                                                      0x3FF0000000000000ull));
-    const double r = fma(m, G[j], -1.0);
-    double p = 1.0 / 8.0;
-    p = fma(p, -r, 1.0 / 7.0);
-    p = fma(p, r, -1.0 / 6.0);
-    p = fma(p, r, 1.0 / 5.0);
-    p = fma(p, r, -0.25);
-    p = fma(p, r, 1.0 / 3.0);
-    p = fma(p, r, -0.5);
-    p = fma(p * r, r, r);
-    const double ed = (double)e;
-    return fma(ed, 6.93147180369123816490e-01, T[j]) + fma(ed, 1.90821492927058770002e-10, p);
-}
+// The log of the corner terms: table_log (fastlog.hpp) with the table
+// copied to LDS per block.
+constexpr int kLogTable = kLogTableN;
 
 // Node weights, stored level by level: w[k * (NX*NY) + i * NY + j], so the
 // threads of a block (consecutive node columns) read consecutive addresses.
@@ -117,9 +87,8 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
     const int64_t ip = blockIdx.x;
     if (ip >= N_obs) return;
     for (int t = threadIdx.x; t < kLogTable; t += kBlk) {
-        const double g = 1.0 / (1.0 + t * (1.0 / kLogTable));
-        lg_G[t] = g;
-        lg_T[t] = -log(g);
+        lg_G[t] = kLogG[t];
+        lg_T[t] = kLogT[t];
     }
     const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
     const int NY = (int)ny + 1, NZ = (int)nz + 1, NCOL = ((int)nx + 1) * NY;

@@ -7,6 +7,9 @@
 #include <cstdint>
 
 #include "prism_math.hpp"
+#include "fastlog.hpp"
+
+extern "C" double hc_table_log(double v) { return gra::fast_log(v); }
 
 extern "C" int hc_prism_field(const double *pt, const double *pr, double rho, uint32_t flag,
                               double *out10)
