@@ -63,7 +63,7 @@ METRIC = "G entries/s, CG iter/s (mixed vs FP64 plain) at stated error; p50/p99 
 HBM_PEAK_GBS = 8000.0
 # FP64 flop per G entry of the assembly kernels (gz+Tzz / all nine components),
 # from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters
-FLOP_PER_ENTRY = {True: 1321.0, False: 845.0}
+FLOP_PER_ENTRY = {True: 1165.0, False: 740.0}
 BYTES = {"f64": 8, "f32": 4, "bf16": 2}
 
 

@@ -71,13 +71,15 @@ def build(verbose: bool = False) -> str:
 def build_hostcheck() -> str:
     """Test-support libraries: the prism math compiled for the host (g++), in
     the reference's operation order (libhostcheck.so) and with the GPU's
-    pairwise atan differences (libhostcheck_pairs.so)."""
+    formulation -- pairwise atan differences, table log, polynomial atan --
+    (libhostcheck_pairs.so)."""
     src = os.path.join(ROOT, "tests", "hostcheck", "prism_hostcheck.cpp")
     out = os.path.join(ROOT, "tests", "hostcheck", "libhostcheck.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
           "-I", CSRC, "-o", out, src, "-lm"])
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-          "-DGRA_ATAN_PAIRS=1", "-I", CSRC, "-o", out.replace(".so", "_pairs.so"), src, "-lm"])
+          "-DGRA_ATAN_PAIRS=1", "-DGRA_TABLE_LOG=1", "-DGRA_FAST_ATAN=1", "-I", CSRC,
+          "-o", out.replace(".so", "_pairs.so"), src, "-lm"])
     return out
 
 

@@ -55,6 +55,19 @@
 #if GRA_TABLE_LOG
 #include "fastlog.hpp"
 #endif
+// GRA_FAST_ATAN = 1 takes the atans of atan_diff from atan_ratio
+// (fastatan.hpp: one division, a degree-20 polynomial, within ~1 ulp); the
+// GPU build uses it, the host build keeps the library atan unless asked.
+#ifndef GRA_FAST_ATAN
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GRA_FAST_ATAN 1
+#else
+#define GRA_FAST_ATAN 0
+#endif
+#endif
+#if GRA_FAST_ATAN
+#include "fastatan.hpp"
+#endif
 
 namespace gra {
 
@@ -194,7 +207,11 @@ template <int F> GRA_HD inline void face_edges(const Geo &g, const double *pr, E
 // principal range).  den = 0 gives atan(+-inf) = +-pi/2 as the exact value.
 GRA_HD inline double atan_diff(double num, double den)
 {
+#if GRA_FAST_ATAN
+    double a = atan_ratio(num, den);
+#else
     double a = atan(num / den);
+#endif
     if (den < 0.0) a += (num < 0.0) ? -3.14159265358979323846 : 3.14159265358979323846;
     return a;
 }

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
+#include "fastatan.hpp"
 #include "fastlog.hpp"
 
 namespace gra {
@@ -68,9 +69,10 @@ __global__ void node_weights_kernel(const double *__restrict__ rho, int64_t nx, 
 }
 
 // One block per observation point; a thread takes node columns (i, j), with
-// x and y fixed, and walks their nz+1 nodes.  The atan arguments use one
-// reciprocal per node (1/r) with 1/x, 1/y per column and 1/z per level (LDS);
-// when all three atans are needed the third follows from the corner identity
+// x and y fixed, and walks their nz+1 nodes.  The atans are atan_ratio
+// (fastatan.hpp: one division each, no library call) and the logs table_log
+// (fastlog.hpp, table in LDS); when all three atans are needed the third
+// follows from the corner identity
 //   atan(xy/(zr)) + atan(yz/(xr)) + atan(xz/(yr)) = (pi/2) sgn(x) sgn(y)
 // (z > 0; sgn(+0) = +1, which is also the limit the x = 0 / y = 0 / axis
 // cases take).
@@ -82,7 +84,7 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
                     const double *__restrict__ w, const double *__restrict__ pts, int64_t N_obs,
                     uint32_t flag, double *__restrict__ out)
 {
-    __shared__ double zl[kMaxLevels], izl[kMaxLevels];
+    __shared__ double zl[kMaxLevels];
     __shared__ double lg_G[kLogTable], lg_T[kLogTable];
     const int64_t ip = blockIdx.x;
     if (ip >= N_obs) return;
@@ -94,11 +96,7 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
     const int NY = (int)ny + 1, NZ = (int)nz + 1, NCOL = ((int)nx + 1) * NY;
     const bool lds_levels = NZ <= kMaxLevels;
     if (lds_levels) {
-        for (int k = threadIdx.x; k < NZ; k += kBlk) {
-            const double z = zs[k] - pz;
-            zl[k] = z;
-            izl[k] = 1.0 / z;
-        }
+        for (int k = threadIdx.x; k < NZ; k += kBlk) zl[k] = zs[k] - pz;
     }
     __syncthreads();
     // which transcendental terms the requested components need (uniform)
@@ -118,7 +116,7 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
         const double x = xs[i] - px, y = ys[j] - py;
         const double xy2 = fma(x, x, y * y);
         const bool axis = (x == 0.0) && (y == 0.0);
-        const double xy = x * y, ix = 1.0 / x, iy = 1.0 / y;
+        const double xy = x * y;
         // (pi/2) sgn(x) sgn(y) with the sign of a zero: the limits 1/(+-0)
         // gives the atans it replaces
         const double half_pi_s = copysign(1.5707963267948966, x) * copysign(1.0, y);
@@ -131,19 +129,17 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
             const double lz = need_lz ? table_log(z + r, lg_G, lg_T) : 0.0;
             double axy = 0.0, ayz = 0.0, axz = 0.0;
             if (any_atan) {
-                const double ir = 1.0 / r;
-                const double iz = lds_levels ? izl[k] : 1.0 / z;
-                if (need_axy) axy = atan(xy * (iz * ir));
+                if (need_axy) axy = atan_ratio(xy, z * r);
                 if (axis) {
-                    const double a = atan(z * ir);
+                    const double a = atan_ratio(z, r);
                     ayz = a;
                     axz = a;
                 } else if (all_atan) {
-                    ayz = atan((y * z) * (ix * ir));
+                    ayz = atan_ratio(y * z, x * r);
                     axz = half_pi_s - axy - ayz;
                 } else {
-                    if (need_ayz) ayz = atan((y * z) * (ix * ir));
-                    if (need_axz) axz = atan((x * z) * (iy * ir));
+                    if (need_ayz) ayz = atan_ratio(y * z, x * r);
+                    if (need_axz) axz = atan_ratio(x * z, y * r);
                 }
             }
             if (flag & (1u << 1)) acc[1] += wn * (x * ly + y * lx - z * axy);

@@ -8,6 +8,9 @@
 
 #include "prism_math.hpp"
 #include "fastlog.hpp"
+#include "fastatan.hpp"
+
+extern "C" double hc_atan_ratio(double n, double d) { return gra::atan_ratio(n, d); }
 
 extern "C" double hc_table_log(double v) { return gra::fast_log(v); }
 
