@@ -117,10 +117,11 @@ def _gauss_field(pt, pr, n=40):
 
 
 def test_pairwise_atan_form_vs_oracle(hcp):
-    """The GPU's formulation (each atan(x) - atan(y) as one atan) against the
-    reference's order: equal to the last bits on ordinary geometry (median
-    1e-15 of the evaluation's largest component), at most ~3e-10 on far
-    prisms, where both forms lose digits to cancellation (next test)."""
+    """The GPU's formulation (each atan(x) - atan(y) as one atan, the table
+    log, the polynomial atan) built for the host against the reference's
+    order: equal to the last bits on ordinary geometry (median ~1e-15 of the
+    evaluation's largest component), at most ~3e-10 on far prisms, where
+    both forms lose digits to cancellation (next test)."""
     rng = np.random.default_rng(1)
     dev = []
     for trial in range(3000):
