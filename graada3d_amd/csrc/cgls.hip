@@ -3,15 +3,19 @@
 // helpers around it (:856-861 barrier P, :877-911 model update, :913-916
 // misfit, :962-1022 IRLS weights).
 //
-// One CG iteration = 8 launches on the context stream, none of which waits on
+// One CG iteration = 6 launches on the context stream, none of which waits on
 // the host:
-//   p_update  -> gemv_n(G, Pp) -> t_rows (+ t^2 partials) -> alpha (1 block)
-//   -> s_update (s += a t, dx += a p, w = Wd s_d) -> gemv_t(G, w)
-//   -> r_update (r = A^T s, + r^2 partials) -> scalar_end (1 block)
-// The scalars (r2, t2, alpha, beta, err, iterations) live in device memory;
-// when err < tol the scalar kernel sets `done` and every later kernel of the
-// queued iterations returns at once.  All reductions have a fixed shape, so a
-// solve is bit-for-bit repeatable.
+//   p_update (ends the previous iteration: beta, err, stop test; then
+//   p = -r + beta p) -> gemv_n(G, Pp) -> t_rows (+ t^2 partials)
+//   -> s_update (alpha; s += a t, dx += a p, w = Wd s_d) -> gemv_t(G, w)
+//   -> r_loop (r = A^T s, + r^2 partials)
+// and one scalar launch ends the last iteration.  The scalars a kernel needs
+// from the previous kernel's block partials (alpha, beta) are summed by every
+// block of the consuming kernel in the same fixed order, so each block holds
+// the identical value and block 0 records it.  The scalars (r2, t2, alpha,
+// beta, err, iterations) live in device memory; when err < tol `done` is set
+// and every later kernel of the queued iterations returns at once.  All
+// reductions have a fixed shape, so a solve is bit-for-bit repeatable.
 //
 // Operator association follows the reference's Eigen expressions: a term such
 // as sqrt(l) * P * Ws^T * WL_s^T * v is summed as ((sqrt(l)*P_i)*Ws_ji)*WL_j*v_j
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(kB) k_scalar_init(CgDev d, int64_t nblk)
     double r2 = sum_partials(d.red, nblk);
     double atb2 = sum_partials(d.red2, nblk);
     if (threadIdx.x == 0) {
-        d.sc->r2 = r2;
+        d.sc->r2[0] = r2;
         d.sc->ATb2 = atb2;
         d.sc->beta = 0.0;
         d.sc->done = 0;
@@ -271,13 +275,62 @@ __global__ void __launch_bounds__(kB) k_scalar_init(CgDev d, int64_t nblk)
     }
 }
 
-// p = -r + beta p; Pp = P p   (GN.cpp:1558-1560)
-__global__ void __launch_bounds__(kB) k_p_update(CgDev d)
+// End of iteration it (GN.cpp:1674-1691) from the r^2 partials of its
+// residual: beta = r2/r2_last, err = r2/ATb2 and the stop test.  Every
+// thread gets beta and the stop flag; block 0 records the scalars (no other
+// block reads the fields it writes: r2 goes to the slot of iteration it+1).
+__device__ inline void cg_end_iteration(const CgDev &d, int64_t nblk, int it, double tol,
+                                        double &beta, bool &stop)
 {
-    if (d.sc->done) return;
+    __shared__ double beta_sh;
+    __shared__ int stop_sh;
+    double r2 = sum_partials(d.red, nblk);
+    if (threadIdx.x == 0) {
+        CgScalars *sc = d.sc;
+        double b = r2 / sc->r2[it & 1];
+        double err = r2 / sc->ATb2;
+        int st = err < tol;
+        if (blockIdx.x == 0) {
+            sc->r2[(it + 1) & 1] = r2;
+            sc->beta = b;
+            sc->err = err;
+            sc->iterations = it + 1;
+            if (st) sc->done = 1;
+        }
+        beta_sh = b;
+        stop_sh = st;
+    }
+    __syncthreads();
+    beta = beta_sh;
+    stop = stop_sh != 0;
+}
+
+// The stop flag as one value per block: block 0 of the p update may set it
+// while other blocks of the same launch read it.
+__device__ inline bool cg_done_block(const CgDev &d)
+{
+    __shared__ int done_sh;
+    if (threadIdx.x == 0) done_sh = d.sc->done;
+    __syncthreads();
+    return done_sh != 0;
+}
+
+// Iteration it: end iteration it-1 (FIRST: it = 0, beta = 0 from the initial
+// scalars), then p = -r + beta p; Pp = P p   (GN.cpp:1558-1560)
+template <bool FIRST>
+__global__ void __launch_bounds__(kB) k_p_update(CgDev d, int64_t nblk, int it, double tol)
+{
+    if (cg_done_block(d)) return;
+    double beta;
+    if (FIRST) {
+        beta = d.sc->beta;
+    } else {
+        bool stop;
+        cg_end_iteration(d, nblk, it - 1, tol, beta, stop);
+        if (stop) return;
+    }
     int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (j >= d.nm) return;
-    double beta = d.sc->beta;
     double p = -d.r[j] + beta * d.p[j];
     d.p[j] = p;
     d.Pp[j] = d.P[j] * p;
@@ -311,25 +364,41 @@ __global__ void __launch_bounds__(kB) k_t_rows(CgDev d)
     if (threadIdx.x == 0) d.red[blockIdx.x] = s;
 }
 
-// alpha = r2 / t2   (GN.cpp:1588-1603)
-__global__ void __launch_bounds__(kB) k_alpha(CgDev d, int64_t nblk)
+// alpha = r2 / t2   (GN.cpp:1588-1603) in every thread; block 0 records it.
+// t2 sums the t_rows partials, or for the single-read iteration
+// ||t_d||^2 + ||t_reg||^2, each sum in a fixed order.
+template <bool FUSED>
+__device__ inline double cg_alpha(const CgDev &d, int64_t nblk, int it)
 {
-    if (d.sc->done) return;
-    double t2 = sum_partials(d.red, nblk);
-    if (threadIdx.x == 0) {
-        double a = d.sc->r2 / t2;
-        d.sc->t2 = t2;
-        d.sc->alpha = a;
-        if (!(t2 != 0.0) || !isfinite(a)) d.sc->breakdown = 1;
+    __shared__ double a_sh;
+    double t2;
+    if (FUSED) {
+        double td2 = sum_partials(d.t2part, d.nt2);
+        double tr2 = sum_partials(d.red, nblk);
+        t2 = td2 + tr2;
+    } else {
+        t2 = sum_partials(d.red, nblk);
     }
+    if (threadIdx.x == 0) {
+        double a = d.sc->r2[it & 1] / t2;
+        if (blockIdx.x == 0) {
+            d.sc->t2 = t2;
+            d.sc->alpha = a;
+            if (!(t2 != 0.0) || !isfinite(a)) d.sc->breakdown = 1;
+        }
+        a_sh = a;
+    }
+    __syncthreads();
+    return a_sh;
 }
 
-// s += alpha t; dx += alpha p; w = Wd s_d   (GN.cpp:1604-1629, 1634)
-__global__ void __launch_bounds__(kB) k_s_update(CgDev d)
+// alpha; s += alpha t; dx += alpha p; w = Wd s_d   (GN.cpp:1588-1629, 1634)
+template <bool FUSED>
+__global__ void __launch_bounds__(kB) k_s_update(CgDev d, int64_t nblk, int it)
 {
     if (d.sc->done) return;
+    double a = cg_alpha<FUSED>(d, nblk, it);
     int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double a = d.sc->alpha;
     if (g < d.nrow) {
         double s = d.s[g] + a * d.t[g];
         d.s[g] = s;
@@ -338,21 +407,13 @@ __global__ void __launch_bounds__(kB) k_s_update(CgDev d)
     if (g < d.nm) d.dx[g] = d.dx[g] + a * d.p[g];
 }
 
-// beta = r2/r2_last; err = r2/ATb2; stop test   (GN.cpp:1674-1691)
+// End of the last iteration (one block).
 __global__ void __launch_bounds__(kB) k_scalar_end(CgDev d, int64_t nblk, int iter, double tol)
 {
     if (d.sc->done) return;
-    double r2 = sum_partials(d.red, nblk);
-    if (threadIdx.x == 0) {
-        CgScalars *sc = d.sc;
-        double last = sc->r2;
-        sc->r2_prev = last;
-        sc->r2 = r2;
-        sc->beta = r2 / last;
-        sc->err = r2 / sc->ATb2;
-        sc->iterations = iter + 1;
-        if (sc->err < tol) sc->done = 1;
-    }
+    double beta;
+    bool stop;
+    cg_end_iteration(d, nblk, iter, tol, beta, stop);
 }
 
 // ---- single-read fused iteration -------------------------------------------
@@ -375,21 +436,6 @@ __global__ void __launch_bounds__(kB) k_t_rows_reg(CgDev d)
     }
     double s = block_sum(t2);
     if (threadIdx.x == 0) d.red[blockIdx.x] = s;
-}
-
-// alpha = r2 / (||t_d||^2 + ||t_reg||^2), both sums in a fixed order.
-__global__ void __launch_bounds__(kB) k_alpha_fused(CgDev d, int64_t nblk)
-{
-    if (d.sc->done) return;
-    double td2 = sum_partials(d.t2part, d.nt2);
-    double tr2 = sum_partials(d.red, nblk);
-    if (threadIdx.x == 0) {
-        double t2 = td2 + tr2;
-        double a = d.sc->r2 / t2;
-        d.sc->t2 = t2;
-        d.sc->alpha = a;
-        if (!(t2 != 0.0) || !isfinite(a)) d.sc->breakdown = 1;
-    }
 }
 
 // T*_scaled = (a*S_i) * T0_ij, and for the transpose (a*S_j) * T0_ji.
@@ -502,6 +548,19 @@ __global__ void k_irls(const int64_t *rp, const int32_t *ci, const double *v, in
 }
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
+
+// Iteration it's p update; from it = 1 on it also ends iteration it-1 from
+// the nblk_r partials of its residual.
+inline void launch_p_update(const CgDev &d, int64_t nm, int64_t nblk_r, int it, double tol,
+                            hipStream_t st)
+{
+    if (it == 0)
+        hipLaunchKernelGGL(k_p_update<true>, dim3(nblocks(nm)), dim3(kB), 0, st, d, nblk_r, it,
+                           tol);
+    else
+        hipLaunchKernelGGL(k_p_update<false>, dim3(nblocks(nm)), dim3(kB), 0, st, d, nblk_r, it,
+                           tol);
+}
 
 }  // namespace
 
@@ -651,22 +710,21 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     if (!fused) {
         for (int it = 0; it < a.maxit; ++it) {
             TimedRegion tr_it(c, kTimeCgIter);
-            hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+            launch_p_update(d, nm, nblk_r, it, a.tol, st);
             {
                 TimedRegion tr(c, kTimeGemvN);
                 e = launch_G_n(c, shape, d.Pp, c.part_n.as<double>(), done, st);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
             }
             hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
-            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, d, nblk_row);
-            hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_s_update<false>, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0,
+                               st, d, nblk_row, it);
             {
                 TimedRegion tr(c, kTimeGemvT);
                 e = launch_G_t(c, shape, d.w, c.part_t.as<double>(), done, st);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
             }
             hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st, d);
-            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_r, it, a.tol);
             if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
         }
     } else {
@@ -674,7 +732,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         const int64_t nblk_reg = nblocks(nreg);
         for (int it = 0; it < a.maxit; ++it) {
             TimedRegion tr_it(c, kTimeCgIter);
-            hipLaunchKernelGGL(k_p_update, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+            launch_p_update(d, nm, nblk_r, it, a.tol, st);
             {
                 TimedRegion tr(c, kTimeFused);
                 double *zp = nullptr, *t2p = nullptr;
@@ -685,17 +743,20 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
                 d.t2part = t2p;
             }
             hipLaunchKernelGGL(k_t_rows_reg, dim3((unsigned)nblk_reg), dim3(kB), 0, st, d);
-            hipLaunchKernelGGL(k_alpha_fused, dim3(1), dim3(kB), 0, st, d, nblk_reg);
-            hipLaunchKernelGGL(k_s_update, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_s_update<true>, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0,
+                               st, d, nblk_reg, it);
             if (it == 0)
                 hipLaunchKernelGGL((k_r_loop<true, true>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
                                    d);
             else
                 hipLaunchKernelGGL((k_r_loop<true, false>), dim3((unsigned)nblk_r), dim3(kB), 0,
                                    st, d);
-            hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_r, it, a.tol);
             if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
         }
+    }
+    if (a.maxit > 0) {
+        hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_r, a.maxit - 1, a.tol);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg end");
     }
 
     CgScalars hs;

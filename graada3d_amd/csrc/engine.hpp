@@ -46,8 +46,7 @@ struct DevCsr {
 // `done` first and returns when it is set, so a captured or pre-queued
 // sequence of iterations drains as no-ops after convergence.
 struct CgScalars {
-    double r2;        // ||r||^2 of the current residual
-    double r2_prev;
+    double r2[2];     // ||r||^2 of the residual entering iteration `it`: r2[it & 1]
     double t2;
     double alpha;
     double beta;
