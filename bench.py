@@ -22,7 +22,10 @@ and with the single-read iteration (one fused pass over G per CG iteration)
 for every storage type, and reported as speed-up and error against the plain
 FP64 run (final model and misfit).  G assembly throughput (entries/s) is
 measured for every storage type.  Kernel rooflines use HIP events on the
-engine's stream; the CPU baseline is the single-threaded C oracle on a
+engine's stream around the G products of every TIMING_STRIDE-th CG
+iteration of the timed steps (an event is a queue packet of its own and
+slows the launches next to it: timing every iteration cost 3-5 % of the
+iteration rate); the CPU baseline is the single-threaded C oracle on a
 bounded sample.
 
 * batch: 64 uneven instances (2:1 octree meshes of 2k-60k cells refined at
@@ -65,6 +68,7 @@ HBM_PEAK_GBS = 8000.0
 # from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters
 FLOP_PER_ENTRY = {True: 1165.0, False: 740.0}
 BYTES = {"f64": 8, "f32": 4, "bf16": 2}
+TIMING_STRIDE = 10      # CG iterations 0, 10, 20, ... of each solve carry events
 
 
 def build_problem(name):
@@ -142,7 +146,7 @@ def run(args):
         for k in range(args.warmup):
             inv.gn_step(pb["lambdas"][k], 0.0, pb["cg_maxit"])
         setup()
-        ctx.timing(True)
+        ctx.timing(TIMING_STRIDE)
         ctx.timing_reset()
         ctx.sync()
         t0 = time.perf_counter()
@@ -642,7 +646,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "traffic_unit": "GB per launch"},
-        "kernels": {"gemv_n": ref["gemv_n"], "gemv_t": ref["gemv_t"]},
+        "kernels": {"gemv_n": ref["gemv_n"], "gemv_t": ref["gemv_t"],
+                    "events": f"HIP events around the G products of CG iterations 0, "
+                              f"{TIMING_STRIDE}, {2 * TIMING_STRIDE}, ... of every timed solve "
+                              f"and the set-up products"},
         "g_assembly": {"entries_per_s": ref["entries_per_s"], "avg_ms": ref["asm_ms"],
                        "entries": nd * nm, "store": "f64",
                        "kernel": "face/edge FP64 evaluation, " + ("gz+Tzz" if small else "9 comps"),

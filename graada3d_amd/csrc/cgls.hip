@@ -709,10 +709,10 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     const int *done = &d.sc->done;
     if (!fused) {
         for (int it = 0; it < a.maxit; ++it) {
-            TimedRegion tr_it(c, kTimeCgIter);
+            TimedRegion tr_it(c, kTimeCgIter, it);
             launch_p_update(d, nm, nblk_r, it, a.tol, st);
             {
-                TimedRegion tr(c, kTimeGemvN);
+                TimedRegion tr(c, kTimeGemvN, it);
                 e = launch_G_n(c, shape, d.Pp, c.part_n.as<double>(), done, st);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
             }
@@ -720,7 +720,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
             hipLaunchKernelGGL(k_s_update<false>, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0,
                                st, d, nblk_row, it);
             {
-                TimedRegion tr(c, kTimeGemvT);
+                TimedRegion tr(c, kTimeGemvT, it);
                 e = launch_G_t(c, shape, d.w, c.part_t.as<double>(), done, st);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
             }
@@ -731,10 +731,10 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         const int64_t nreg = nrow - nd;
         const int64_t nblk_reg = nblocks(nreg);
         for (int it = 0; it < a.maxit; ++it) {
-            TimedRegion tr_it(c, kTimeCgIter);
+            TimedRegion tr_it(c, kTimeCgIter, it);
             launch_p_update(d, nm, nblk_r, it, a.tol, st);
             {
-                TimedRegion tr(c, kTimeFused);
+                TimedRegion tr(c, kTimeFused, it);
                 double *zp = nullptr, *t2p = nullptr;
                 e = launch_fused(plan, c.G.p, nd, ld, d.Pp, dg.Wd, c.fused_coef.as<double>(), d.t,
                                  c.fused_scratch.p, fused_err, done, &zp, &t2p, st);

@@ -33,12 +33,15 @@ gra_status hip_fail(Ctx &c, hipError_t e, const char *what)
     return GRA_HIP_ERROR;
 }
 
-TimedRegion::TimedRegion(Ctx &ctx, int klass) : c(ctx), cls(klass)
+TimedRegion::TimedRegion(Ctx &ctx, int klass, int iter) : c(ctx), cls(klass)
 {
-    if (!c.timing) return;
+    if (c.timing <= 0 || iter % c.timing != 0) return;
     while (c.ev_used + 2 > c.ev_pool.size()) {
         hipEvent_t ev;
-        if (hipEventCreate(&ev) != hipSuccess) return;
+        // timing only: read after a stream synchronisation, so the record
+        // needs no system-scope cache writeback (which would slow the kernels
+        // around it)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableSystemFence) != hipSuccess) return;
         c.ev_pool.push_back(ev);
     }
     a = c.ev_pool[c.ev_used++];
@@ -1078,7 +1081,7 @@ static hipError_t time_pair(Ctx &c, const GemvShape &s, int reps, const double *
     std::vector<hipEvent_t> ev(3 * (size_t)reps, nullptr);
     hipError_t e = hipSuccess;
     for (auto &v : ev)
-        if ((e = hipEventCreate(&v)) != hipSuccess) break;
+        if ((e = hipEventCreateWithFlags(&v, hipEventDisableSystemFence)) != hipSuccess) break;
     if (e == hipSuccess) {
         // one untimed pair first
         e = launch_gemv_n(c.G.p, c.store, c.Nd, c.ld, s, x, c.part_n.as<double>(), nullptr, c.stream);
@@ -1200,7 +1203,8 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
 gra_status gra_timing_enable(gra_ctx *ctx, int on)
 {
     CTX_GUARD(ctx);
-    c.timing = on != 0;
+    if (on < 0) return set_error(c, GRA_BAD_ARG, "timing: negative stride");
+    c.timing = on;
     return GRA_OK;
 }
 

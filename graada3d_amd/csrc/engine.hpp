@@ -149,7 +149,7 @@ struct Ctx {
     int fused_depth = 0;          // 0: default (lag, prefetch) variant
 
     // timing
-    bool timing = false;
+    int timing = 0;            // 0 off; n: every launch, CG-loop launches every n-th iteration
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_marks;
@@ -160,7 +160,9 @@ struct TimedRegion {
     Ctx &c;
     int cls;
     hipEvent_t a = nullptr, b = nullptr;
-    TimedRegion(Ctx &ctx, int klass);
+    // iter: the CG iteration of a launch inside the CG loop (timed when
+    // iter % timing == 0), 0 elsewhere
+    TimedRegion(Ctx &ctx, int klass, int iter = 0);
     ~TimedRegion();
 };
 

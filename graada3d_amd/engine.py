@@ -90,8 +90,10 @@ class Context:
         self.check(self.L.gra_sync(self.h), "gra_sync")
 
     # timing (GPU events on the context stream)
-    def timing(self, on: bool):
-        self.check(self.L.gra_timing_enable(self.h, 1 if on else 0), "timing")
+    def timing(self, on):
+        """False/True, or a stride n: CG-loop launches timed on every n-th
+        iteration (others on every launch)."""
+        self.check(self.L.gra_timing_enable(self.h, int(on)), "timing")
 
     def timing_reset(self):
         self.check(self.L.gra_timing_reset(self.h), "timing_reset")

@@ -332,7 +332,11 @@ gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t);
  * launches of kernel class `which` since the last gra_timing_reset:
  * 0 = GEMV G*x, 1 = GEMV G^T*x, 2 = G assembly, 3 = one full CG iteration,
  * 4 = fused single-read product (G Pp and G^T Wd^2 G Pp).
- * Timing events are recorded only while enabled.
+ * Timing events are recorded only while enabled: `on` = 0 disables; n >= 1
+ * times every launch of these classes, except inside the CG loop, where only
+ * iterations 0, n, 2n, ... are timed (each event is a queue packet of its
+ * own and slows the launches around it, so a stride keeps most iterations
+ * free of them).
  */
 gra_status gra_timing_enable(gra_ctx *ctx, int on);
 gra_status gra_timing_reset(gra_ctx *ctx);
