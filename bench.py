@@ -131,7 +131,7 @@ def run(args):
         ctx.timing(False)
         asm_ms = asm[store]
         inv.set_regularization(*pb["W"])
-        ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1)
+        ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1, n_block=-1)
         tune_s = 0.0
         if args.autotune and not fused:
             t0 = time.perf_counter()

@@ -150,6 +150,7 @@ GemvShape gemv_shape(const Ctx &c)
     if (o.t_chunks) s.t_chunks = o.t_chunks;
     if (o.t_unroll) s.t_unroll = o.t_unroll;
     if (o.nontemporal >= 0) s.nontemporal = o.nontemporal;
+    if (o.n_block >= 0) s.n_block = o.n_block;
     if (c.wav) s.n_splits = s.t_chunks = 1;   // the compressed products write whole sums
     return s;
 }
@@ -1066,6 +1067,10 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
             return set_error(c, GRA_BAD_ARG, "fused lag variant: 0 (default) or 1..7");
         c.fused_depth = (int)value;
         break;
+    case GRA_TUNE_N_BLOCK:
+        if (value < -1 || value > 1) return set_error(c, GRA_BAD_ARG, "block-shared rows: -1..1");
+        o.n_block = (int)value;
+        break;
 
     default:
         return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
@@ -1149,22 +1154,26 @@ gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t)
             if (tt < bt) { best = s; bt = tt; bn = tn; }
         }
     }
-    // G x: rows per wave x column steps x column splits
+    // G x: wave-owned / block-shared rows x rows x column steps x column splits
     const int V = c.store == GRA_F64 ? 2 : (c.store == GRA_F32 ? 4 : 8);
-    const int64_t max_splits = c.Nm / (4 * V * 64) > 0 ? c.Nm / (4 * V * 64) : 1;
-    for (int rows : {2, 4, 8}) {
-        if (c.store == GRA_F64 && rows == 8) continue;
-        for (int un : {1, 2})
-            for (int sp : {1, 2, 4, 8, 16}) {
-                if (sp > max_splits) continue;
-                GemvShape s = best;
-                s.n_rows = rows;
-                s.n_unroll = un;
-                s.n_splits = sp;
-                double tn, tt;
-                HIPCHK(time_pair(c, s, reps, xp, wp, &tn, &tt), "time G x");
-                if (tn < bn) { best = s; bn = tn; bt = tt; }
-            }
+    for (int blk : {0, 1}) {
+        const int64_t span = blk ? 2 * V * 256 : 4 * V * 64;   // columns a split keeps at least
+        const int64_t max_splits = c.Nm / span > 0 ? c.Nm / span : 1;
+        for (int rows : {2, 4, 8}) {
+            if (c.store == GRA_F64 && rows == 8 && !blk) continue;
+            for (int un : {1, 2})
+                for (int sp : {1, 2, 4, 8, 16}) {
+                    if (sp > max_splits) continue;
+                    GemvShape s = best;
+                    s.n_block = blk;
+                    s.n_rows = rows;
+                    s.n_unroll = un;
+                    s.n_splits = sp;
+                    double tn, tt;
+                    HIPCHK(time_pair(c, s, reps, xp, wp, &tn, &tt), "time G x");
+                    if (tn < bn) { best = s; bn = tn; bt = tt; }
+                }
+        }
     }
     // non-temporal loads: judged on the pair (they interact across kernels)
     for (int nt = 0; nt <= 3; ++nt) {
@@ -1195,6 +1204,7 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_NONTEMPORAL: *value = s.nontemporal; break;
     case GRA_TUNE_FUSED_WGS: *value = c.fused_wgs; break;
     case GRA_TUNE_FUSED_DEPTH: *value = c.fused_depth; break;
+    case GRA_TUNE_N_BLOCK: *value = s.n_block; break;
     default: return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
     }
     return GRA_OK;

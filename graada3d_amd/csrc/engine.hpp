@@ -66,6 +66,7 @@ struct GemvShape {
     int t_chunks = 1;      // row chunks of G^T w
     int t_unroll = 8;      // rows per loop iteration (4, 8)
     int nontemporal = 3;   // non-temporal loads of G: bit 0 for G x, bit 1 for G^T w
+    int n_block = 0;       // G x: 1 = the waves of a block share its n_rows rows
 };
 
 // Kernel timing classes (gra_timing_get).
@@ -144,7 +145,7 @@ struct Ctx {
     bool gn_have_wl = false;
 
     // launch-shape overrides (0 = automatic; nontemporal -1 = automatic)
-    GemvShape tune_override{0, 0, 0, 0, 0, -1};
+    GemvShape tune_override{0, 0, 0, 0, 0, -1, -1};
     int fused_wgs = 0;            // 0: two per CU
     int fused_depth = 0;          // 0: default (lag, prefetch) variant
 

@@ -12,6 +12,9 @@
 //
 //   y = G x   (Fwd::G_vec_mul dense branch, Fwd.cpp:145): a wave owns R rows
 //             of one column split; each x chunk is loaded once for the R rows.
+//             Block-shared form: the four waves of a workgroup share its R
+//             rows, so each step reads 4 KB contiguous of every row (the
+//             footprint of the G^T w kernel), and the wave sums meet in LDS.
 //   z = G^T w (Fwd::GT_vec_mul dense branch, Fwd.cpp:111): a lane owns the
 //             16-byte column group it loads and walks a chunk of rows; w[row]
 //             is a wave-uniform scalar.
@@ -145,6 +148,82 @@ gemv_n_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t cols_per_
     }
 }
 
+// Block-shared rows: lane t of the workgroup handles columns V*t + k*BSTEP of
+// the split, BSTEP = 256*V (4 KB of G per row and step); per row the lane
+// sums in column order, the wave by the xor butterfly, then the four wave
+// sums in wave order.
+template <typename T, int R, int U, bool NT>
+__global__ void __launch_bounds__(kBlk)
+gemv_nb_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t cols_per_split,
+               const double *__restrict__ x, double *__restrict__ part, const int *done)
+{
+    if (done && *done) return;
+    constexpr int V = Vec16<T>::n;
+    constexpr int BSTEP = V * kBlk;
+    constexpr int W = kBlk / kWave;
+    __shared__ double red[W][R];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int64_t row0 = (int64_t)blockIdx.x * R;
+    const int split = blockIdx.y;
+    int64_t c0 = (int64_t)split * cols_per_split;
+    int64_t c1 = c0 + cols_per_split;
+    if (c1 > ld) c1 = ld;
+    const T *rows[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int64_t row = row0 + r < Nd ? row0 + r : Nd - 1;   // clamp: duplicate reads, result dropped
+        rows[r] = G + row * ld;
+    }
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    int64_t c = c0 + (int64_t)V * threadIdx.x;
+    for (; c + (U - 1) * BSTEP < c1; c += U * BSTEP) {
+        double xv[U][V];
+        double g[U][R][V];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) Load16<T, NT>::run(rows[r] + c + u * BSTEP, g[u][r]);
+#pragma unroll
+            for (int q = 0; q < V; q += 2) {
+                double2 xx = *reinterpret_cast<const double2 *>(x + c + u * BSTEP + q);
+                xv[u][q] = xx.x;
+                xv[u][q + 1] = xx.y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int q = 0; q < V; ++q) acc[r] = fma(g[u][r][q], xv[u][q], acc[r]);
+    }
+    for (; c < c1; c += BSTEP) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double g[V];
+            Load16<T, NT>::run(rows[r] + c, g);
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[r] = fma(g[q], x[c + q], acc[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double v = acc[r];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if (lane == 0) red[wv][r] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < R && row0 + threadIdx.x < Nd) {
+        double s = red[0][threadIdx.x];
+#pragma unroll
+        for (int q = 1; q < W; ++q) s += red[q][threadIdx.x];
+        part[(int64_t)split * Nd + row0 + threadIdx.x] = s;
+    }
+}
+
 // ---------------------------------------------------------------- G^T w
 // SQ: column sums of squares instead (w unused).
 template <typename T, int UR, bool NT, bool SQ = false>
@@ -235,28 +314,47 @@ void launch_n(const void *G, int64_t Nd, int64_t ld, int splits, const double *x
                        x, part, done);
 }
 
-template <typename T, int R, int U>
-void launch_n_nt(const void *G, int64_t Nd, int64_t ld, int splits, bool nt, const double *x,
-                 double *part, const int *done, hipStream_t st)
+template <typename T, int R, int U, bool NT>
+void launch_nb(const void *G, int64_t Nd, int64_t ld, int splits, const double *x, double *part,
+               const int *done, hipStream_t st)
 {
-    if (nt) launch_n<T, R, U, true>(G, Nd, ld, splits, x, part, done, st);
-    else launch_n<T, R, U, false>(G, Nd, ld, splits, x, part, done, st);
+    constexpr int BSTEP = Vec16<T>::n * kBlk;
+    int64_t blocks = (Nd + R - 1) / R;
+    int64_t cps = (ld + splits - 1) / splits;
+    cps = (cps + BSTEP - 1) / BSTEP * BSTEP;   // keep 16-B lane alignment
+    dim3 grid((unsigned)blocks, (unsigned)splits), block(kBlk);
+    hipLaunchKernelGGL((gemv_nb_kernel<T, R, U, NT>), grid, block, 0, st, (const T *)G, Nd, ld,
+                       cps, x, part, done);
+}
+
+template <typename T, int R, int U>
+void launch_n_nt(const void *G, int64_t Nd, int64_t ld, int splits, bool nt, bool blk,
+                 const double *x, double *part, const int *done, hipStream_t st)
+{
+    if (blk) {
+        if (nt) launch_nb<T, R, U, true>(G, Nd, ld, splits, x, part, done, st);
+        else launch_nb<T, R, U, false>(G, Nd, ld, splits, x, part, done, st);
+    } else {
+        if (nt) launch_n<T, R, U, true>(G, Nd, ld, splits, x, part, done, st);
+        else launch_n<T, R, U, false>(G, Nd, ld, splits, x, part, done, st);
+    }
 }
 
 template <typename T>
 void launch_n_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, const double *x,
                     double *part, const int *done, hipStream_t st)
 {
-    const bool nt = (s.nontemporal & 1) != 0;
+    const bool nt = (s.nontemporal & 1) != 0, blk = s.n_block != 0;
+    const int sp = s.n_splits;
     if (s.n_rows == 2) {
-        if (s.n_unroll >= 2) launch_n_nt<T, 2, 2>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
-        else launch_n_nt<T, 2, 1>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+        if (s.n_unroll >= 2) launch_n_nt<T, 2, 2>(G, Nd, ld, sp, nt, blk, x, part, done, st);
+        else launch_n_nt<T, 2, 1>(G, Nd, ld, sp, nt, blk, x, part, done, st);
     } else if (s.n_rows >= 8) {
-        if (s.n_unroll >= 2) launch_n_nt<T, 8, 2>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
-        else launch_n_nt<T, 8, 1>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+        if (s.n_unroll >= 2) launch_n_nt<T, 8, 2>(G, Nd, ld, sp, nt, blk, x, part, done, st);
+        else launch_n_nt<T, 8, 1>(G, Nd, ld, sp, nt, blk, x, part, done, st);
     } else {
-        if (s.n_unroll >= 2) launch_n_nt<T, 4, 2>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
-        else launch_n_nt<T, 4, 1>(G, Nd, ld, s.n_splits, nt, x, part, done, st);
+        if (s.n_unroll >= 2) launch_n_nt<T, 4, 2>(G, Nd, ld, sp, nt, blk, x, part, done, st);
+        else launch_n_nt<T, 4, 1>(G, Nd, ld, sp, nt, blk, x, part, done, st);
     }
 }
 
@@ -283,9 +381,10 @@ void launch_t_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, c
 }  // namespace
 
 // Default launch shapes, from a sweep on MI355X (5,000 x 32,000 G in each
-// storage type): G x is fastest with 2 / 4 / 8 rows per wave for FP64 / FP32 /
-// BF16 and ~8k waves in flight; G^T w with long row chunks, ~256 blocks
-// (one per CU) in total; non-temporal loads help both.
+// storage type): G x is fastest with 4 / 8 rows per wave for FP32 / BF16 and
+// ~8k waves in flight, and for FP64 with 4 rows shared by a workgroup;
+// G^T w with long row chunks, ~256 blocks (one per CU) in total;
+// non-temporal loads help both.
 GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm)
 {
     GemvShape s;
@@ -297,6 +396,17 @@ GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm)
     int64_t waves = (Nd + s.n_rows - 1) / s.n_rows;
     int64_t sp = (8192 + waves - 1) / waves;
     int64_t max_by_cols = Nm / (4 * V * kWave) > 0 ? Nm / (4 * V * kWave) : 1;
+    if (dt == GRA_F64) {
+        // FP64: block-shared rows, 4 per workgroup, about 1,250 workgroups
+        // (186 us against 190 us for the best wave-owned shape on the
+        // 5,000 x 32,000 G, both between G^T w launches as in CG)
+        s.n_block = 1;
+        s.n_rows = 4;
+        s.n_unroll = 1;
+        int64_t blocks = (Nd + 3) / 4;
+        sp = (1250 + blocks - 1) / blocks;
+        max_by_cols = Nm / (2 * V * kBlk) > 0 ? Nm / (2 * V * kBlk) : 1;
+    }
     sp = sp > max_by_cols ? max_by_cols : sp;
     s.n_splits = (int)(sp < 1 ? 1 : (sp > 64 ? 64 : sp));
     // G^T w: about 224 workgroups (column blocks x row chunks), never just

@@ -108,7 +108,8 @@ class Context:
 
     def tune(self, **kw):
         """Override G-product launch shapes (n_rows, n_unroll, n_splits,
-        t_chunks, t_unroll, nontemporal); 0 (or -1 for nontemporal) = auto."""
+        t_chunks, t_unroll, nontemporal, n_block); 0 (-1 for nontemporal and
+        n_block) = auto."""
         for k, v in kw.items():
             self.check(self.L.gra_tune_set(self.h, _lib.TUNE[k], int(v)), f"tune {k}")
 

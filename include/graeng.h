@@ -312,6 +312,8 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
                                    capped at what can be co-resident      */
 #define GRA_TUNE_FUSED_DEPTH 7  /* fused: (lag L, prefetch PF) variant 1..7 =
                                    (3,2) (4,2) (5,1) (6,1) (6,2) (6,3) (8,2) */
+#define GRA_TUNE_N_BLOCK 8      /* G*x rows owned by a wave (0) or shared by the
+                                   4 waves of a workgroup (1); -1 = auto */
 gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
 /*

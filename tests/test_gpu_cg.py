@@ -174,13 +174,16 @@ def test_launch_shapes_do_not_change_results_beyond_rounding(ctx):
     rng = np.random.default_rng(2)
     x, y = rng.standard_normal(inv.Nm), rng.standard_normal(inv.Nd)
     a, b = inv.G_vec_mul(x), inv.GT_vec_mul(y)
-    for kw in (dict(n_rows=8, n_unroll=1, n_splits=3, nontemporal=0),
-               dict(n_rows=2, n_unroll=2, n_splits=1, t_chunks=1, t_unroll=4),
-               dict(t_chunks=37, t_unroll=8, nontemporal=1)):
+    for kw in (dict(n_block=0, n_rows=8, n_unroll=1, n_splits=3, nontemporal=0),
+               dict(n_block=0, n_rows=2, n_unroll=2, n_splits=1, t_chunks=1, t_unroll=4),
+               dict(t_chunks=37, t_unroll=8, nontemporal=1),
+               dict(n_block=1, n_rows=4, n_unroll=1, n_splits=1),
+               dict(n_block=1, n_rows=2, n_unroll=2, n_splits=3, nontemporal=0),
+               dict(n_block=1, n_rows=8, n_unroll=2, n_splits=2, nontemporal=1)):
         inv.ctx.tune(**kw)
         assert rel_norm(inv.G_vec_mul(x), a) < 1e-14
         assert rel_norm(inv.GT_vec_mul(y), b) < 1e-14
-    inv.ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1)
+    inv.ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1, n_block=-1)
 
 
 def test_autotune_keeps_results(ctx):
@@ -192,11 +195,12 @@ def test_autotune_keeps_results(ctx):
         t = inv.ctx.tuning()
         assert ms_n > 0 and ms_t > 0
         assert t["n_rows"] in (2, 4, 8) and t["n_unroll"] in (1, 2) and 1 <= t["n_splits"] <= 16
+        assert t["n_block"] in (0, 1)
         assert 1 <= t["t_chunks"] <= 32 and t["t_unroll"] in (4, 8) and 0 <= t["nontemporal"] <= 3
         assert rel_norm(inv.G_vec_mul(x), oracle.gemv(G, x)) < 1e-14
         assert rel_norm(inv.GT_vec_mul(y), oracle.gemv(G, y, trans=True)) < 1e-14
     finally:
-        inv.ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1)
+        inv.ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1, n_block=-1)
 
 
 def test_helpers_vs_oracle(ctx):
