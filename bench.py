@@ -620,7 +620,7 @@ def main():
     # measured for the 5000 x 32000 FP64 G only.
     traffic = None
     if small:
-        traffic = {"gemv_t": 1280.6e6 + 16.9e6, "gemv_n": 1282.1e6 + 0.3e6}[dom] / 1e9
+        traffic = {"gemv_t": 1280.6e6 + 0.8e6, "gemv_n": 1282.1e6 + 0.04e6}[dom] / 1e9
     out = {
         "metric": METRIC,
         "value": ref["iter_s"],
