@@ -133,6 +133,13 @@ bool points_valid(const double *pt, int64_t n)
 
 int ncomp(uint32_t flag) { return __builtin_popcount(flag & 0x3FEu); }
 
+bool increasing(const double *v, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i]) || (i && !(v[i] > v[i - 1]))) return false;
+    return true;
+}
+
 size_t dtype_size(gra_dtype d) { return d == GRA_F64 ? 8 : (d == GRA_F32 ? 4 : 2); }
 
 // A G (dense or wavelet-compressed) is assembled.
@@ -300,6 +307,60 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
     if (hbad)
         return set_error(c, GRA_DEGENERATE_GEOMETRY,
                          "an observation point lies on a prism edge (s0*s1 <= 0, Integral.cpp:77)");
+    return GRA_OK;
+}
+
+gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, const double *ys,
+                                   int64_t ny, const double *zs, int64_t nz, const double *pts,
+                                   int64_t N_obs, uint32_t field_flag, gra_dtype store)
+{
+    CTX_GUARD(ctx);
+    if (!xs || !ys || !zs || !pts || nx <= 0 || ny <= 0 || nz <= 0 || N_obs <= 0)
+        return set_error(c, GRA_BAD_ARG, "empty mesh or point set");
+    if ((field_flag & 1u) || (field_flag & ~0x3FFu) || !(field_flag & 0x3FEu))
+        return set_error(c, GRA_BAD_ARG, "field_flag must select components among bits 1..9");
+    if (store != GRA_F64 && store != GRA_F32 && store != GRA_BF16)
+        return set_error(c, GRA_BAD_ARG, "unknown storage type");
+    if (!increasing(xs, nx + 1) || !increasing(ys, ny + 1) || !increasing(zs, nz + 1))
+        return set_error(c, GRA_BAD_ARG, "node coordinates must be finite and strictly increasing");
+    if (nx * ny > (int64_t)INT32_MAX / nz)
+        return set_error(c, GRA_BAD_ARG, "Nm exceeds int32 column indices");
+    if (!points_valid(pts, N_obs)) return set_error(c, GRA_BAD_ARG, "non-finite point");
+    for (int64_t i = 0; i < N_obs; ++i)
+        if (!(pts[3 * i + 2] < zs[0]))
+            return set_error(c, GRA_BAD_ARG,
+                             "points must lie strictly above the mesh (z < zs[0], z down)");
+    const int nc = ncomp(field_flag);
+    const int tile = node_assemble_tile(nc, nz);
+    if (tile < 1)
+        return set_error(c, GRA_BAD_ARG, "too many z levels for the corner-sum assembly");
+    const int64_t Nm = nx * ny * nz;
+    const int64_t Nd = (int64_t)nc * N_obs;
+    const int64_t ld = (Nm + 63) / 64 * 64;
+    c.Nd = c.Nm = c.N_obs = c.ld = 0;          // G is invalid until the kernel has run
+    c.gn_ready = false;
+    c.wav = false;
+    c.partn_holds_gn_model = false;
+    HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
+    if (ld > Nm)
+        HIPCHK(hipMemsetAsync(c.G.p, 0, dtype_size(store) * (size_t)Nd * (size_t)ld, c.stream),
+               "clear G padding");
+    std::vector<double> axes((size_t)(nx + ny + nz + 3));
+    std::copy(xs, xs + nx + 1, axes.begin());
+    std::copy(ys, ys + ny + 1, axes.begin() + nx + 1);
+    std::copy(zs, zs + nz + 1, axes.begin() + nx + ny + 2);
+    HIPCHK(put(c.rect_axes, axes.data(), axes.size(), c.stream), "upload axes");
+    HIPCHK(put(c.pts_dev, pts, (size_t)(3 * N_obs), c.stream), "upload points");
+    const double *da = c.rect_axes.as<double>();
+    {
+        TimedRegion tr(c, kTimeAssemble);
+        HIPCHK(launch_node_assemble(da, da + nx + 1, da + nx + ny + 2, nx, ny, nz,
+                                    c.pts_dev.as<double>(), N_obs, field_flag, store, c.G.p, ld,
+                                    tile, c.stream),
+               "corner-sum assembly");
+    }
+    HIPCHK(hipStreamSynchronize(c.stream), "assemble");
+    c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
     return GRA_OK;
 }
 
@@ -668,13 +729,6 @@ gra_status gra_forward(gra_ctx *ctx, const double *prisms, const double *rho, in
         return set_error(c, GRA_DEGENERATE_GEOMETRY,
                          "an observation point lies on a prism edge (s0*s1 <= 0, Integral.cpp:77)");
     return GRA_OK;
-}
-
-static bool increasing(const double *v, int64_t n)
-{
-    for (int64_t i = 0; i < n; ++i)
-        if (!std::isfinite(v[i]) || (i && !(v[i] > v[i - 1]))) return false;
-    return true;
 }
 
 gra_status gra_forward_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, const double *ys,

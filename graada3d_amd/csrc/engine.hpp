@@ -109,6 +109,7 @@ struct Ctx {
     DevBuf vec_nd[4], vec_nm[12], vec_row[3];
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
+    DevBuf rect_axes;                                // gra_assemble_rectilinear
     DevBuf pts_dev;                                  // the assembled G's points (expansion)
     DevBuf batch_prisms, batch_items, batch_bad;     // gra_assemble_batch (first context)
     DevBuf red;                // block partial sums for reductions
@@ -189,6 +190,14 @@ hipError_t launch_assemble_batch(const BatchItem *d_items, int n_items, int64_t 
 
 // Node-sum forward on a rectilinear mesh (rectilinear_kernels.hip); w holds
 // (nx+1)(ny+1)(nz+1) node weights.
+// G of a rectilinear mesh by corner sums (rectilinear_kernels.hip): tile =
+// cell columns per tile side; its corner values take
+// 8 * ncomp * (tile+1)^2 * (nz+1) bytes of LDS.
+hipError_t launch_node_assemble(const double *xs, const double *ys, const double *zs, int64_t nx,
+                                int64_t ny, int64_t nz, const double *pts, int64_t N_obs,
+                                uint32_t flag, gra_dtype store, void *G, int64_t ld, int tile,
+                                hipStream_t st);
+int node_assemble_tile(int ncomp, int64_t nz);
 hipError_t launch_node_forward(const double *xs, const double *ys, const double *zs, int64_t nx,
                                int64_t ny, int64_t nz, const double *rho, double *w,
                                const double *pts, int64_t N_obs, uint32_t flag, double *out,

@@ -24,6 +24,12 @@
 // -0 in x; where x = y = 0 both ratio terms take the limit along the shift
 // (-d, -d, 0), atan(z/r) -> pi/4.  The field is continuous outside the body,
 // so one consistent limit for all nodes yields the field at p.
+//
+// The same corner sums assemble G (node_assemble_kernel): entry (k*N_obs + i,
+// cell) = c_k * sum over the cell's 8 corners of s * F_k(corner - p_i), s = +1
+// at the corner (x1, y1, z1) and alternating, with the corner values of a
+// tile of cells computed once per point and shared through LDS.
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
@@ -78,6 +84,90 @@ __global__ void node_weights_kernel(const double *__restrict__ rho, int64_t nx, 
 // cases take).
 constexpr int kMaxLevels = 1024;
 
+// Which transcendental terms the requested components need.
+struct NodeTerms {
+    bool lx, ly, lz, axy, ayz, axz, all_atan, any_atan;
+    __device__ explicit NodeTerms(uint32_t flag)
+        : lx(flag & ((1u << 1) | (1u << 3) | (1u << 6))),
+          ly(flag & ((1u << 1) | (1u << 2) | (1u << 5))),
+          lz(flag & ((1u << 2) | (1u << 3) | (1u << 8))),
+          axy(flag & ((1u << 1) | (1u << 4))),
+          ayz(flag & ((1u << 2) | (1u << 7))),
+          axz(flag & ((1u << 3) | (1u << 9)))
+    {
+        all_atan = axy && ayz && axz;
+        any_atan = axy || ayz || axz;
+    }
+};
+
+// The transcendental terms at one node: x, y, z = node - point (z > 0);
+// xy2 = x^2 + y^2 (fma), axis = (x == 0 && y == 0), half_pi_s = (pi/2) sgn(x)
+// sgn(y) with signed zeros.
+struct CornerTerms {
+    double lx, ly, lz, axy, ayz, axz;
+};
+
+__device__ inline CornerTerms corner_terms(double x, double y, double z, double xy2, bool axis,
+                                          double half_pi_s, const NodeTerms &nt,
+                                          const double *lg_G, const double *lg_T)
+{
+    CornerTerms c;
+    const double r = sqrt(xy2 + z * z);
+    // ln(u + r) for u < 0 would lose digits to the cancellation u + r (a
+    // point far along -u): there it is ln((r^2 - u^2) / (r - u)), the
+    // numerator formed from the other two squares (positive: z > 0).
+    // z > 0 needs no care.
+    if (nt.lx) c.lx = table_log(x >= 0.0 ? x + r : fma(y, y, z * z) / (r - x), lg_G, lg_T);
+    else c.lx = 0.0;
+    if (nt.ly) c.ly = table_log(y >= 0.0 ? y + r : fma(x, x, z * z) / (r - y), lg_G, lg_T);
+    else c.ly = 0.0;
+    c.lz = nt.lz ? table_log(z + r, lg_G, lg_T) : 0.0;
+    c.axy = c.ayz = c.axz = 0.0;
+    if (nt.any_atan) {
+        if (nt.axy) c.axy = atan_ratio(x * y, z * r);
+        if (axis) {
+            const double a = atan_ratio(z, r);
+            c.ayz = a;
+            c.axz = a;
+        } else if (nt.all_atan) {
+            c.ayz = atan_ratio(y * z, x * r);
+            c.axz = half_pi_s - c.axy - c.ayz;
+        } else {
+            if (nt.ayz) c.ayz = atan_ratio(y * z, x * r);
+            if (nt.axz) c.axz = atan_ratio(x * z, y * r);
+        }
+    }
+    return c;
+}
+
+// Corner function of component `bit` from the terms.
+__device__ inline double corner_component(int bit, double x, double y, double z,
+                                          const CornerTerms &c)
+{
+    switch (bit) {
+    case 1: return x * c.ly + y * c.lx - z * c.axy;
+    case 2: return y * c.lz + z * c.ly - x * c.ayz;
+    case 3: return z * c.lx + x * c.lz - y * c.axz;
+    case 4: return c.axy;
+    case 5: return c.ly;
+    case 6: return c.lx;
+    case 7: return c.ayz;
+    case 8: return c.lz;
+    default: return c.axz;
+    }
+}
+
+// Output scale of component bit: -G0*1e5 (g, mGal), -/+G0*1e9 (T, Eotvos).
+__device__ inline double comp_scale(int bit)
+{
+    const double cg = -kG0 * 1e5, ct = kG0 * 1e9;
+    switch (bit) {
+    case 1: case 2: case 3: return cg;
+    case 4: case 7: case 9: return -ct;
+    default: return ct;
+    }
+}
+
 __global__ void __launch_bounds__(kBlk)
 node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys,
                     const double *__restrict__ zs, int64_t nx, int64_t ny, int64_t nz,
@@ -99,15 +189,7 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
         for (int k = threadIdx.x; k < NZ; k += kBlk) zl[k] = zs[k] - pz;
     }
     __syncthreads();
-    // which transcendental terms the requested components need (uniform)
-    const bool need_lx = flag & ((1u << 1) | (1u << 3) | (1u << 6));
-    const bool need_ly = flag & ((1u << 1) | (1u << 2) | (1u << 5));
-    const bool need_lz = flag & ((1u << 2) | (1u << 3) | (1u << 8));
-    const bool need_axy = flag & ((1u << 1) | (1u << 4));
-    const bool need_ayz = flag & ((1u << 2) | (1u << 7));
-    const bool need_axz = flag & ((1u << 3) | (1u << 9));
-    const bool all_atan = need_axy && need_ayz && need_axz;
-    const bool any_atan = need_axy || need_ayz || need_axz;
+    const NodeTerms nt(flag);   // uniform
     double acc[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) acc[q] = 0.0;
@@ -116,41 +198,16 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
         const double x = xs[i] - px, y = ys[j] - py;
         const double xy2 = fma(x, x, y * y);
         const bool axis = (x == 0.0) && (y == 0.0);
-        const double xy = x * y;
         // (pi/2) sgn(x) sgn(y) with the sign of a zero: the limits 1/(+-0)
         // gives the atans it replaces
         const double half_pi_s = copysign(1.5707963267948966, x) * copysign(1.0, y);
         for (int k = 0; k < NZ; ++k) {
             const double wn = w[(int64_t)k * NCOL + col];
             const double z = lds_levels ? zl[k] : zs[k] - pz;
-            const double r = sqrt(xy2 + z * z);
-            const double lx = need_lx ? table_log(x + r, lg_G, lg_T) : 0.0;
-            const double ly = need_ly ? table_log(y + r, lg_G, lg_T) : 0.0;
-            const double lz = need_lz ? table_log(z + r, lg_G, lg_T) : 0.0;
-            double axy = 0.0, ayz = 0.0, axz = 0.0;
-            if (any_atan) {
-                if (need_axy) axy = atan_ratio(xy, z * r);
-                if (axis) {
-                    const double a = atan_ratio(z, r);
-                    ayz = a;
-                    axz = a;
-                } else if (all_atan) {
-                    ayz = atan_ratio(y * z, x * r);
-                    axz = half_pi_s - axy - ayz;
-                } else {
-                    if (need_ayz) ayz = atan_ratio(y * z, x * r);
-                    if (need_axz) axz = atan_ratio(x * z, y * r);
-                }
-            }
-            if (flag & (1u << 1)) acc[1] += wn * (x * ly + y * lx - z * axy);
-            if (flag & (1u << 2)) acc[2] += wn * (y * lz + z * ly - x * ayz);
-            if (flag & (1u << 3)) acc[3] += wn * (z * lx + x * lz - y * axz);
-            if (flag & (1u << 4)) acc[4] += wn * axy;
-            if (flag & (1u << 5)) acc[5] += wn * ly;
-            if (flag & (1u << 6)) acc[6] += wn * lx;
-            if (flag & (1u << 7)) acc[7] += wn * ayz;
-            if (flag & (1u << 8)) acc[8] += wn * lz;
-            if (flag & (1u << 9)) acc[9] += wn * axz;
+            const CornerTerms ct = corner_terms(x, y, z, xy2, axis, half_pi_s, nt, lg_G, lg_T);
+#pragma unroll
+            for (int bit = 1; bit < 10; ++bit)
+                if (flag & (1u << bit)) acc[bit] += wn * corner_component(bit, x, y, z, ct);
         }
     }
     __shared__ double red[kBlk / 64][10];
@@ -165,8 +222,6 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double cg = -kG0 * 1e5, ct = kG0 * 1e9;
-        const double scale[10] = {0.0, cg, cg, cg, -ct, ct, ct, -ct, ct, -ct};
         int nc = 0;
         for (int bit = 1; bit < 10; ++bit) nc += (flag >> bit) & 1u;
         int col = 0;
@@ -174,13 +229,130 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
             if (!(flag & (1u << bit))) continue;
             double s = red[0][bit];
             for (int q = 1; q < kBlk / 64; ++q) s += red[q][bit];
-            out[ip * nc + col] = scale[bit] * s;
+            out[ip * nc + col] = comp_scale(bit) * s;
             ++col;
         }
     }
 }
 
+template <typename T> __device__ inline T store_as(double v);
+template <> __device__ inline double store_as<double>(double v) { return v; }
+template <> __device__ inline float store_as<float>(double v) { return (float)v; }
+template <> __device__ inline __hip_bfloat16 store_as<__hip_bfloat16>(double v)
+{
+    return __float2bfloat16((float)v);
+}
+
+// G by corner sums.  One workgroup per (point, tile of tile x tile cell
+// columns with all nz levels): the corner values of the tile's
+// (tile+1)^2 (nz+1) nodes go to LDS once (component-major, nmax per
+// component), then each cell's entry is the signed 8-corner sum, formed as
+// differences along x, then y, then z (differences of neighbouring values
+// first).  Threads take consecutive cells of one x column of the tile, i.e.
+// consecutive G columns.
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ ys,
+                     const double *__restrict__ zs, int64_t nx, int64_t ny, int64_t nz,
+                     const double *__restrict__ pts, int64_t N_obs, int64_t p0, uint32_t flag,
+                     int tile, int64_t tiles_y, int nmax, T *__restrict__ G, int64_t ld)
+{
+    extern __shared__ double vals[];   // [nc][nmax]
+    __shared__ double lg_G[kLogTable], lg_T[kLogTable];
+    const int64_t ip = p0 + blockIdx.y, t = blockIdx.x;
+    if (ip >= N_obs) return;
+    for (int q = threadIdx.x; q < kLogTable; q += kBlk) {
+        lg_G[q] = kLogG[q];
+        lg_T[q] = kLogT[q];
+    }
+    const int64_t i0 = (t / tiles_y) * tile, j0 = (t % tiles_y) * tile;
+    const int cx = (int)(nx - i0 < tile ? nx - i0 : tile);
+    const int cy = (int)(ny - j0 < tile ? ny - j0 : tile);
+    const int NZ = (int)nz + 1, plane = (cy + 1) * NZ, nnodes = (cx + 1) * plane;
+    const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
+    const NodeTerms nt(flag);
+    __syncthreads();
+    for (int n = threadIdx.x; n < nnodes; n += kBlk) {
+        const int a = n / plane, rem = n - a * plane, b = rem / NZ, k = rem - b * NZ;
+        const double x = xs[i0 + a] - px, y = ys[j0 + b] - py, z = zs[k] - pz;
+        const double xy2 = fma(x, x, y * y);
+        const bool axis = (x == 0.0) && (y == 0.0);
+        const double half_pi_s = copysign(1.5707963267948966, x) * copysign(1.0, y);
+        const CornerTerms ct = corner_terms(x, y, z, xy2, axis, half_pi_s, nt, lg_G, lg_T);
+        int q = 0;
+#pragma unroll
+        for (int bit = 1; bit < 10; ++bit)
+            if (flag & (1u << bit)) vals[q++ * nmax + n] = corner_component(bit, x, y, z, ct);
+    }
+    __syncthreads();
+    const int nzc = (int)nz, per_a = cy * nzc, ncell = cx * per_a;
+    for (int c = threadIdx.x; c < ncell; c += kBlk) {
+        const int a = c / per_a, rem = c - a * per_a, b = rem / nzc, k = rem - b * nzc;
+        const int n0 = a * plane + b * NZ + k;
+        const int64_t j = ((i0 + a) * ny + (j0 + b)) * nz + k;
+        int q = 0;
+#pragma unroll
+        for (int bit = 1; bit < 10; ++bit) {
+            if (!(flag & (1u << bit))) continue;
+            const double *v = vals + q * nmax + n0;
+            const double d00 = v[plane] - v[0];
+            const double d10 = v[plane + NZ] - v[NZ];
+            const double d01 = v[plane + 1] - v[1];
+            const double d11 = v[plane + NZ + 1] - v[NZ + 1];
+            const double s = (d11 - d01) - (d10 - d00);
+            G[((int64_t)q * N_obs + ip) * ld + j] = store_as<T>(comp_scale(bit) * s);
+            ++q;
+        }
+    }
+}
+
 }  // namespace
+
+// Largest tile side (<= 16 cell columns) whose corner values fit the LDS
+// budget; 0 if not even one column fits.
+int node_assemble_tile(int ncomp, int64_t nz)
+{
+    constexpr double kLdsBudget = 32768.0;
+    for (int t = 16; t >= 1; --t)
+        if (8.0 * ncomp * (t + 1) * (t + 1) * (double)(nz + 1) <= kLdsBudget) return t;
+    return 0;
+}
+
+hipError_t launch_node_assemble(const double *xs, const double *ys, const double *zs, int64_t nx,
+                                int64_t ny, int64_t nz, const double *pts, int64_t N_obs,
+                                uint32_t flag, gra_dtype store, void *G, int64_t ld, int tile,
+                                hipStream_t st)
+{
+    int nc = 0;
+    for (int bit = 1; bit < 10; ++bit) nc += (flag >> bit) & 1u;
+    const int nmax = (tile + 1) * (tile + 1) * (int)(nz + 1);
+    const size_t lds = sizeof(double) * (size_t)nc * (size_t)nmax;
+    const int64_t tiles_x = (nx + tile - 1) / tile, tiles_y = (ny + tile - 1) / tile;
+    const int64_t ntiles = tiles_x * tiles_y;
+    constexpr int64_t kMaxY = 65535;   // points per launch (grid y)
+    for (int64_t p0 = 0; p0 < N_obs; p0 += kMaxY) {
+        const int64_t np = N_obs - p0 < kMaxY ? N_obs - p0 : kMaxY;
+        const dim3 grid((unsigned)ntiles, (unsigned)np), block(kBlk);
+        switch (store) {
+        case GRA_F64:
+            hipLaunchKernelGGL(node_assemble_kernel<double>, grid, block, lds, st, xs, ys, zs, nx,
+                               ny, nz, pts, N_obs, p0, flag, tile, tiles_y, nmax, (double *)G, ld);
+            break;
+        case GRA_F32:
+            hipLaunchKernelGGL(node_assemble_kernel<float>, grid, block, lds, st, xs, ys, zs, nx,
+                               ny, nz, pts, N_obs, p0, flag, tile, tiles_y, nmax, (float *)G, ld);
+            break;
+        default:
+            hipLaunchKernelGGL(node_assemble_kernel<__hip_bfloat16>, grid, block, lds, st, xs, ys,
+                               zs, nx, ny, nz, pts, N_obs, p0, flag, tile, tiles_y, nmax,
+                               (__hip_bfloat16 *)G, ld);
+            break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 hipError_t launch_node_forward(const double *xs, const double *ys, const double *zs, int64_t nx,
                                int64_t ny, int64_t nz, const double *rho, double *w,

@@ -158,12 +158,30 @@ class Fwd:
         self.Nd = len(component_bits(self.field_flag)) * self.N_obs
         self._assembled = False
 
-    def compute_G(self, store: str = "f64"):
-        """Fwd::compute_G (Fwd.cpp:47-79); G stays in device memory."""
+    def compute_G(self, store: str = "f64", method: str = "faces"):
+        """Fwd::compute_G (Fwd.cpp:47-79); G stays in device memory.
+
+        method "faces": the reference's per-prism face/edge formulation
+        (gra_assemble); "nodes": corner sums shared between the cells of a
+        tensor-product mesh (gra_assemble_rectilinear; the prisms must be a
+        regular_mesh-ordered rectilinear mesh and the points above it)."""
         c = self.ctx
-        st = c.L.gra_assemble(c.h, _p(self.prisms), self.Nm, _p(self.points), self.N_obs,
-                              self.field_flag, _lib.DTYPES[store])
-        c.check(st, "gra_assemble")
+        if method == "faces":
+            st = c.L.gra_assemble(c.h, _p(self.prisms), self.Nm, _p(self.points), self.N_obs,
+                                  self.field_flag, _lib.DTYPES[store])
+            c.check(st, "gra_assemble")
+        elif method == "nodes":
+            from .mesh import rectilinear_axes
+            axes = rectilinear_axes(self.prisms)
+            if axes is None:
+                raise ValueError("method 'nodes' needs a tensor-product mesh in regular_mesh order")
+            xs, ys, zs = axes
+            st = c.L.gra_assemble_rectilinear(c.h, _p(xs), xs.size - 1, _p(ys), ys.size - 1,
+                                              _p(zs), zs.size - 1, _p(self.points), self.N_obs,
+                                              self.field_flag, _lib.DTYPES[store])
+            c.check(st, "gra_assemble_rectilinear")
+        else:
+            raise ValueError(f"unknown assembly method {method!r}")
         self._assembled = True
 
     def set_mesh(self, prisms):

@@ -42,6 +42,35 @@ def regular_mesh(x_lim, nx, y_lim, ny, z_lim, nz) -> np.ndarray:
     return np.stack([xs[i], xs[i + 1], ys[j], ys[j + 1], zs[k], zs[k + 1]], axis=1)
 
 
+def rectilinear_axes(prisms: np.ndarray):
+    """(xs, ys, zs) if the prisms are exactly the tensor-product cells of those
+    node coordinates in the order of ``regular_mesh`` (cell id = i*ny*nz +
+    j*nz + k), else None.  Bitwise comparison: shared faces must carry
+    identical coordinates."""
+    p = np.ascontiguousarray(prisms, dtype=np.float64).reshape(-1, 6)
+    n = p.shape[0]
+    if n == 0:
+        return None
+    # z levels: the leading run of cells in the first (i, j) column
+    same_col = (p[:, 0] == p[0, 0]) & (p[:, 2] == p[0, 2])
+    nz = int(np.argmin(same_col)) if not same_col.all() else n
+    # y columns: the leading run of cells in the first x slab
+    same_slab = p[:, 0] == p[0, 0]
+    nyz = int(np.argmin(same_slab)) if not same_slab.all() else n
+    if nz == 0 or nyz % nz or n % nyz:
+        return None
+    ny, nx = nyz // nz, n // nyz
+    zs = np.append(p[:nz, 4], p[nz - 1, 5])
+    ys = np.append(p[0:nyz:nz, 2], p[nyz - nz, 3])
+    xs = np.append(p[0:n:nyz, 0], p[n - nyz, 1])
+    if not (np.all(np.diff(xs) > 0) and np.all(np.diff(ys) > 0) and np.all(np.diff(zs) > 0)):
+        return None
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    ref = np.stack([xs[i], xs[i + 1], ys[j], ys[j + 1], zs[k], zs[k + 1]], axis=1)
+    return (xs, ys, zs) if np.array_equal(ref, p) else None
+
+
 def set_region(prisms: np.ndarray, values: np.ndarray, box, value: float) -> None:
     """Assign ``value`` to every cell fully inside ``box`` = (x0,x1,y0,y1,z0,z1),
     with the 1e-10 boundary tolerance of ``Mesh::set_parameter_in_a_region``

@@ -85,6 +85,25 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
                         int64_t N_obs, uint32_t field_flag, gra_dtype store);
 
 /*
+ * Fwd::compute_G (Fwd.cpp:47-79) for a mesh built by
+ * Mesh::generate_regular_mesh (Mesh.cpp:828-846) or any other tensor-product
+ * mesh: cells are the products of the node coordinates xs[nx+1], ys[ny+1],
+ * zs[nz+1] (strictly increasing, z down), column (i*ny + j)*nz + k is cell
+ * (i, j, k), the row layout and storage are gra_assemble's.  Each entry is
+ * the signed sum of the closed-form corner function over the cell's 8
+ * corners (the quantity the reference's face/edge integrals evaluate), with
+ * the corner values of a tile of cells computed once per point and shared
+ * through LDS.  Agrees with gra_assemble to ~1e-13 of each component block's
+ * largest entry; the relative accuracy of a single entry falls with its
+ * distance (corner-sum cancellation).  Points must lie strictly above the
+ * mesh (z < zs[0]), else GRA_BAD_ARG; a mesh too deep for the LDS tile
+ * (more than ~500 levels for one component) is GRA_BAD_ARG as well.
+ */
+gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, const double *ys,
+                                   int64_t ny, const double *zs, int64_t nz, const double *pts,
+                                   int64_t N_obs, uint32_t field_flag, gra_dtype store);
+
+/*
  * Batched Fwd::compute_G over uneven instances (no counterpart in the
  * reference, which assembles one G per inversion): ctxs[b] (distinct, one
  * device) receives the G of items[b] -- its own mesh, points and component

@@ -152,3 +152,91 @@ def check_against_run(run, fx, out_dir, misfit_rtol=1e-3, final_rtol=1e-6, model
     np.testing.assert_array_equal(cells, fx["cells"])
     assert rel_norm(run.m, fx["model"]) < model_rtol
     assert rel_norm(run.engine.G_vec_mul(run.m), fx["dpred"]) < final_rtol
+
+
+# ---- 60-digit evaluation of a prism's field by its corner sum ---------------
+# The mathematics that pins both the reference's face/edge values and the
+# corner-sum (node) assembly where the two disagree: the prism integral of
+# component `bit` equals c_bit * sum over the 8 corners of s * F_bit(corner - p)
+# (s = +1 at (x1, y1, z1), alternating), evaluated here in decimal arithmetic
+# at 60 digits with the same limits at x = 0 / y = 0 as the device code.
+
+def _dec_pi():
+    from decimal import Decimal
+    return 16 * _dec_atan_small(Decimal(1) / 5) - 4 * _dec_atan_small(Decimal(1) / 239)
+
+
+def _dec_atan_small(v):
+    """atan(v) for |v| <= 1: halve the argument until |v| < 0.05
+    (atan v = 2 atan(v / (1 + sqrt(1 + v^2)))), then the Taylor series."""
+    from decimal import Decimal, getcontext
+    n = 0
+    while abs(v) > Decimal("0.05"):
+        v = v / (1 + (1 + v * v).sqrt())
+        n += 1
+    eps = Decimal(10) ** (-getcontext().prec - 2)
+    s, term, k, v2 = Decimal(0), v, 0, v * v
+    while True:
+        t = term / (2 * k + 1)
+        if abs(t) < eps:
+            break
+        s += -t if k % 2 else t
+        term *= v2
+        k += 1
+    return s * (2 ** n)
+
+
+def _dec_atan_ratio(num, den, half_pi):
+    """atan(num / den) with den = +0 giving sgn(num) * pi/2 (the IEEE limit of
+    num / +0) and num = 0 giving 0."""
+    from decimal import Decimal
+    if num == 0:
+        return Decimal(0)
+    if den == 0:
+        return half_pi if num > 0 else -half_pi
+    q = num / den
+    if abs(q) <= 1:
+        return _dec_atan_small(q)
+    return (half_pi if q > 0 else -half_pi) - _dec_atan_small(1 / q)
+
+
+def corner_exact(prism, point, bit, digits=60):
+    """Field component `bit` (1..9, the reference's field bits) of the unit
+    density prism [x0, x1, y0, y1, z0, z1] at `point`, from its corner sum at
+    `digits` decimal digits; the point must lie above the prism (z < z0)."""
+    from decimal import Decimal, localcontext
+    G0 = Decimal("6.673e-11")
+    with localcontext() as ctx:
+        ctx.prec = digits
+        half_pi = _dec_pi() / 2
+        tot = Decimal(0)
+        for a in (0, 1):
+            for b in (0, 1):
+                for c in (0, 1):
+                    x = Decimal(float(prism[a])) - Decimal(float(point[0]))
+                    y = Decimal(float(prism[2 + b])) - Decimal(float(point[1]))
+                    z = Decimal(float(prism[4 + c])) - Decimal(float(point[2]))
+                    r = (x * x + y * y + z * z).sqrt()
+                    axis = x == 0 and y == 0            # on the corner's vertical line
+
+                    def lx(): return (x + r).ln()
+                    def ly(): return (y + r).ln()
+                    def lz(): return (z + r).ln()
+                    def axy(): return _dec_atan_ratio(x * y, z * r, half_pi)
+                    def ayz():
+                        return _dec_atan_small(z / r) if axis else _dec_atan_ratio(y * z, x * r,
+                                                                                  half_pi)
+                    def axz():
+                        return _dec_atan_small(z / r) if axis else _dec_atan_ratio(x * z, y * r,
+                                                                                  half_pi)
+                    F = {1: lambda: x * ly() + y * lx() - z * axy(),
+                         2: lambda: y * lz() + z * ly() - x * ayz(),
+                         3: lambda: z * lx() + x * lz() - y * axz(),
+                         4: axy, 5: ly, 6: lx, 7: ayz, 8: lz, 9: axz}[bit]()
+                    s = (1 if a else -1) * (1 if b else -1) * (1 if c else -1)
+                    tot += s * F
+        scale = {1: -G0 * Decimal(10) ** 5, 2: -G0 * Decimal(10) ** 5,
+                 3: -G0 * Decimal(10) ** 5}.get(bit)
+        if scale is None:
+            scale = (-1 if bit in (4, 7, 9) else 1) * G0 * Decimal(10) ** 9
+        return float(tot * scale)

@@ -156,3 +156,22 @@ def test_batch_instances_and_their_operators():
         assert (T is not None) == b.cross_gradient
     f = smooth_random_field(np.random.default_rng(0).uniform(0, 4000, (5000, 3)), 300.0, 1)
     assert abs(f.mean()) < 1e-12 and abs(f.std() - 1.0) < 1e-12
+
+
+def test_rectilinear_axes_recovers_tensor_meshes():
+    """mesh.rectilinear_axes: the node coordinates of a regular_mesh-ordered
+    tensor-product mesh, None for anything else (bitwise checks)."""
+    from graada3d_amd import mesh as M
+    for dims in [(40, 40, 20), (1, 4, 3), (3, 1, 1), (1, 1, 1), (5, 3, 1)]:
+        p = M.regular_mesh((0.0, 200.0), dims[0], (-50.0, 10.0), dims[1], (0.0, 30.0), dims[2])
+        axes = M.rectilinear_axes(p)
+        assert axes is not None and [a.size - 1 for a in axes] == list(dims)
+        xs, ys, zs = axes
+        assert xs[0] == 0.0 and xs[-1] == 200.0 and ys[0] == -50.0 and zs[-1] == 30.0
+    p = M.regular_mesh((0.0, 200.0), 4, (0.0, 10.0), 3, (0.0, 30.0), 2)
+    q = p.copy()
+    q[5, 1] = np.nextafter(q[5, 1], 1e9)          # one face off by an ulp
+    assert M.rectilinear_axes(q) is None
+    assert M.rectilinear_axes(p[::-1]) is None
+    assert M.rectilinear_axes(p[:-1]) is None
+    assert M.rectilinear_axes(np.zeros((0, 6))) is None

@@ -191,3 +191,25 @@ def test_threaded_oracle_is_bit_identical_to_single_threaded():
     oracle.set_threads(1)
     for a, b in zip(out[1], out[4]):
         assert np.array_equal(a, b)
+
+
+def test_oracle_matches_60_digit_corner_sums():
+    """The oracle's face/edge values (the reference's formulation) against
+    the prism integral evaluated as a corner sum in 60-digit decimal
+    arithmetic (helpers.corner_exact): all nine components, random prisms
+    and points above them, max error <= 1e-12 of each component's largest
+    value.  A wrong sign, scale or term in either fails this."""
+    from helpers import corner_exact
+    rng = np.random.default_rng(3)
+    got, ref = [], []
+    for _ in range(60):
+        pr = np.concatenate([np.sort(rng.uniform(-100, 100, 2)), np.sort(rng.uniform(-100, 100, 2)),
+                             np.sort(rng.uniform(0, 100, 2))])
+        pt = np.array([rng.uniform(-150, 150), rng.uniform(-150, 150), rng.uniform(-50, -1)])
+        G, st = oracle.assemble(pr[None, :], pt[None, :], 0x3FE)
+        assert st == 0
+        got.append(G[:, 0])
+        ref.append([corner_exact(pr, pt, bit) for bit in range(1, 10)])
+    got, ref = np.array(got), np.array(ref)
+    for k in range(9):
+        assert np.abs(got[:, k] - ref[:, k]).max() <= 1e-12 * np.abs(ref[:, k]).max(), k + 1
