@@ -111,6 +111,28 @@ def run(args):
     dobs, sigma = add_noise(d_clean, fp.points.shape[0], ncomp, fp.seed)
     Wd = np.repeat(1.0 / sigma, fp.points.shape[0])
 
+    # the same G by corner sums over the tensor-product mesh (shared corner
+    # values): its rate, and its products against the face/edge G's
+    xr = np.random.default_rng(11).standard_normal(nm)
+    y_face = inv.G_vec_mul(xr)
+    inv.compute_G("f64", method="nodes")          # untimed first call
+    ctx.timing(True)
+    ctx.timing_reset()
+    for _ in range(2):
+        inv.compute_G("f64", method="nodes")
+    node_ms, _ = ctx.timing_get(TIME_ASSEMBLE)
+    ctx.timing(False)
+    y_node = inv.G_vec_mul(xr)
+    nobs = fp.points.shape[0]
+    node_dev = [float(np.linalg.norm(y_node[k * nobs:(k + 1) * nobs] - y_face[k * nobs:(k + 1) * nobs])
+                      / np.linalg.norm(y_face[k * nobs:(k + 1) * nobs])) for k in range(ncomp)]
+    nodes_info = {"entries_per_s": nd * nm / (node_ms * 1e-3), "avg_ms": node_ms,
+                  "kernel": "corner sums over the tensor-product mesh, corner values shared "
+                            "through LDS (gra_assemble_rectilinear), FP64",
+                  "Gx_rel_dev_vs_face_edge_per_component": node_dev}
+    print(f"[bench] node-sum assembly: {nodes_info['entries_per_s']:.3e} entries/s "
+          f"(face/edge next), G x deviation {max(node_dev):.1e}", file=sys.stderr, flush=True)
+
     results = {}
     variants = [(s, False) for s in args.stores]
     if args.fused:
@@ -190,6 +212,7 @@ def run(args):
     ctx.close()
     if skipped:
         results["_skipped"] = skipped
+    results["_nodes"] = nodes_info
     return pb, nd, nm, results, cpu
 
 
@@ -598,6 +621,7 @@ def main():
 
     pb, nd, nm, res, cpu = run(args)
     skipped = res.pop("_skipped", [])
+    nodes_info = res.pop("_nodes")
     ref = res["f64"]
     mixed = {}
     for s, r in res.items():
@@ -658,6 +682,7 @@ def main():
                                     "achieved": ref["entries_per_s"] * FLOP_PER_ENTRY[small] / 1e12,
                                     "peak": 78.6, "unit": "TFLOP/s",
                                     "frac": ref["entries_per_s"] * FLOP_PER_ENTRY[small] / 78.6e12}},
+        "g_assembly_nodes": nodes_info,
         "final_misfit": ref["misfit"],
         "mixed": mixed,
         "mixed_not_run": {k: "fused grid not co-resident for this G (two-pass iteration ran)"

@@ -331,9 +331,11 @@ gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, 
             return set_error(c, GRA_BAD_ARG,
                              "points must lie strictly above the mesh (z < zs[0], z down)");
     const int nc = ncomp(field_flag);
-    const int tile = node_assemble_tile(nc, nz);
+    const int tile = c.node_tile > 0 ? c.node_tile : node_assemble_tile(nc, nz);
     if (tile < 1)
         return set_error(c, GRA_BAD_ARG, "too many z levels for the corner-sum assembly");
+    if (8.0 * nc * (tile + 1) * (tile + 1) * (double)(nz + 1) > 61440.0)
+        return set_error(c, GRA_BAD_ARG, "corner-sum tile too large for LDS");
     const int64_t Nm = nx * ny * nz;
     const int64_t Nd = (int64_t)nc * N_obs;
     const int64_t ld = (Nm + 63) / 64 * 64;
@@ -1125,6 +1127,10 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
         if (value < -1 || value > 1) return set_error(c, GRA_BAD_ARG, "block-shared rows: -1..1");
         o.n_block = (int)value;
         break;
+    case GRA_TUNE_NODE_TILE:
+        if (value < 0 || value > 32) return set_error(c, GRA_BAD_ARG, "corner-sum tile: 0..32");
+        c.node_tile = (int)value;
+        break;
 
     default:
         return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
@@ -1259,6 +1265,7 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_FUSED_WGS: *value = c.fused_wgs; break;
     case GRA_TUNE_FUSED_DEPTH: *value = c.fused_depth; break;
     case GRA_TUNE_N_BLOCK: *value = s.n_block; break;
+    case GRA_TUNE_NODE_TILE: *value = c.node_tile; break;
     default: return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");
     }
     return GRA_OK;

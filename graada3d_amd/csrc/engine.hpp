@@ -110,6 +110,7 @@ struct Ctx {
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
     DevBuf rect_axes;                                // gra_assemble_rectilinear
+    int node_tile = 0;                               // its tile side (0: by LDS budget)
     DevBuf pts_dev;                                  // the assembled G's points (expansion)
     DevBuf batch_prisms, batch_items, batch_bad;     // gra_assemble_batch (first context)
     DevBuf red;                // block partial sums for reductions

@@ -235,6 +235,31 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
     }
 }
 
+// (a, b, k) of the flat index i = (a * nb + b) * nk + k, advanced by kBlk per
+// step with carries (the step's own (da, db, dk) computed once).
+struct Walk3 {
+    int a, b, k, da, db, dk, nb, nk;
+    __device__ Walk3(int i, int nb_, int nk_) : nb(nb_), nk(nk_)
+    {
+        k = i % nk;
+        b = (i / nk) % nb;
+        a = i / (nk * nb);
+        dk = kBlk % nk;
+        db = (kBlk / nk) % nb;
+        da = kBlk / (nk * nb);
+    }
+    __device__ inline void next()
+    {
+        k += dk;
+        int carry = k >= nk;
+        k -= carry ? nk : 0;
+        b += db + carry;
+        carry = b >= nb;
+        b -= carry ? nb : 0;
+        a += da + carry;
+    }
+};
+
 template <typename T> __device__ inline T store_as(double v);
 template <> __device__ inline double store_as<double>(double v) { return v; }
 template <> __device__ inline float store_as<float>(double v) { return (float)v; }
@@ -272,8 +297,11 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
     const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
     const NodeTerms nt(flag);
     __syncthreads();
-    for (int n = threadIdx.x; n < nnodes; n += kBlk) {
-        const int a = n / plane, rem = n - a * plane, b = rem / NZ, k = rem - b * NZ;
+    // node n = (a * (cy+1) + b) * NZ + k; (a, b, k) advanced by kBlk per step
+    // without integer divisions in the loop
+    Walk3 wn(threadIdx.x, cy + 1, NZ);
+    for (int n = threadIdx.x; n < nnodes; n += kBlk, wn.next()) {
+        const int a = wn.a, b = wn.b, k = wn.k;
         const double x = xs[i0 + a] - px, y = ys[j0 + b] - py, z = zs[k] - pz;
         const double xy2 = fma(x, x, y * y);
         const bool axis = (x == 0.0) && (y == 0.0);
@@ -285,9 +313,10 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
             if (flag & (1u << bit)) vals[q++ * nmax + n] = corner_component(bit, x, y, z, ct);
     }
     __syncthreads();
-    const int nzc = (int)nz, per_a = cy * nzc, ncell = cx * per_a;
-    for (int c = threadIdx.x; c < ncell; c += kBlk) {
-        const int a = c / per_a, rem = c - a * per_a, b = rem / nzc, k = rem - b * nzc;
+    const int nzc = (int)nz, ncell = cx * cy * nzc;
+    Walk3 wc(threadIdx.x, cy, nzc);
+    for (int c = threadIdx.x; c < ncell; c += kBlk, wc.next()) {
+        const int a = wc.a, b = wc.b, k = wc.k;
         const int n0 = a * plane + b * NZ + k;
         const int64_t j = ((i0 + a) * ny + (j0 + b)) * nz + k;
         int q = 0;
@@ -308,14 +337,24 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
 
 }  // namespace
 
-// Largest tile side (<= 16 cell columns) whose corner values fit the LDS
-// budget; 0 if not even one column fits.
+// Tile side (cell columns, <= 16): the largest whose corner values fit 60 KB
+// of LDS and whose node phase takes at most 8 passes of the workgroup (2,048
+// nodes); without such a tile (deep meshes), the largest that fits.  Swept
+// on MI355X: gz+Tzz on the 40x40x20 mesh peaks at 8 (1.75e11 entries/s; 10:
+// 1.72e11, 12: 1.01e11, 5: 1.36e11), nine components on 64x64x32 at 4
+// (2.63e11; 2: 2.01e11).  0 if not even one column fits.
 int node_assemble_tile(int ncomp, int64_t nz)
 {
-    constexpr double kLdsBudget = 32768.0;
-    for (int t = 16; t >= 1; --t)
-        if (8.0 * ncomp * (t + 1) * (t + 1) * (double)(nz + 1) <= kLdsBudget) return t;
-    return 0;
+    constexpr double kLdsBudget = 61440.0;
+    constexpr double kMaxNodes = 2048.0;
+    int fit = 0;
+    for (int t = 16; t >= 1; --t) {
+        const double nodes = (double)(t + 1) * (t + 1) * (double)(nz + 1);
+        if (8.0 * ncomp * nodes > kLdsBudget) continue;
+        if (!fit) fit = t;
+        if (nodes <= kMaxNodes) return t;
+    }
+    return fit;
 }
 
 hipError_t launch_node_assemble(const double *xs, const double *ys, const double *zs, int64_t nx,

@@ -14,8 +14,9 @@ Runs a fixed, short sequence on GPU 0 for ``rocprofv3 --pmc`` passes:
 
 With ``--assemble`` it instead assembles G once for the same survey (gz+Tzz,
 1.6e8 entries), once with all nine components on a 20x20x10 mesh and a 40x40
-grid (1.44e7 entries) and runs one node-sum forward (40x40x20 mesh, 50x50
-points, 9 components), for FP64 instruction counters:
+grid (1.44e7 entries), the same two G by corner sums
+(gra_assemble_rectilinear) and runs one node-sum forward (40x40x20 mesh,
+50x50 points, 9 components), for FP64 instruction counters:
 
     rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 \
         SQ_INSTS_VALU_TRANS_F64 -d <dir> -o asm -- python3 -m graada3d_amd.probe --assemble
@@ -64,6 +65,9 @@ def main(argv=None) -> None:
         g = Fwd(pr, pts, 0x3FE, ctx=ctx)
         g.compute_G("f64")
         print(f"[probe] assembled {g.Nd} x {g.Nm} (9 components)", flush=True)
+        f.compute_G("f64", method="nodes")
+        g.compute_G("f64", method="nodes")
+        print("[probe] the same two G by corner sums", flush=True)
         xs = M.linspaced(41, 0.0, 2000.0)
         zs = M.linspaced(21, 0.0, 1000.0)
         forward_rectilinear(xs, xs, zs, fp.rho, fp.points, 0x3FE, ctx=ctx)

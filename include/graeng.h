@@ -97,7 +97,8 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
  * largest entry; the relative accuracy of a single entry falls with its
  * distance (corner-sum cancellation).  Points must lie strictly above the
  * mesh (z < zs[0]), else GRA_BAD_ARG; a mesh too deep for the LDS tile
- * (more than ~500 levels for one component) is GRA_BAD_ARG as well.
+ * (more than ~1,900 levels for one component, ~210 for all nine) is
+ * GRA_BAD_ARG as well.
  */
 gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, const double *ys,
                                    int64_t ny, const double *zs, int64_t nz, const double *pts,
@@ -333,6 +334,8 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
                                    (3,2) (4,2) (5,1) (6,1) (6,2) (6,3) (8,2) */
 #define GRA_TUNE_N_BLOCK 8      /* G*x rows owned by a wave (0) or shared by the
                                    4 waves of a workgroup (1); -1 = auto */
+#define GRA_TUNE_NODE_TILE 9    /* gra_assemble_rectilinear tile side in cell
+                                   columns, 1..32; 0 = auto (LDS budget) */
 gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
 /*

@@ -3,6 +3,7 @@
 //
 //   graeng::Fwd                 <-> class Fwd               (src/Forward_modelling/Fwd.h)
 //     compute_G()                    Fwd::compute_G          (Fwd.cpp:47-79)
+//     compute_G_rectilinear(xs,ys,zs)  the same for tensor-product meshes (corner sums)
 //     G_vec_mul(vec, product)        Fwd::G_vec_mul          (Fwd.cpp:114-147)
 //     GT_vec_mul(vec, product)       Fwd::GT_vec_mul         (Fwd.cpp:80-113)
 //     compute_gobs(rho)              Fwd::compute_gobs       (Fwd.h:23-27)
@@ -124,6 +125,21 @@ public:
         check(ctx_.get(), gra_assemble(ctx_.get(), prisms_.data(), Nm, points_.data(), N_obs, flag_,
                                        store),
               "compute_G");
+    }
+    // compute_G for a tensor-product mesh whose prisms are the cells of the
+    // node coordinates xs, ys, zs in Mesh::generate_regular_mesh order (the
+    // Fwd's prisms are not read): corner sums shared through LDS
+    // (gra_assemble_rectilinear).
+    template <class V>
+    void compute_G_rectilinear(const V &xs, const V &ys, const V &zs, gra_dtype store = GRA_F64)
+    {
+        const int64_t nx = (int64_t)xs.size() - 1, ny = (int64_t)ys.size() - 1,
+                      nz = (int64_t)zs.size() - 1;
+        if (nx * ny * nz != Nm) throw Error(GRA_BAD_ARG, "compute_G_rectilinear: cell count");
+        check(ctx_.get(), gra_assemble_rectilinear(ctx_.get(), xs.data(), nx, ys.data(), ny,
+                                                   zs.data(), nz, points_.data(), N_obs, flag_,
+                                                   store),
+              "compute_G_rectilinear");
     }
     template <class V> void G_vec_mul(const V &vec, V &product) const
     {
