@@ -202,6 +202,10 @@ class ResidentCache:
                                             500.0)])
         ncomp = bin(prob.flag).count("1")
         dobs, sigma = add_noise(inv.G_vec_mul(rho), pts.shape[0], ncomp, 7)
+        if not np.all(sigma > 0):
+            ctx.close()
+            raise ValueError(f"problem {prob.name}: the synthetic anomaly holds no cell "
+                             "(zero data, no noise scale)")
         Wd = np.repeat(1.0 / sigma, pts.shape[0])
         nm = prisms.shape[0]
         P = np.full(nm, (1500.0 * 1500.0) / 3000.0)     # barrier P at m = 0, bounds +-1500
@@ -226,9 +230,11 @@ def _percentiles(x):
 
 def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahead: int = 32,
           progress=None, forward_method: str = "nodes", records: bool = False,
-          workers: int = 3):
+          workers: int = 3, outputs: bool = False):
     """Replays `requests` in arrival order; returns a summary dict (times in ms;
-    with `records`, also the per-request arrival / start / finish times).
+    with `records`, also the per-request arrival / start / finish times; with
+    `outputs`, each request's result by request id: the FP32 forward array,
+    or dict(dx, err, iterations) of the CG solve).
 
     `workers` server threads, each with its own engine context (stream),
     take the released requests in arrival order: while one request's kernel
@@ -258,6 +264,7 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
     th = threading.Thread(target=producer, daemon=True)
     th.start()
     rec, rec_lock = [], threading.Lock()
+    results = {}
     work = _queue.Queue()
     errors = []
     t_start = [0.0]
@@ -285,6 +292,7 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
                     size = dict(cells=q["prisms"].shape[0], points=q["pts"].shape[0],
                                 comps=out.shape[1])
                     cold = False
+                    result = out
                 else:
                     t0 = time.perf_counter() - t_start[0]
                     info = {}
@@ -292,15 +300,18 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
                     try:
                         cold = info["admitted"]
                         zero = np.zeros(e["nm"])
-                        e["inv"].solve_cg(0.0, 20, r.params["lam"], zero, zero, e["Wd"],
-                                          e["dobs"], e["P"])
+                        dx, err, its = e["inv"].solve_cg(0.0, 20, r.params["lam"], zero, zero,
+                                                         e["Wd"], e["dobs"], e["P"])
                     finally:
                         cache.release(e)
+                    result = dict(dx=dx, err=err, iterations=its)
                     size = dict(problem=pool[r.params["problem"]].name)
                 t1 = time.perf_counter() - t_start[0]
                 with rec_lock:
-                    rec.append(dict(kind=r.kind, arrival=r.t_arrival, start=t0, finish=t1,
-                                    cold=cold, **size))
+                    rec.append(dict(rid=r.rid, kind=r.kind, arrival=r.t_arrival, start=t0,
+                                    finish=t1, cold=cold, **size))
+                    if outputs:
+                        results[r.rid] = result
                     n = len(rec)
                 if progress and n % progress == 0:
                     print(f"[serve] {n}/{len(requests)} requests, t = {t1:.1f} s", flush=True)
@@ -359,4 +370,16 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         sustainable_rate=(1e3 / np.mean(svc) if svc else 0.0) if workers == 1 else None,
         latency_ms=_percentiles(lat), service_ms=_percentiles(svc), by_kind=by,
         cg_warm_service_ms=_percentiles(warm_cg), cache=cache_stats,
-        **(dict(records=rec) if records else {}))
+        max_in_service=_max_overlap(rec),
+        **(dict(records=rec) if records else {}),
+        **(dict(outputs=results) if outputs else {}))
+
+
+def _max_overlap(rec):
+    """Largest number of requests in service (start..finish) at one time."""
+    ev = sorted([(x["start"], 1) for x in rec] + [(x["finish"], -1) for x in rec])
+    n = best = 0
+    for _, d in ev:
+        n += d
+        best = max(best, n)
+    return best

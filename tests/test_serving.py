@@ -135,3 +135,31 @@ def test_short_replay_on_gpu():
     for k in range(got.shape[1]):
         assert np.abs(got[:, k] - ref[:, k]).max() <= 1e-9 * np.abs(ref[:, k]).max()
         assert np.abs(got_p[:, k] - ref[:, k]).max() <= 1e-10 * np.abs(ref[:, k]).max()
+
+
+@pytest.mark.gpu
+def test_results_alone_equal_in_the_mix():
+    """Each request's result in the concurrent mix (3 server streams with
+    requests overlapping on the device, problems admitted as memory allows)
+    equals its result served alone (one stream, one request on the GPU at a
+    time), bit for bit: the forward arrays and the CG solves' dx, error and
+    iteration count."""
+    pool = [S.CgProblem("tiny", (10, 10, 5), (1000.0, 1000.0, 500.0), 8, 0x12, 0.5),
+            S.CgProblem("tiny9", (10, 10, 5), (1000.0, 1000.0, 500.0), 6, 0x3FE, 0.5)]
+    reqs = S.request_stream(400.0, 0.1, 11, pool, p_forward=0.6)    # a burst
+    for r in reqs:
+        if r.kind == "forward":
+            r.params.update(cells=min(r.params["cells"], 6000), npts=min(r.params["npts"], 1500))
+    kinds = {r.rid: r.kind for r in reqs}
+    assert {"forward", "cg"} <= set(kinds.values())
+    mix = S.serve(reqs, pool, budget_bytes=1e9, workers=3, outputs=True)
+    alone = S.serve(reqs, pool, budget_bytes=1e9, workers=1, outputs=True)
+    assert mix["max_in_service"] >= 2 and alone["max_in_service"] == 1
+    assert set(mix["outputs"]) == set(alone["outputs"]) == set(kinds)
+    for rid, a in alone["outputs"].items():
+        b = mix["outputs"][rid]
+        if kinds[rid] == "forward":
+            assert a.dtype == np.float32 and np.array_equal(a, b), rid
+        else:
+            assert np.array_equal(a["dx"], b["dx"]), rid
+            assert a["err"] == b["err"] and a["iterations"] == b["iterations"] == 20, rid
