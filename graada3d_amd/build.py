@@ -22,7 +22,8 @@ LIB = os.path.join(HERE, "libgraeng.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
 SOURCES = ["prism_kernels.hip", "rectilinear_kernels.hip", "linalg_kernels.hip",
-           "fused_kernels.hip", "wavelet_kernels.hip", "cgls.hip", "engine.hip"]
+           "fused_kernels.hip", "wavelet_kernels.hip", "gemm_kernels.hip", "cgls.hip",
+           "engine.hip"]
 ARCH = os.environ.get("GRAENG_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

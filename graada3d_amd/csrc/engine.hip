@@ -499,6 +499,40 @@ static gra_status gemv_host(gra_ctx *ctx, int trans, const double *vec, double *
     return GRA_OK;
 }
 
+static gra_status matmul_host(gra_ctx *ctx, int trans, const double *in, int64_t k, double *out)
+{
+    CTX_GUARD(ctx);
+    if (!in || !out) return set_error(c, GRA_BAD_ARG, "null matrix");
+    if (k < 1 || k > 1024) return set_error(c, GRA_BAD_ARG, "right-hand sides: 1..1024");
+    if (c.wav) return set_error(c, GRA_BAD_ARG, "not available for the wavelet-compressed G");
+    if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
+    const int64_t nin = trans ? c.Nd : c.Nm, nout = trans ? c.Nm : c.Nd;
+    const int64_t ldv = trans ? c.Nd : c.ld;
+    const MatMulPlan p = plan_mat_mul(c.store, trans != 0, c.Nd, c.Nm, c.ld, k);
+    const size_t in_b = c.store == GRA_F64 ? 8 : 4;
+    HIPCHK(put(c.mm_in, in, (size_t)(nin * k), c.stream), "upload right-hand sides");
+    HIPCHK(c.mm_pack.reserve(in_b * (size_t)p.ncol * (size_t)ldv), "alloc packed");
+    HIPCHK(c.mm_part.reserve(sizeof(double) * (size_t)p.splits * p.ncol * (size_t)p.ldp),
+           "alloc partials");
+    HIPCHK(c.mm_out.reserve(sizeof(double) * (size_t)(nout * k)), "alloc out");
+    HIPCHK(launch_pack_rhs(c.store, c.mm_in.as<double>(), nin, k, ldv, p.ncol, c.mm_pack.p,
+                           c.stream),
+           "pack");
+    {
+        TimedRegion tr(c, kTimeMatMul);
+        HIPCHK(launch_mat_mul(c.G.p, c.store, trans != 0, c.Nd, c.Nm, c.ld, c.mm_pack.p, ldv, p,
+                              c.mm_part.as<double>(), c.stream),
+               "matrix-core product");
+    }
+    HIPCHK(launch_sum_splits(c.mm_part.as<double>(), p, nout, k, c.mm_out.as<double>(), c.stream),
+           "sum splits");
+    HIPCHK(hipMemcpyAsync(out, c.mm_out.p, sizeof(double) * (size_t)(nout * k),
+                          hipMemcpyDeviceToHost, c.stream),
+           "download");
+    HIPCHK(hipStreamSynchronize(c.stream), "mat_mul");
+    return GRA_OK;
+}
+
 gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item *items,
                               gra_dtype store)
 {
@@ -697,6 +731,16 @@ gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product)
 gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product)
 {
     return gemv_host(ctx, 1, vec, product);
+}
+
+gra_status gra_G_mat_mul(gra_ctx *ctx, const double *X, int64_t k, double *Y)
+{
+    return matmul_host(ctx, 0, X, k, Y);
+}
+
+gra_status gra_GT_mat_mul(gra_ctx *ctx, const double *R, int64_t k, double *Z)
+{
+    return matmul_host(ctx, 1, R, k, Z);
 }
 
 gra_status gra_forward(gra_ctx *ctx, const double *prisms, const double *rho, int64_t Nm,

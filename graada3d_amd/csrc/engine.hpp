@@ -72,7 +72,7 @@ struct GemvShape {
 // Kernel timing classes (gra_timing_get).
 enum TimingClass {
     kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeFused = 4,
-    kTimeClasses = 5
+    kTimeMatMul = 5, kTimeClasses = 6
 };
 
 // Launch plan of the single-read fused product (fused_kernels.hip).
@@ -110,6 +110,7 @@ struct Ctx {
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
     DevBuf rect_axes;                                // gra_assemble_rectilinear
+    DevBuf mm_in, mm_pack, mm_part, mm_out;         // gra_G_mat_mul / gra_GT_mat_mul
     int node_tile = 0;                               // its tile side (0: by LDS budget)
     DevBuf pts_dev;                                  // the assembled G's points (expansion)
     DevBuf batch_prisms, batch_items, batch_bad;     // gra_assemble_batch (first context)
@@ -191,6 +192,26 @@ hipError_t launch_assemble_batch(const BatchItem *d_items, int n_items, int64_t 
 
 // Node-sum forward on a rectilinear mesh (rectilinear_kernels.hip); w holds
 // (nx+1)(ny+1)(nz+1) node weights.
+// Multi-right-hand-side products on the matrix cores (gemm_kernels.hip).
+// ncol: right-hand sides padded to tiles of 16; span: summation columns
+// (G x) or rows (G^T r) per split; ldp: leading dimension of the partials.
+struct MatMulPlan {
+    int ncol = 16;
+    int splits = 1;
+    int64_t span = 0;
+    int64_t ldp = 0;
+};
+MatMulPlan plan_mat_mul(gra_dtype dt, bool trans, int64_t Nd, int64_t Nm, int64_t ld, int64_t k);
+// Right-hand sides (host layout, n rows x k columns) into the MFMA input
+// type (FP64 for FP64 G, FP32 otherwise), ncol columns of ldv rows.
+hipError_t launch_pack_rhs(gra_dtype dt, const double *src, int64_t n, int64_t k, int64_t ldv,
+                           int ncol, void *dst, hipStream_t st);
+hipError_t launch_mat_mul(const void *G, gra_dtype dt, bool trans, int64_t Nd, int64_t Nm,
+                          int64_t ld, const void *V, int64_t ldv, const MatMulPlan &p,
+                          double *part, hipStream_t st);
+hipError_t launch_sum_splits(const double *part, const MatMulPlan &p, int64_t n, int64_t k,
+                             double *out, hipStream_t st);
+
 // G of a rectilinear mesh by corner sums (rectilinear_kernels.hip): tile =
 // cell columns per tile side; its corner values take
 // 8 * ncomp * (tile+1)^2 * (nz+1) bytes of LDS.

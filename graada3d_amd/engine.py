@@ -130,7 +130,7 @@ class Context:
         return out
 
 
-TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER, TIME_FUSED = 0, 1, 2, 3, 4
+TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER, TIME_FUSED, TIME_MAT_MUL = 0, 1, 2, 3, 4, 5
 
 
 def csr_struct(m, keep: list) -> GraCsr:
@@ -230,6 +230,34 @@ class Fwd:
         v = _f64(vec, self.Nd, "vec")
         out = np.empty(self.Nm)
         self.ctx.check(self.ctx.L.gra_GT_vec_mul(self.ctx.h, _p(v), _p(out)), "GT_vec_mul")
+        return out
+
+    def G_mat_mul(self, X):
+        """G @ X for the k columns of X (Nm x k) on the matrix cores
+        (gra_G_mat_mul); returns Nd x k."""
+        self._need_G()
+        X = np.asarray(X, dtype=np.float64)
+        if X.ndim != 2 or X.shape[0] != self.Nm:
+            raise ValueError(f"X: expected shape ({self.Nm}, k)")
+        k = X.shape[1]
+        Xc = np.asfortranarray(X)
+        out = np.empty((self.Nd, k), order="F")
+        self.ctx.check(self.ctx.L.gra_G_mat_mul(self.ctx.h, Xc.ctypes.data_as(c_dp), k,
+                                                out.ctypes.data_as(c_dp)), "G_mat_mul")
+        return out
+
+    def GT_mat_mul(self, R):
+        """G^T @ R for the k columns of R (Nd x k) on the matrix cores
+        (gra_GT_mat_mul); returns Nm x k."""
+        self._need_G()
+        R = np.asarray(R, dtype=np.float64)
+        if R.ndim != 2 or R.shape[0] != self.Nd:
+            raise ValueError(f"R: expected shape ({self.Nd}, k)")
+        k = R.shape[1]
+        Rc = np.asfortranarray(R)
+        out = np.empty((self.Nm, k), order="F")
+        self.ctx.check(self.ctx.L.gra_GT_mat_mul(self.ctx.h, Rc.ctypes.data_as(c_dp), k,
+                                                 out.ctypes.data_as(c_dp)), "GT_mat_mul")
         return out
 
     def compute_gobs(self, rho):

@@ -163,6 +163,19 @@ gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product);
 gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product);
 
 /*
+ * k right-hand sides at once on the matrix cores: Y = G X (X: Nm x k, Y:
+ * Nd x k) and Z = G^T R (R: Nd x k, Z: Nm x k), all column-major (column j
+ * at base + j * rows).  The batched form of Fwd::G_vec_mul / GT_vec_mul
+ * (Fwd.cpp:114-147, 80-113), which the reference calls one vector at a time;
+ * no counterpart for k > 1.  FP64-stored G: FP64 MFMA (FP64 throughout);
+ * FP32 / BF16-stored G: FP32 MFMA with the vectors rounded to FP32 and the
+ * FP32 partial sums added into FP64 every 16-32 products.  1 <= k <= 1024;
+ * not for the wavelet-compressed G.  Bit-for-bit repeatable.
+ */
+gra_status gra_G_mat_mul(gra_ctx *ctx, const double *X, int64_t k, double *Y);
+gra_status gra_GT_mat_mul(gra_ctx *ctx, const double *R, int64_t k, double *Z);
+
+/*
  * Replaces AdaptiveInversion::expand_G (AdaptiveInversion.cpp:23-106): the
  * cells at the n_split old columns split_cols[] (strictly increasing) are
  * refined, and each parent's column is replaced in place by the columns of
@@ -355,7 +368,8 @@ gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t);
  * Average device time (ms, from GPU events on the context's stream) of the
  * launches of kernel class `which` since the last gra_timing_reset:
  * 0 = GEMV G*x, 1 = GEMV G^T*x, 2 = G assembly, 3 = one full CG iteration,
- * 4 = fused single-read product (G Pp and G^T Wd^2 G Pp).
+ * 4 = fused single-read product (G Pp and G^T Wd^2 G Pp), 5 = the
+ * matrix-core multi-right-hand-side product (gra_G_mat_mul / gra_GT_mat_mul).
  * Timing events are recorded only while enabled: `on` = 0 disables; n >= 1
  * times every launch of these classes, except inside the CG loop, where only
  * iterations 0, n, 2n, ... are timed (each event is a queue packet of its
