@@ -12,11 +12,13 @@
 // element i of lane l = row 4 (l / 16) + i (f32) or l / 16 + 4 i (f64),
 // column l % 16.
 //
-// Work split: a wave owns one 16 x 16 (rows x right-hand sides) output tile
-// of one split of the summation range; every lane loads 16 bytes of G per
-// step (coalesced row segments), and the splits' FP64 partial tiles are summed
-// in split order by a second kernel, so the products are bit-for-bit
-// repeatable.
+// Work split: G x -- a workgroup owns 16 rows of G and up to four 16-wide
+// tiles of right-hand sides over one split of the summation range, staging G
+// through LDS; G^T r -- a wave owns 16 E columns of G and one tile of
+// right-hand sides over one split of the rows, every lane loading 16 bytes
+// of one row (coalesced 256-byte row segments).  The splits' FP64 partial
+// tiles are summed in split order by a second kernel, so the products are
+// bit-for-bit repeatable.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -45,6 +47,11 @@ template <> struct Gload<double> {
         v2d v = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p));
         o[0] = v.x; o[1] = v.y;
     }
+    __device__ static inline void from_lds(const double *p, double *o)
+    {
+        v2d v = *reinterpret_cast<const v2d *>(p);
+        o[0] = v.x; o[1] = v.y;
+    }
 };
 template <> struct Gload<float> {
     static constexpr int E = 4;
@@ -54,19 +61,31 @@ template <> struct Gload<float> {
         v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
         o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
     }
+    __device__ static inline void from_lds(const float *p, float *o)
+    {
+        v4f v = *reinterpret_cast<const v4f *>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    }
 };
 template <> struct Gload<__hip_bfloat16> {
     static constexpr int E = 8;
     typedef float In;
-    __device__ static inline void run(const __hip_bfloat16 *p, float *o)
+    __device__ static inline void widen(v4u v, float *o)
     {
-        v4u v = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(p));
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             o[2 * q] = __uint_as_float(w[q] << 16);
             o[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
         }
+    }
+    __device__ static inline void run(const __hip_bfloat16 *p, float *o)
+    {
+        widen(__builtin_nontemporal_load(reinterpret_cast<const v4u *>(p)), o);
+    }
+    __device__ static inline void from_lds(const __hip_bfloat16 *p, float *o)
+    {
+        widen(*reinterpret_cast<const v4u *>(p), o);
     }
 };
 
@@ -87,65 +106,124 @@ template <typename In> struct Acc;
 template <> struct Acc<double> { typedef v4d V; };
 template <> struct Acc<float> { typedef v4f V; };
 
-// Y tile: rows rt*16.., right-hand sides ct*16.., summation over columns
-// [split*kspl, min(+kspl, ld)) of G (kspl and ld multiples of 64).
-// X: the right-hand sides in MFMA input type, column c at X + c*ldx (zero
-// padded to ld rows and to whole tiles of 16 columns).
-// part[(split * ncol + c) * Nd + row] (ncol = 16 * column tiles).
-template <typename T>
+// Y tile through LDS: the workgroup's four waves share 16 rows of G; each
+// chunk of 256 columns is read with coalesced 16-byte loads (a wave covers
+// 1 KB of one row per instruction), held in registers while the previous
+// chunk's MFMAs run from LDS, and wave w multiplies columns 64w..64w+63 of
+// the chunk.  The four wave sums of each output element meet in LDS and
+// are added in wave order.  Rows are padded by 16 bytes in LDS so that the
+// 16 rows of an MFMA operand fall on different banks.  NCT tiles of 16
+// right-hand sides (tiles blockIdx.z * NCT ..) share each pass over G.
+template <typename T, int NCT>
 __global__ void __launch_bounds__(kBlk)
-gm_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
-          const typename Gload<T>::In *__restrict__ X, int64_t ldx, int64_t row_tiles, int ncol,
-          double *__restrict__ part)
+gm_lds_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
+              const typename Gload<T>::In *__restrict__ X, int64_t ldx, int ncol,
+              double *__restrict__ part)
 {
     typedef typename Gload<T>::In In;
-    constexpr int E = Gload<T>::E;
-    constexpr int U = 16 / E;                 // steps per iteration: 4 * E * U = 64 columns
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t rt = (int64_t)blockIdx.x * kWavesPerBlk + (threadIdx.x / kWave);
-    if (rt >= row_tiles) return;
-    const int split = blockIdx.y, ct = blockIdx.z;
+    constexpr int E = Gload<T>::E;              // elements per 16 bytes
+    constexpr int U = 16 / E;                   // 16-byte steps per wave quarter (64 columns)
+    constexpr int kChunk = 256;                 // columns per chunk
+    constexpr int CH = kChunk / E;              // 16-byte pieces per row and chunk
+    constexpr int PER = 16 * CH / kBlk;         // pieces per thread and chunk
+    constexpr int ROWB = kChunk * sizeof(T) + 16;   // padded LDS row, bytes
+    __shared__ __attribute__((aligned(16))) unsigned char tile[16 * ROWB];
+    __shared__ double red[kWavesPerBlk][16][16];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int64_t rt = blockIdx.x;
+    const int split = blockIdx.y, ct0 = blockIdx.z * NCT;
     const int r = lane & 15, kk = lane >> 4;
-    const int64_t row = rt * 16 + r < Nd ? rt * 16 + r : Nd - 1;   // clamp: result dropped
-    const T *g = G + row * ld;
-    const In *x = X + (int64_t)(ct * 16 + r) * ldx;
     const int64_t k0 = (int64_t)split * kspl;
     const int64_t k1 = k0 + kspl < ld ? k0 + kspl : ld;
-    typename Acc<In>::V acc;
-    double a64[4] = {0.0, 0.0, 0.0, 0.0};
+    const In *x[NCT];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = 0;
-    for (int64_t kb = k0 + E * kk; kb < k1; kb += 4 * E * U) {
-        In gv[U][E], xv[U][E];
+    for (int t = 0; t < NCT; ++t) x[t] = X + (int64_t)((ct0 + t) * 16 + r) * ldx;
+    // this thread's pieces: q = threadIdx.x + kBlk * j -> row q / CH, piece q % CH
+    // (k1 - k0 is a multiple of 64, not of the chunk: the last chunk of a
+    // range may be partial; pieces past k1 are not loaded and wave quarters
+    // past k1 skip their MFMAs)
+    const T *src[PER];
+    int dst[PER], off[PER];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            Gload<T>::run(g + kb + u * 4 * E, gv[u]);
+    for (int j = 0; j < PER; ++j) {
+        const int q = threadIdx.x + kBlk * j, row = q / CH, pc = q % CH;
+        const int64_t grow = rt * 16 + row < Nd ? rt * 16 + row : Nd - 1;   // clamp: dropped
+        src[j] = G + grow * ld + (int64_t)pc * E;
+        dst[j] = row * ROWB + pc * 16;
+        off[j] = pc * E;
+    }
+    v4u buf[PER];
+    typename Acc<In>::V acc[NCT];
+    double a64[NCT][4];
 #pragma unroll
-            for (int e = 0; e < E; ++e) xv[u][e] = x[kb + u * 4 * E + e];
+    for (int t = 0; t < NCT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[t][i] = 0;
+            a64[t][i] = 0.0;
         }
+    int64_t kc = k0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        buf[j] = v4u{0u, 0u, 0u, 0u};
+        if (kc + off[j] < k1)
+            buf[j] = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(src[j] + kc));
+    }
+    for (; kc < k1; kc += kChunk) {
+        __syncthreads();                        // previous chunk's LDS reads done
+#pragma unroll
+        for (int j = 0; j < PER; ++j) *reinterpret_cast<v4u *>(tile + dst[j]) = buf[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (kc + kChunk + off[j] < k1)
+                buf[j] = __builtin_nontemporal_load(
+                    reinterpret_cast<const v4u *>(src[j] + kc + kChunk));
+        }
+        const int cbase = 64 * wv;              // this wave's 64 columns of the chunk
+        if (kc + cbase >= k1) continue;         // uniform per wave
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+            const int col = cbase + u * 4 * E + E * kk;
+            In gv[E];
+            Gload<T>::from_lds(reinterpret_cast<const T *>(tile + r * ROWB) + col, gv);
 #pragma unroll
-            for (int e = 0; e < E; ++e) mfma(gv[u][e], xv[u][e], acc);
-            if (sizeof(In) == 4) {            // FP32 partial -> FP64 every 4E products
+            for (int t = 0; t < NCT; ++t) {
+                In xv[E];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    a64[i] += (double)acc[i];
-                    acc[i] = 0;
+                for (int e = 0; e < E; ++e) xv[e] = x[t][kc + col + e];
+#pragma unroll
+                for (int e = 0; e < E; ++e) mfma(gv[e], xv[e], acc[t]);
+                if (sizeof(In) == 4) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        a64[t][i] += (double)acc[t][i];
+                        acc[t][i] = 0;
+                    }
                 }
             }
         }
     }
     if (sizeof(In) == 8) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a64[i] = (double)acc[i];
-    }
-    const int c = ct * 16 + (lane & 15);
-    double *out = part + ((int64_t)split * ncol + c) * Nd;
+        for (int t = 0; t < NCT; ++t)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t rr = rt * 16 + out_row<In>(lane, i);
-        if (rr < Nd) out[rr] = a64[i];
+            for (int i = 0; i < 4; ++i) a64[t][i] = (double)acc[t][i];
+    }
+    const int orow = threadIdx.x / 16, ocol = threadIdx.x % 16;   // 256 threads = 16 x 16
+    const int64_t rr = rt * 16 + orow;
+#pragma unroll
+    for (int t = 0; t < NCT; ++t) {
+        __syncthreads();                        // red free again
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[wv][out_row<In>(lane, i)][lane & 15] = a64[t][i];
+        __syncthreads();
+        if (rr < Nd) {
+            double s = red[0][orow][ocol];
+#pragma unroll
+            for (int q = 1; q < kWavesPerBlk; ++q) s += red[q][orow][ocol];
+            part[((int64_t)split * ncol + (ct0 + t) * 16 + ocol) * Nd + rr] = s;
+        }
     }
 }
 
@@ -261,9 +339,18 @@ hipError_t mm_n(const void *G, int64_t Nd, int64_t ld, const void *X, int64_t ld
                 int splits, int64_t kspl, double *part, hipStream_t st)
 {
     const int64_t row_tiles = (Nd + 15) / 16;
-    dim3 grid(nblk(row_tiles, kWavesPerBlk), (unsigned)splits, (unsigned)(ncol / 16));
-    hipLaunchKernelGGL(gm_kernel<T>, grid, dim3(kBlk), 0, st, (const T *)G, Nd, ld, kspl,
-                       (const typename Gload<T>::In *)X, ldx, row_tiles, ncol, part);
+    const int ctiles = ncol / 16;
+    // right-hand-side tiles per pass over G: 4 when they divide evenly, else 2 or 1
+    const int nct = ctiles % 4 == 0 ? 4 : (ctiles % 2 == 0 ? 2 : 1);
+    dim3 grid((unsigned)row_tiles, (unsigned)splits, (unsigned)(ctiles / nct));
+    typedef typename Gload<T>::In In;
+#define GRA_GM(N)                                                                              \
+    hipLaunchKernelGGL((gm_lds_kernel<T, N>), grid, dim3(kBlk), 0, st, (const T *)G, Nd, ld,   \
+                       kspl, (const In *)X, ldx, ncol, part)
+    if (nct == 4) GRA_GM(4);
+    else if (nct == 2) GRA_GM(2);
+    else GRA_GM(1);
+#undef GRA_GM
     return hipGetLastError();
 }
 
