@@ -205,6 +205,8 @@ def run(args):
             continue
         print(f"[bench] {key}: {results[key]['iter_s']:.1f} CG iter/s, assembly "
               f"{results[key]['entries_per_s']:.3e} entries/s", file=sys.stderr, flush=True)
+        if not fused:
+            results[key]["mat_mul"] = mat_mul_rates(ctx, inv, store, gbytes, gn_ms, gt_ms)
 
     cpu = None
     if not args.no_cpu:
@@ -214,6 +216,36 @@ def run(args):
         results["_skipped"] = skipped
     results["_nodes"] = nodes_info
     return pb, nd, nm, results, cpu
+
+
+MFMA_PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 157.3}   # TFLOP/s of the MFMA flavour used
+MM_RHS = 16
+
+
+def mat_mul_rates(ctx, inv, store, gbytes, gn_ms, gt_ms):
+    """The matrix-core products with MM_RHS right-hand sides on the resident G
+    (HIP events around each launch, 3 launches each after one untimed):
+    G-byte rate, MFMA throughput against the flavour's peak, and the time per
+    vector against the single-vector products of the timed CG steps."""
+    from graada3d_amd.engine import TIME_MAT_MUL
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal((inv.Nm, MM_RHS))
+    R = rng.standard_normal((inv.Nd, MM_RHS))
+    out = {"rhs": MM_RHS, "mfma": "f64 16x16x4" if store == "f64" else "f32 16x16x4",
+           "peak_TFLOPs": MFMA_PEAK[store]}
+    for name, fn, arg, vec_ms in (("G_X", inv.G_mat_mul, X, gn_ms), ("GT_R", inv.GT_mat_mul, R, gt_ms)):
+        fn(arg)
+        ctx.timing(True)
+        ctx.timing_reset()
+        for _ in range(3):
+            fn(arg)
+        ms, _ = ctx.timing_get(TIME_MAT_MUL)
+        ctx.timing(False)
+        tf = 2.0 * inv.Nd * inv.Nm * MM_RHS / (ms * 1e-3) / 1e12
+        out[name] = {"avg_ms": ms, "G_GBps": gbytes / (ms * 1e-3) / 1e9, "TFLOPs": tf,
+                     "mfma_frac": tf / MFMA_PEAK[store],
+                     "per_vector_speedup_vs_gemv": vec_ms * MM_RHS / ms if vec_ms else None}
+    return out
 
 
 def all_cores(fn, unit_count, unit):
@@ -637,6 +669,7 @@ def main():
             mixed[s]["fused_avg_ms"] = r["fused_product"]["avg_ms"]
         else:
             mixed[s]["gemv_GBps"] = {"n": r["gemv_n"]["GBps"], "t": r["gemv_t"]["GBps"]}
+            mixed[s]["mat_mul"] = r.get("mat_mul")
     dom = "gemv_t" if ref["gemv_t"]["avg_ms"] >= ref["gemv_n"]["avg_ms"] else "gemv_n"
     achieved = ref[dom]["GBps"]
     # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs
@@ -683,6 +716,7 @@ def main():
                                     "peak": 78.6, "unit": "TFLOP/s",
                                     "frac": ref["entries_per_s"] * FLOP_PER_ENTRY[small] / 78.6e12}},
         "g_assembly_nodes": nodes_info,
+        "mat_mul": ref.get("mat_mul"),
         "final_misfit": ref["misfit"],
         "mixed": mixed,
         "mixed_not_run": {k: "fused grid not co-resident for this G (two-pass iteration ran)"

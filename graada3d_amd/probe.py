@@ -20,6 +20,13 @@ grid (1.44e7 entries), the same two G by corner sums
 
     rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 \
         SQ_INSTS_VALU_TRANS_F64 -d <dir> -o asm -- python3 -m graada3d_amd.probe --assemble
+
+With ``--mat-mul`` it runs the matrix-core products with 16 right-hand sides
+(gra_G_mat_mul, gra_GT_mat_mul; 2 launches each) on the same 5,000 x 32,000 G
+stored in FP64, FP32 and BF16, for MFMA counters:
+
+    rocprofv3 --pmc SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE -d <dir> -o mm \
+        -- python3 -m graada3d_amd.probe --mat-mul
 """
 from __future__ import annotations
 
@@ -49,10 +56,26 @@ def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--all-dtypes", action="store_true")
     ap.add_argument("--assemble", action="store_true")
+    ap.add_argument("--mat-mul", action="store_true")
     args = ap.parse_args(argv)
     from .engine import Context, Fwd, forward_rectilinear
     from .workloads import forward_problem_small
     from . import mesh as M
+
+    if args.mat_mul:
+        ctx = Context(0)
+        fp = forward_problem_small()
+        f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
+        rng = np.random.default_rng(0)
+        X, R = rng.standard_normal((f.Nm, 16)), rng.standard_normal((f.Nd, 16))
+        for store in ("f64", "f32", "bf16"):
+            f.compute_G(store)
+            for _ in range(2):
+                f.G_mat_mul(X)
+                f.GT_mat_mul(R)
+            print(f"[probe] {store}: G X and G^T R with 16 right-hand sides", flush=True)
+        ctx.close()
+        return
 
     if args.assemble:
         ctx = Context(0)
