@@ -33,9 +33,10 @@ configuration, its data and model files, and the adaptive Gauss-Newton loop.
   (set_depth_weighting flag 0).
 
 Supported subset: methods 1 (own CG on a stored G) and 2 (own CG on the
-wavelet-compressed G); no padding; no cross-gradient or petrophysical model
-files (the reference interpolates them from gridded files with Linterp);
-anything else raises ValueError.
+wavelet-compressed G), with or without padding cells (the level-0 mesh is
+then the tensor product of the padded node coordinates); no cross-gradient
+or petrophysical model files (the reference interpolates them from gridded
+files with Linterp); anything else raises ValueError.
 """
 from __future__ import annotations
 
@@ -184,6 +185,21 @@ def data_weights(dobs, n_obs, rel, abs_):
     return w
 
 
+def build_mesh(model: ModelConfig):
+    """The initial inversion mesh (GraAda3D.cpp:456-475): regular, or with
+    padding cells around the region of interest when any padding count is
+    non-zero (Mesh::generate_regular_mesh_with_padding, Mesh.cpp:516-590)."""
+    from .mesh import padded_nodes
+    from .octree import OctreeMesh
+    npx, fx, npy, fy, npz, fz = model.padding
+    if int(npx) == 0 and int(npy) == 0 and int(npz) == 0:
+        return OctreeMesh(model.x, model.n[0], model.y, model.n[1], model.z, model.n[2])
+    nodes = (padded_nodes(model.x, model.n[0], int(npx), fx),
+             padded_nodes(model.y, model.n[1], int(npy), fy),
+             padded_nodes(model.z, model.n[2], int(npz), fz, both_sides=False))
+    return OctreeMesh(None, 0, None, 0, None, 0, nodes=nodes)
+
+
 class AdaptiveInversion:
     """invert_with_own_CG with adaptive refinement on the engine.  Logs go to
     `out_dir` as lambda_misfit_GN and Iteration_misfit_GN."""
@@ -195,8 +211,6 @@ class AdaptiveInversion:
         if inv.method not in (1, 2):
             raise ValueError("method must be 1 (stored G) or 2 (wavelet-compressed G); "
                              "method 0 (Eigen's LSCG) is not supported")
-        if any(model.padding[k] for k in (0, 2, 4)):
-            raise ValueError("padding cells are not supported")
         if inv.use_crg or inv.use_petro:
             raise ValueError("cross-gradient / petrophysical model files are not supported")
         self.O = O
@@ -205,7 +219,7 @@ class AdaptiveInversion:
         self.points, self.dobs = read_data(data)
         self.n_obs = self.points.shape[0]
         self.Wd = data_weights(self.dobs, self.n_obs, data.rel_err, data.abs_err)
-        self.mesh = OctreeMesh(model.x, model.n[0], model.y, model.n[1], model.z, model.n[2])
+        self.mesh = build_mesh(model)
         if engine_factory is None:
             from .engine import Context, GaussNewtonInversion
             ctx = ctx or Context(0)

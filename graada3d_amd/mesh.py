@@ -42,6 +42,25 @@ def regular_mesh(x_lim, nx, y_lim, ny, z_lim, nz) -> np.ndarray:
     return np.stack([xs[i], xs[i + 1], ys[j], ys[j + 1], zs[k], zs[k + 1]], axis=1)
 
 
+def padded_nodes(lim, n: int, n_pad: int, factor: float, both_sides: bool = True) -> np.ndarray:
+    """Node coordinates of one axis of Mesh::generate_regular_mesh_with_padding
+    (Mesh.cpp:541-577): n equal cells over lim (Eigen LinSpaced), then n_pad
+    cells on each side (x, y) or below (z: both_sides=False) whose sizes grow
+    by `factor` per cell from the inner cell size, in the reference's order
+    of additions."""
+    inner = linspaced(n + 1, *lim)
+    lead = n_pad if both_sides else 0
+    out = np.empty(n + 1 + lead + n_pad)
+    out[lead:lead + n + 1] = inner
+    d = inner[1] - inner[0]
+    for i in range(n_pad):
+        d *= factor
+        out[lead + n + 1 + i] = out[lead + n + i] + d
+        if both_sides:
+            out[lead - 1 - i] = out[lead - i] - d
+    return out
+
+
 def rectilinear_axes(prisms: np.ndarray):
     """(xs, ys, zs) if the prisms are exactly the tensor-product cells of those
     node coordinates in the order of ``regular_mesh`` (cell id = i*ny*nz +

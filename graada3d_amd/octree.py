@@ -48,9 +48,18 @@ class _Leaf:
 
 
 class OctreeMesh:
-    def __init__(self, x_lim, nx, y_lim, ny, z_lim, nz):
+    def __init__(self, x_lim, nx, y_lim, ny, z_lim, nz, nodes=None):
+        """Level-0 cells: nx x ny x nz equal cells over the limits, or, with
+        `nodes` = (xs, ys, zs), the tensor product of those node coordinates
+        (Mesh::generate_regular_mesh over given grid points, Mesh.cpp:828-846,
+        as the padded mesh builds it)."""
+        if nodes is not None:
+            xs, ys, zs = (np.asarray(a, dtype=np.float64) for a in nodes)
+            nx, ny, nz = xs.size - 1, ys.size - 1, zs.size - 1
+        else:
+            xs, ys, zs = (linspaced(nx + 1, *x_lim), linspaced(ny + 1, *y_lim),
+                          linspaced(nz + 1, *z_lim))
         self.base = (nx, ny, nz)
-        xs, ys, zs = linspaced(nx + 1, *x_lim), linspaced(ny + 1, *y_lim), linspaced(nz + 1, *z_lim)
         self.leaves = []
         for i in range(nx):
             for j in range(ny):

@@ -61,12 +61,45 @@ def test_read_data_sorts_components(tmp_path):
     np.testing.assert_allclose(wd[3:], 1 / (0.05 * tzz + 1.0), rtol=1e-15)
 
 
+def test_padded_nodes_follow_the_reference_construction():
+    """Mesh::generate_regular_mesh_with_padding (Mesh.cpp:541-577): the inner
+    nodes are LinSpaced, padding cells grow by the factor from the inner size
+    outwards, on both sides in x / y and below only in z."""
+    from graada3d_amd.mesh import linspaced, padded_nodes
+    x = padded_nodes((0.0, 2000.0), 10, 3, 1.5)
+    np.testing.assert_array_equal(x[3:14], linspaced(11, 0.0, 2000.0))
+    np.testing.assert_allclose(x[14:], [2300.0, 2750.0, 3425.0], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(x[:3], [-1425.0, -750.0, -300.0], rtol=0, atol=1e-9)
+    z = padded_nodes((0.0, 1000.0), 5, 2, 2.0, both_sides=False)
+    np.testing.assert_array_equal(z[:6], linspaced(6, 0.0, 1000.0))
+    np.testing.assert_allclose(z[6:], [1400.0, 2200.0], rtol=0, atol=1e-9)
+    assert np.array_equal(padded_nodes((0.0, 10.0), 4, 0, 3.0), linspaced(5, 0.0, 10.0))
+
+
+def test_padding_mesh_in_the_driver(tmp_path):
+    """A padded configuration: the initial mesh is the tensor product of the
+    padded nodes (in regular-mesh order) and the Gauss-Newton loop runs on it."""
+    from graada3d_amd import mesh as M
+    from graada3d_amd.mesh import padded_nodes
+    cfg, _ = stage_karst("raw", tmp_path)
+    d, m, inv = driver.read_config(cfg)
+    mp = replace(m, padding=(2, 1.5, 1, 2.0, 2, 1.3))
+    out = tmp_path / "out"
+    out.mkdir()
+    run = driver.AdaptiveInversion(d, mp, inv, out_dir=str(out), engine_factory=OracleGN)
+    axes = M.rectilinear_axes(run.mesh.prisms())
+    assert axes is not None
+    np.testing.assert_array_equal(axes[0], padded_nodes(m.x, m.n[0], 2, 1.5))
+    np.testing.assert_array_equal(axes[1], padded_nodes(m.y, m.n[1], 1, 2.0))
+    np.testing.assert_array_equal(axes[2], padded_nodes(m.z, m.n[2], 2, 1.3, both_sides=False))
+    assert run.mesh.n_cells == (m.n[0] + 4) * (m.n[1] + 2) * (m.n[2] + 2)
+    res = run.run(max_trials=2)
+    assert np.isfinite(res["misfit"]) and len(run.trials) == 2
+
+
 def test_unsupported_options(tmp_path):
     cfg, _ = stage_karst("raw", tmp_path)
     d, m, inv = driver.read_config(cfg)
-    with pytest.raises(ValueError, match="padding"):
-        driver.AdaptiveInversion(d, replace(m, padding=(2, 1.5, 0, 0, 0, 0)), inv,
-                                 engine_factory=OracleGN)
     bad = replace(inv, method=0)
     with pytest.raises(ValueError, match="method"):
         driver.AdaptiveInversion(d, m, bad, engine_factory=OracleGN)
