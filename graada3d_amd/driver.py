@@ -32,11 +32,20 @@ configuration, its data and model files, and the adaptive Gauss-Newton loop.
   (InversionBase::set_dobs); depth weighting |mean(z_obs) - z_c|^(-beta/2)
   (set_depth_weighting flag 0).
 
+* Reference and structural models (GaussNewtonInversion.cpp:657-692,
+  :752-766, :1131-1150, :1262-1281; AdaptiveInversion.cpp:206-536): a
+  petrophysical model file sets m0 at the cell centres by multilinear
+  interpolation (graada3d_amd.gridded) and the start model to m0 moved 0.01
+  inside (m_min, m_max); a cross-gradient model file sets m0_s the same way,
+  and with it the T_x / T_y / T_z operators (InversionBase::set_Tmatrix), the
+  weight a_crg, S_crg = I, and the IRLS weights WL_t from T m when p_t != 2.
+  After a refinement the children take interpolated m0 / m0_s; with both
+  files a cell at the refinement threshold (within 1e-6) is refined too.
+
 Supported subset: methods 1 (own CG on a stored G) and 2 (own CG on the
 wavelet-compressed G), with or without padding cells (the level-0 mesh is
-then the tensor product of the padded node coordinates); no cross-gradient
-or petrophysical model files (the reference interpolates them from gridded
-files with Linterp); anything else raises ValueError.
+then the tensor product of the padded node coordinates), with or without
+reference / cross-gradient model files; method 0 raises ValueError.
 """
 from __future__ import annotations
 
@@ -160,9 +169,15 @@ def read_config(path: str):
         m_min=float(mm[0]), m_max=float(mm[1]), m_ini=float(mm[2]),
         max_refinements=int(v[11][0]), min_size=tuple(float(x) for x in v[12][:3]),
         refine_interval=int(v[13][0]), refine_fraction=float(v[14][0]), record=int(v[15][0]),
-        use_crg=int(v[16][0]), crg_model=v[17], use_petro=int(v[18][0]), petro_model=v[19],
+        use_crg=int(v[16][0]), crg_model=_model_line(v[17], res),
+        use_petro=int(v[18][0]), petro_model=_model_line(v[19], res),
         output=v[20][0] if len(v) > 20 else "result")
     return data, model, inv
+
+
+def _model_line(tokens, res):
+    """A model line with its file name resolved like the other files."""
+    return [res(tokens[0])] + list(tokens[1:]) if tokens and tokens[0] != "none" else list(tokens)
 
 
 def read_data(cfg: DataConfig):
@@ -211,8 +226,9 @@ class AdaptiveInversion:
         if inv.method not in (1, 2):
             raise ValueError("method must be 1 (stored G) or 2 (wavelet-compressed G); "
                              "method 0 (Eigen's LSCG) is not supported")
-        if inv.use_crg or inv.use_petro:
-            raise ValueError("cross-gradient / petrophysical model files are not supported")
+        from .gridded import model_from_config
+        self.interp_m0 = model_from_config(inv.petro_model) if inv.use_petro else None
+        self.interp_m0s = model_from_config(inv.crg_model) if inv.use_crg else None
         self.O = O
         self.cfg, self.dcfg, self.model_cfg = inv, data, model
         self.out_dir = out_dir
@@ -231,8 +247,14 @@ class AdaptiveInversion:
         nm = self.mesh.n_cells
         self.m_min = np.full(nm, inv.m_min)
         self.m_max = np.full(nm, inv.m_max)
-        self.m0 = np.zeros(nm)
-        self.m = np.full(nm, inv.m_ini)
+        prisms = self.mesh.prisms()
+        self.m0 = self.interp_m0.at_centres(prisms) if inv.use_petro else np.zeros(nm)
+        self.m0_s = self.interp_m0s.at_centres(prisms) if inv.use_crg else None
+        # the start model is m0 when a reference model is given (GN.cpp:752-766)
+        self.m = self.m0.copy() if inv.use_petro else np.full(nm, inv.m_ini)
+        if inv.use_crg:
+            self.engine.use_cross_gradient_constraint = True
+            self.engine.a_crg = inv.a[4]
         self._build_operators()
 
     def _assemble(self):
@@ -252,7 +274,12 @@ class AdaptiveInversion:
         Z = O.depth_weights_reference(prisms, self.points, c.depth_exponent)
         a_s, a_z, a_x, a_y = c.a[:4]
         self.W = O.weighting_operators_from(prisms, D, Z, a_s=a_s, a_x=a_x, a_y=a_y, a_z=a_z)
-        self.engine.set_regularization(*self.W)
+        self.T = ()
+        if c.use_crg:
+            # InversionBase::set_Tmatrix; S_crg = I (InversionBase.cpp:113-130)
+            self.T = O.cross_gradient_operators_from(prisms, D, self.m0_s)
+            self.engine.S_crg = np.ones(prisms.shape[0])
+        self.engine.set_regularization(*self.W, *self.T)
 
     def _irls(self, m_opt):
         """WL_s from Ws (m - m0), WL_x/y/z from Wx/Wy/Wz m where the exponent
@@ -270,10 +297,15 @@ class AdaptiveInversion:
             WL["y"] = e.irls_weights(Wy, m_opt, p_y, c.eps2)
         if p_z != 2:
             WL["z"] = e.irls_weights(Wz, m_opt, p_z, c.eps2)
+        if c.use_crg and c.p[4] != 2:
+            # GN.cpp:1010-1021, :1266-1280
+            for k, T in zip(("tx", "ty", "tz"), self.T):
+                WL[k] = e.irls_weights(T, m_opt, c.p[4], c.eps2)
         return WL
 
     def _refine(self):
-        """AdaptiveInversion::refine_mesh(a) (AdaptiveInversion.cpp:637-727)."""
+        """AdaptiveInversion::refine_mesh (AdaptiveInversion.cpp:637-727; with
+        model files :206-388 and :389-536)."""
         c = self.cfg
         nm = self.mesh.n_cells
         a = c.refine_fraction
@@ -286,7 +318,10 @@ class AdaptiveInversion:
         small = np.zeros(nm, bool)
         for k in range(3):
             small |= (d[:, k] < c.min_size[k]) | (np.abs(d[:, k] - c.min_size[k]) < 1e-7)
-        marked = np.flatnonzero((ind > tol) & ~small)
+        hit = ind > tol
+        if c.use_crg and c.use_petro:
+            hit |= np.abs(ind - tol) < 1e-6
+        marked = np.flatnonzero(hit & ~small)
         if marked.size == 0:
             return 0
         split, kids = self.mesh.refine(marked)
@@ -302,6 +337,14 @@ class AdaptiveInversion:
         reps[split] = 8
         self.m, self.m0, self.m_min, self.m_max = (np.repeat(v, reps) for v in
                                                    (self.m, self.m0, self.m_min, self.m_max))
+        # children of a split cell take the model files' values at their centres
+        kid_rows = np.flatnonzero(np.repeat(reps == 8, reps))
+        kid_prisms = self.mesh.prisms()[kid_rows]
+        if c.use_petro:
+            self.m0[kid_rows] = self.interp_m0.at_centres(kid_prisms)
+        if c.use_crg:
+            self.m0_s = np.repeat(self.m0_s, reps)
+            self.m0_s[kid_rows] = self.interp_m0s.at_centres(kid_prisms)
         self._build_operators()
         return int(split.size)
 
@@ -316,10 +359,16 @@ class AdaptiveInversion:
         lam_log.write(f"{'lambda':>25}{'misfit':>25}\n")
         it_log.write(f"{'Iteranation number':>25}{'number of tried regulartion parameters':>25}"
                      f"{'misfit':>25}\n")
-        # start model rule (GN.cpp:769-780)
-        hit = (self.m > self.m_max) | (np.abs(self.m - self.m_max) < 1e-7) | \
-              (self.m < self.m_min) | (np.abs(self.m - self.m_min) < 1e-7)
-        self.m = np.where(hit, 0.5 * (self.m_max + self.m_min), self.m)
+        # start model rule (GN.cpp:752-780)
+        hi = (self.m > self.m_max) | (np.abs(self.m - self.m_max) < 1e-7)
+        if c.use_petro:
+            self.m = np.where(hi, self.m_max - 0.01, self.m)
+            lo = (self.m < self.m_min) | (np.abs(self.m - self.m_min) < 1e-7)
+            self.m = np.where(lo, self.m_min + 0.01, self.m)
+        else:
+            self.m = np.where(hi, 0.5 * (self.m_max + self.m_min), self.m)
+            lo = (self.m < self.m_min) | (np.abs(self.m - self.m_min) < 1e-7)
+            self.m = np.where(lo, 0.5 * (self.m_max + self.m_min), self.m)
         misfit = e.misfit(self.m, self.Wd, self.dobs)
         lam = c.lambda_max
         lambda_opt, misfit_opt, m_opt = lam, misfit, self.m.copy()

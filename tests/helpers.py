@@ -92,13 +92,18 @@ class OracleGN:
 
     def set_regularization(self, Ws, Wx, Wy, Wz, Tx=None, Ty=None, Tz=None):
         self.W = (Ws, Wx, Wy, Wz)
+        self.T = None if Tx is None else (Tx, Ty, Tz)
 
     def solve_cg(self, tol, maxit, lam, mk, mref, Wd, dobs, P, WL_s=None, WL_x=None, WL_y=None,
-                 WL_z=None, quirks=True, **_):
+                 WL_z=None, WL_tx=None, WL_ty=None, WL_tz=None, quirks=True, **_):
         import oracle
-        WL = {k: v for k, v in dict(s=WL_s, x=WL_x, y=WL_y, z=WL_z).items() if v is not None}
+        WL = {k: v for k, v in dict(s=WL_s, x=WL_x, y=WL_y, z=WL_z, tx=WL_tx, ty=WL_ty,
+                                    tz=WL_tz).items() if v is not None}
+        crg = None
+        if getattr(self, "use_cross_gradient_constraint", False):
+            crg = dict(Tx=self.T[0], Ty=self.T[1], Tz=self.T[2], a=self.a_crg, S=self.S_crg)
         return oracle.solve_cg(self.G, tol, maxit, lam, mk, mref, Wd, dobs, P, *self.W, WL=WL,
-                               quirks=quirks)
+                               crg=crg, quirks=quirks)
 
     def barrier_P(self, m, m_min, m_max):
         import oracle

@@ -53,3 +53,38 @@ def test_driver_method2_on_gpu(ctx, tmp_path):
           f"({75 * n_dy / nnz:.1f}x), misfit {res['misfit']:.6g}, Nm {res['Nm']}", flush=True)
     assert nnz < 75 * n_dy
     assert res["misfit"] < 1.0
+
+
+def test_driver_with_model_files_vs_oracle(ctx, tmp_path):
+    """Reference density model and cross-gradient structural model (both read
+    from gridded files, a_crg = 1e-3, p_t = 1, refinement after every
+    Gauss-Newton iteration): the GPU engine's run against the same driver on
+    the oracle -- the same lambdas, tried-lambda counts and refined meshes,
+    trial misfits within 1e-3, final misfit within 1e-6 and final model within
+    1e-4 (relative 2-norm)."""
+    import numpy as np
+    from helpers import OracleGN, rel_norm
+    from test_gridded import model_file_config
+    d, m, inv, _, _ = model_file_config(tmp_path)
+    runs = []
+    for name, kw in (("gpu", dict(ctx=ctx)), ("oracle", dict(engine_factory=OracleGN))):
+        out = tmp_path / name
+        out.mkdir()
+        print(f"model-file inversion on the {name}", flush=True)
+        run = driver.AdaptiveInversion(d, m, inv, out_dir=str(out), **kw)
+        res = run.run()
+        runs.append((run, res))
+    (g, rg), (o, ro) = runs
+    print(f"Nm {rg['Nm']} / {ro['Nm']}, misfit {rg['misfit']:.12g} / {ro['misfit']:.12g}, "
+          f"model {rel_norm(g.m, o.m):.3g}", flush=True)
+    assert rg["Nm"] == ro["Nm"] and rg["Nm"] > 2496
+    assert [h["lambdas"] for h in rg["history"]] == [h["lambdas"] for h in ro["history"]]
+    lg = np.array([t["lam"] for t in g.trials])
+    lo = np.array([t["lam"] for t in o.trials])
+    np.testing.assert_allclose(lg, lo, rtol=1e-13)
+    np.testing.assert_allclose([t["misfit"] for t in g.trials],
+                               [t["misfit"] for t in o.trials], rtol=1e-3)
+    np.testing.assert_array_equal(g.mesh.prisms(), o.mesh.prisms())
+    np.testing.assert_array_equal(g.m0_s, o.m0_s)
+    assert abs(rg["misfit"] - ro["misfit"]) <= 1e-6 * ro["misfit"]
+    assert rel_norm(g.m, o.m) < 1e-4
