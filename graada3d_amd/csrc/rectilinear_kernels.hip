@@ -313,10 +313,15 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
             if (flag & (1u << bit)) vals[q++ * nmax + n] = corner_component(bit, x, y, z, ct);
     }
     __syncthreads();
-    const int nzc = (int)nz, ncell = cx * cy * nzc;
-    Walk3 wc(threadIdx.x, cy, nzc);
-    for (int c = threadIdx.x; c < ncell; c += kBlk, wc.next()) {
-        const int a = wc.a, b = wc.b, k = wc.k;
+    // A thread takes the cells k and k+1 of a column: with
+    // e(l) = (x-difference at y1) - (x-difference at y0) of level l, a cell's
+    // sum is e(k+1) - e(k), so the shared level is read once (12 LDS reads
+    // for two cells instead of 16; the same operations as cell by cell).
+    const int nzc = (int)nz, npair = (nzc + 1) >> 1, nwork = cx * cy * npair;
+    Walk3 wc(threadIdx.x, cy, npair);
+    for (int c = threadIdx.x; c < nwork; c += kBlk, wc.next()) {
+        const int a = wc.a, b = wc.b, k = 2 * wc.k;
+        const bool second = k + 1 < nzc;          // level k + 2 <= nz exists
         const int n0 = a * plane + b * NZ + k;
         const int64_t j = ((i0 + a) * ny + (j0 + b)) * nz + k;
         int q = 0;
@@ -324,12 +329,14 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
         for (int bit = 1; bit < 10; ++bit) {
             if (!(flag & (1u << bit))) continue;
             const double *v = vals + q * nmax + n0;
-            const double d00 = v[plane] - v[0];
-            const double d10 = v[plane + NZ] - v[NZ];
-            const double d01 = v[plane + 1] - v[1];
-            const double d11 = v[plane + NZ + 1] - v[NZ + 1];
-            const double s = (d11 - d01) - (d10 - d00);
-            G[((int64_t)q * N_obs + ip) * ld + j] = store_as<T>(comp_scale(bit) * s);
+            const double e0 = (v[plane + NZ] - v[NZ]) - (v[plane] - v[0]);
+            const double e1 = (v[plane + NZ + 1] - v[NZ + 1]) - (v[plane + 1] - v[1]);
+            T *g = G + ((int64_t)q * N_obs + ip) * ld + j;
+            g[0] = store_as<T>(comp_scale(bit) * (e1 - e0));
+            if (second) {
+                const double e2 = (v[plane + NZ + 2] - v[NZ + 2]) - (v[plane + 2] - v[2]);
+                g[1] = store_as<T>(comp_scale(bit) * (e2 - e1));
+            }
             ++q;
         }
     }
