@@ -26,7 +26,7 @@ EXPORTS = [
     "gra_status_string", "gra_last_error", "gra_ctx_create", "gra_ctx_destroy", "gra_sync",
     "gra_assemble", "gra_get_dims", "gra_copy_G", "gra_G_vec_mul", "gra_GT_vec_mul",
     "gra_col_norms", "gra_expand_cols", "gra_assemble_batch", "gra_assemble_wavelet",
-    "gra_wavelet_info", "gra_assemble_rectilinear", "gra_G_mat_mul", "gra_GT_mat_mul",
+    "gra_wavelet_info", "gra_assemble_rectilinear", "gra_assemble_corners", "gra_G_mat_mul", "gra_GT_mat_mul",
     "gra_forward", "gra_forward_rectilinear", "gra_set_regularization", "gra_solve_cg", "gra_barrier_P",
     "gra_barrier_update", "gra_misfit", "gra_irls_weights", "gra_gn_setup_problem",
     "gra_gn_step", "gra_gn_get_model", "gra_timing_enable", "gra_timing_reset",
@@ -96,6 +96,8 @@ def load() -> ctypes.CDLL:
         "gra_assemble_rectilinear": ([vp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64, c_dp,
                                       ctypes.c_int64, c_dp, ctypes.c_int64, ctypes.c_uint32,
                                       ctypes.c_int], ctypes.c_int),
+        "gra_assemble_corners": ([vp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64,
+                                  ctypes.c_uint32, ctypes.c_int], ctypes.c_int),
         "gra_get_dims": ([vp, c_i64p, c_i64p, c_i64p, ctypes.POINTER(ctypes.c_uint32),
                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "gra_copy_G": ([vp, c_dp, ctypes.c_int], ctypes.c_int),

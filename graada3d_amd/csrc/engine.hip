@@ -366,6 +366,51 @@ gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, 
     return GRA_OK;
 }
 
+gra_status gra_assemble_corners(gra_ctx *ctx, const double *prisms, int64_t Nm, const double *pts,
+                                int64_t N_obs, uint32_t field_flag, gra_dtype store)
+{
+    CTX_GUARD(ctx);
+    if (!prisms || !pts || Nm <= 0 || N_obs <= 0)
+        return set_error(c, GRA_BAD_ARG, "empty mesh or point set");
+    if ((field_flag & 1u) || (field_flag & ~0x3FFu) || !(field_flag & 0x3FEu))
+        return set_error(c, GRA_BAD_ARG, "field_flag must select components among bits 1..9");
+    if (store != GRA_F64 && store != GRA_F32 && store != GRA_BF16)
+        return set_error(c, GRA_BAD_ARG, "unknown storage type");
+    if (Nm > INT32_MAX) return set_error(c, GRA_BAD_ARG, "Nm exceeds int32 column indices");
+    if (!prisms_valid(prisms, Nm))
+        return set_error(c, GRA_BAD_ARG, "prism bounds must be finite with x0<x1, y0<y1, z0<z1");
+    if (!points_valid(pts, N_obs)) return set_error(c, GRA_BAD_ARG, "non-finite point");
+    double top = prisms[4];
+    for (int64_t j = 1; j < Nm; ++j) top = std::min(top, prisms[6 * j + 4]);
+    for (int64_t i = 0; i < N_obs; ++i)
+        if (!(pts[3 * i + 2] < top))
+            return set_error(c, GRA_BAD_ARG,
+                             "points must lie strictly above every prism (z < min z0, z down)");
+    const int nc = ncomp(field_flag);
+    const int64_t Nd = (int64_t)nc * N_obs;
+    const int64_t ld = (Nm + 63) / 64 * 64;
+    c.Nd = c.Nm = c.N_obs = c.ld = 0;          // G is invalid until the kernel has run
+    c.gn_ready = false;
+    c.wav = false;
+    c.partn_holds_gn_model = false;
+    HIPCHK(c.G.reserve(dtype_size(store) * (size_t)Nd * (size_t)ld), "allocate G");
+    if (ld > Nm)
+        HIPCHK(hipMemsetAsync(c.G.p, 0, dtype_size(store) * (size_t)Nd * (size_t)ld, c.stream),
+               "clear G padding");
+    DevBuf dpr;
+    HIPCHK(put(dpr, prisms, (size_t)(6 * Nm), c.stream), "upload prisms");
+    HIPCHK(put(c.pts_dev, pts, (size_t)(3 * N_obs), c.stream), "upload points");
+    {
+        TimedRegion tr(c, kTimeAssemble);
+        HIPCHK(launch_cell_corner(dpr.as<double>(), Nm, c.pts_dev.as<double>(), N_obs, field_flag,
+                                  store, c.G.p, ld, c.stream),
+               "per-cell corner assembly");
+    }
+    HIPCHK(hipStreamSynchronize(c.stream), "assemble");
+    c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    return GRA_OK;
+}
+
 gra_status gra_get_dims(const gra_ctx *ctx, int64_t *Nd, int64_t *Nm, int64_t *N_obs,
                         uint32_t *field_flag, int *store)
 {

@@ -215,6 +215,8 @@ hipError_t launch_sum_splits(const double *part, const MatMulPlan &p, int64_t n,
 // G of a rectilinear mesh by corner sums (rectilinear_kernels.hip): tile =
 // cell columns per tile side; its corner values take
 // 8 * ncomp * (tile+1)^2 * (nz+1) bytes of LDS.
+hipError_t launch_cell_corner(const double *prisms, int64_t Nm, const double *pts, int64_t N_obs,
+                              uint32_t flag, gra_dtype store, void *G, int64_t ld, hipStream_t st);
 hipError_t launch_node_assemble(const double *xs, const double *ys, const double *zs, int64_t nx,
                                 int64_t ny, int64_t nz, const double *pts, int64_t N_obs,
                                 uint32_t flag, gra_dtype store, void *G, int64_t ld, int tile,

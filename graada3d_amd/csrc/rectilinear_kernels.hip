@@ -342,6 +342,71 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
     }
 }
 
+// Corner sums for any prisms (cell_corner_kernel): a thread takes one cell
+// for one point and evaluates the cell's eight corners itself (no sharing
+// between cells: octree and other non-tensor meshes), combined in the
+// difference order of node_assemble_kernel -- x differences first, then y,
+// then z -- so a cell of a tensor-product mesh gets the same value both ways.
+// Threads take consecutive cells of one point: consecutive G columns.
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+cell_corner_kernel(const double *__restrict__ prisms, int64_t Nm, const double *__restrict__ pts,
+                   int64_t N_obs, int64_t p0, uint32_t flag, T *__restrict__ G, int64_t ld)
+{
+    __shared__ double lg_G[kLogTable], lg_T[kLogTable];
+    for (int q = threadIdx.x; q < kLogTable; q += kBlk) {
+        lg_G[q] = kLogG[q];
+        lg_T[q] = kLogT[q];
+    }
+    __syncthreads();
+    const int64_t ip = p0 + blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (ip >= N_obs || j >= Nm) return;
+    const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
+    const double *pr = prisms + 6 * j;
+    const double xv[2] = {pr[0] - px, pr[1] - px};
+    const double yv[2] = {pr[2] - py, pr[3] - py};
+    const double zv[2] = {pr[4] - pz, pr[5] - pz};
+    const NodeTerms nt(flag);
+    double dlo[9], e0[9];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+            // x difference F(x1, y_bb, z_kk) - F(x0, y_bb, z_kk) per component
+            double d[9];
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa) {
+                const double x = xv[aa], y = yv[bb], z = zv[kk];
+                const double xy2 = fma(x, x, y * y);
+                const bool axis = (x == 0.0) && (y == 0.0);
+                const double half_pi_s = copysign(1.5707963267948966, x) * copysign(1.0, y);
+                const CornerTerms ct = corner_terms(x, y, z, xy2, axis, half_pi_s, nt, lg_G, lg_T);
+#pragma unroll
+                for (int bit = 1; bit < 10; ++bit) {
+                    if (!(flag & (1u << bit))) continue;
+                    const double v = corner_component(bit, x, y, z, ct);
+                    d[bit - 1] = aa ? v - d[bit - 1] : v;
+                }
+            }
+#pragma unroll
+            for (int bit = 1; bit < 10; ++bit) {
+                if (!(flag & (1u << bit))) continue;
+                if (bb == 0) dlo[bit - 1] = d[bit - 1];
+                else if (kk == 0) e0[bit - 1] = d[bit - 1] - dlo[bit - 1];
+                else dlo[bit - 1] = (d[bit - 1] - dlo[bit - 1]) - e0[bit - 1];
+            }
+        }
+    }
+    int q = 0;
+#pragma unroll
+    for (int bit = 1; bit < 10; ++bit) {
+        if (!(flag & (1u << bit))) continue;
+        G[((int64_t)q * N_obs + ip) * ld + j] = store_as<T>(comp_scale(bit) * dlo[bit - 1]);
+        ++q;
+    }
+}
+
 }  // namespace
 
 // Tile side (cell columns, <= 16): the largest whose corner values fit 60 KB
@@ -392,6 +457,34 @@ hipError_t launch_node_assemble(const double *xs, const double *ys, const double
             hipLaunchKernelGGL(node_assemble_kernel<__hip_bfloat16>, grid, block, lds, st, xs, ys,
                                zs, nx, ny, nz, pts, N_obs, p0, flag, tile, tiles_y, nmax,
                                (__hip_bfloat16 *)G, ld);
+            break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_cell_corner(const double *prisms, int64_t Nm, const double *pts, int64_t N_obs,
+                              uint32_t flag, gra_dtype store, void *G, int64_t ld, hipStream_t st)
+{
+    constexpr int64_t kMaxY = 65535;   // points per launch (grid y)
+    const unsigned bx = (unsigned)((Nm + kBlk - 1) / kBlk);
+    for (int64_t p0 = 0; p0 < N_obs; p0 += kMaxY) {
+        const int64_t np = N_obs - p0 < kMaxY ? N_obs - p0 : kMaxY;
+        const dim3 grid(bx, (unsigned)np), block(kBlk);
+        switch (store) {
+        case GRA_F64:
+            hipLaunchKernelGGL(cell_corner_kernel<double>, grid, block, 0, st, prisms, Nm, pts,
+                               N_obs, p0, flag, (double *)G, ld);
+            break;
+        case GRA_F32:
+            hipLaunchKernelGGL(cell_corner_kernel<float>, grid, block, 0, st, prisms, Nm, pts,
+                               N_obs, p0, flag, (float *)G, ld);
+            break;
+        default:
+            hipLaunchKernelGGL(cell_corner_kernel<__hip_bfloat16>, grid, block, 0, st, prisms, Nm,
+                               pts, N_obs, p0, flag, (__hip_bfloat16 *)G, ld);
             break;
         }
         hipError_t e = hipGetLastError();

@@ -164,12 +164,18 @@ class Fwd:
         method "faces": the reference's per-prism face/edge formulation
         (gra_assemble); "nodes": corner sums shared between the cells of a
         tensor-product mesh (gra_assemble_rectilinear; the prisms must be a
-        regular_mesh-ordered rectilinear mesh and the points above it)."""
+        regular_mesh-ordered rectilinear mesh and the points above it);
+        "corners": corner sums per prism for any mesh (gra_assemble_corners;
+        the points above every prism)."""
         c = self.ctx
         if method == "faces":
             st = c.L.gra_assemble(c.h, _p(self.prisms), self.Nm, _p(self.points), self.N_obs,
                                   self.field_flag, _lib.DTYPES[store])
             c.check(st, "gra_assemble")
+        elif method == "corners":
+            st = c.L.gra_assemble_corners(c.h, _p(self.prisms), self.Nm, _p(self.points),
+                                          self.N_obs, self.field_flag, _lib.DTYPES[store])
+            c.check(st, "gra_assemble_corners")
         elif method == "nodes":
             from .mesh import rectilinear_axes
             axes = rectilinear_axes(self.prisms)

@@ -105,6 +105,20 @@ gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, 
                                    int64_t N_obs, uint32_t field_flag, gra_dtype store);
 
 /*
+ * Fwd::compute_G (Fwd.cpp:47-79) by corner sums for any prisms (octree and
+ * other non-tensor meshes): same arguments, G layout and storage rounding as
+ * gra_assemble, each entry c_k * sum over the prism's 8 corners of
+ * s * F_k(corner - point), evaluated per (point, prism) with no sharing
+ * between prisms, in the difference order of gra_assemble_rectilinear (so a
+ * tensor-product mesh gives the same G both ways).  Agrees with gra_assemble
+ * to ~1e-12 of each component block's largest entry; the relative accuracy
+ * of a single entry falls with its distance (corner-sum cancellation).
+ * Points must lie strictly above every prism (z < min z0), else GRA_BAD_ARG.
+ */
+gra_status gra_assemble_corners(gra_ctx *ctx, const double *prisms, int64_t Nm, const double *pts,
+                                int64_t N_obs, uint32_t field_flag, gra_dtype store);
+
+/*
  * Batched Fwd::compute_G over uneven instances (no counterpart in the
  * reference, which assembles one G per inversion): ctxs[b] (distinct, one
  * device) receives the G of items[b] -- its own mesh, points and component
