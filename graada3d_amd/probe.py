@@ -21,6 +21,14 @@ grid (1.44e7 entries), the same two G by corner sums
     rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 \
         SQ_INSTS_VALU_TRANS_F64 -d <dir> -o asm -- python3 -m graada3d_amd.probe --assemble
 
+With ``--octree`` it assembles G on an octree -- the 40x40x20 mesh with two
+rounds of refinement of a seeded eighth of the cells (158,427 cells) -- three
+times each by the face/edge kernel (gra_assemble) and by per-prism corner
+sums (gra_assemble_corners): gz+Tzz at 50x50 points and all nine components
+at 30x30 points, z = -1 m:
+
+    rocprofv3 --kernel-trace --stats -d <dir> -o oct -- python3 -m graada3d_amd.probe --octree
+
 With ``--mat-mul`` it runs the matrix-core products with 16 right-hand sides
 (gra_G_mat_mul, gra_GT_mat_mul; 2 launches each) on the same 5,000 x 32,000 G
 stored in FP64, FP32 and BF16, for MFMA counters:
@@ -57,6 +65,7 @@ def main(argv=None) -> None:
     ap.add_argument("--all-dtypes", action="store_true")
     ap.add_argument("--assemble", action="store_true")
     ap.add_argument("--mat-mul", action="store_true")
+    ap.add_argument("--octree", action="store_true")
     args = ap.parse_args(argv)
     from .engine import Context, Fwd, forward_rectilinear
     from .workloads import forward_problem_small
@@ -74,6 +83,27 @@ def main(argv=None) -> None:
                 f.G_mat_mul(X)
                 f.GT_mat_mul(R)
             print(f"[probe] {store}: G X and G^T R with 16 right-hand sides", flush=True)
+        ctx.close()
+        return
+
+    if args.octree:
+        from .octree import OctreeMesh
+        ctx = Context(0)
+        mesh = OctreeMesh((0.0, 2000.0), 40, (0.0, 2000.0), 40, (0.0, 1000.0), 20)
+        rng = np.random.default_rng(3)
+        for _ in range(2):
+            n = mesh.n_cells
+            mesh.refine(np.sort(rng.choice(n, n // 8, replace=False)))
+        pr = mesh.prisms()
+        for flag, npt in ((0x12, 50), (0x3FE, 30)):
+            pts = M.grid_points(npt, 0.0, 2000.0, npt, 0.0, 2000.0, -1.0)
+            f = Fwd(pr, pts, flag, ctx=ctx)
+            for method in ("faces", "corners"):
+                for _ in range(3):
+                    f.compute_G("f64", method=method)
+                print(f"[probe] octree {len(pr)} cells, {f.Nd} x {f.Nm} G, flag {flag:#x}, "
+                      f"{method} x3", flush=True)
+            del f
         ctx.close()
         return
 
