@@ -338,6 +338,18 @@ def run_batch(args):
         print(f"[bench] batch step {s}: batched {t_batch[-1]:.3f} s (kernel {k_ms / 1e3:.3f} s), "
               f"sequential {t_seq[-1]:.3f} s", file=sys.stderr, flush=True)
     tb, ts = min(t_batch), min(t_seq)
+    # the same batch by per-prism corner sums (gra_assemble_batch_corners),
+    # then the face/edge G again for the inversion below
+    t_corner = []
+    assemble_batch(fwds, method="corners")
+    for s in range(steps):
+        c0 = time.perf_counter()
+        assemble_batch(fwds, method="corners")
+        t_corner.append(time.perf_counter() - c0)
+    tc = min(t_corner)
+    print(f"[bench] batch corner sums: {tc:.3f} s ({entries / tc:.3e} entries/s)",
+          file=sys.stderr, flush=True)
+    assemble_batch(fwds)
     # batched inversion: per instance, L0 (IRLS, eps^2 = 1e-2) or L2 Gauss-Newton
     # on its octree operators, half of them with the cross-gradient term; the
     # host-side operator build and uploads are not timed
@@ -394,6 +406,10 @@ def run_batch(args):
                    "points": [int(sizes[:, 1].min()), int(sizes[:, 1].max())]},
         "batched_s": tb, "sequential_s": ts, "sequential_entries_per_s": entries / ts,
         "speedup_batched_vs_sequential": ts / tb,
+        "corner_sums": {"batched_s": tc, "entries_per_s": entries / tc,
+                        "speedup_vs_face_edge": tb / tc,
+                        "kernel": "per-prism corner sums, one batched launch "
+                                  "(gra_assemble_batch_corners), FP64"},
         "batch_inversion": {"instances": len(inst), "gn_steps": gn_steps, "cg_iters_per_step": cg_its,
                             "cg_iterations": total_its, "seconds": t_inv,
                             "cg_iter_per_s": total_its / t_inv,

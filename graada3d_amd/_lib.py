@@ -25,7 +25,8 @@ DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
 EXPORTS = [
     "gra_status_string", "gra_last_error", "gra_ctx_create", "gra_ctx_destroy", "gra_sync",
     "gra_assemble", "gra_get_dims", "gra_copy_G", "gra_G_vec_mul", "gra_GT_vec_mul",
-    "gra_col_norms", "gra_expand_cols", "gra_assemble_batch", "gra_assemble_wavelet",
+    "gra_col_norms", "gra_expand_cols", "gra_assemble_batch", "gra_assemble_batch_corners",
+    "gra_assemble_wavelet",
     "gra_wavelet_info", "gra_assemble_rectilinear", "gra_assemble_corners", "gra_G_mat_mul", "gra_GT_mat_mul",
     "gra_forward", "gra_forward_rectilinear", "gra_set_regularization", "gra_solve_cg", "gra_barrier_P",
     "gra_barrier_update", "gra_misfit", "gra_irls_weights", "gra_gn_setup_problem",
@@ -109,6 +110,9 @@ def load() -> ctypes.CDLL:
         "gra_expand_cols": ([vp, c_i64p, ctypes.c_int64, c_dp], ctypes.c_int),
         "gra_assemble_batch": ([ctypes.POINTER(vp), ctypes.c_int64,
                                 ctypes.POINTER(GraBatchItem), ctypes.c_int], ctypes.c_int),
+        "gra_assemble_batch_corners": ([ctypes.POINTER(vp), ctypes.c_int64,
+                                        ctypes.POINTER(GraBatchItem), ctypes.c_int],
+                                       ctypes.c_int),
         "gra_forward": ([vp, c_dp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64, ctypes.c_uint32,
                          c_dp], ctypes.c_int),
         "gra_forward_rectilinear": ([vp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64, c_dp,

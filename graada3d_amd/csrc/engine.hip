@@ -578,8 +578,11 @@ static gra_status matmul_host(gra_ctx *ctx, int trans, const double *in, int64_t
     return GRA_OK;
 }
 
-gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item *items,
-                              gra_dtype store)
+namespace {
+// gra_assemble_batch (face/edge values) and gra_assemble_batch_corners
+// (per-prism corner sums) over one item table.
+gra_status assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item *items,
+                          gra_dtype store, bool corners)
 {
     if (!ctxs || !items || n <= 0 || n > INT32_MAX) return GRA_BAD_ARG;
     std::vector<gra_ctx *> order(ctxs, ctxs + n);
@@ -607,6 +610,15 @@ gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_i
             !points_valid(it.pts, it.N_obs))
             return set_error(c, GRA_BAD_ARG,
                              "batch item " + std::to_string(b) + ": invalid mesh, points or flag");
+        if (corners) {
+            double top = it.prisms[4];
+            for (int64_t j = 1; j < it.Nm; ++j) top = std::min(top, it.prisms[6 * j + 4]);
+            for (int64_t i = 0; i < it.N_obs; ++i)
+                if (!(it.pts[3 * i + 2] < top))
+                    return set_error(c, GRA_BAD_ARG,
+                                     "batch item " + std::to_string(b) +
+                                         ": points must lie strictly above every prism");
+        }
         tot_prisms += it.Nm;
     }
     std::vector<double> prisms((size_t)(6 * tot_prisms));
@@ -655,9 +667,14 @@ gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_i
     HIPCHK(hipMemsetAsync(c.batch_bad.p, 0, sizeof(int) * (size_t)n, c.stream), "clear status");
     {
         TimedRegion tr(c, kTimeAssemble);
-        HIPCHK(launch_assemble_batch(c.batch_items.as<BatchItem>(), (int)n, blocks, store,
-                                     c.batch_bad.as<int>(), c.stream),
-               "batched assembly");
+        if (corners)
+            HIPCHK(launch_cell_corner_batch(c.batch_items.as<BatchItem>(), (int)n, blocks, store,
+                                            c.stream),
+                   "batched corner-sum assembly");
+        else
+            HIPCHK(launch_assemble_batch(c.batch_items.as<BatchItem>(), (int)n, blocks, store,
+                                         c.batch_bad.as<int>(), c.stream),
+                   "batched assembly");
     }
     std::vector<int> bad((size_t)n, 0);
     HIPCHK(hipMemcpyAsync(bad.data(), c.batch_bad.p, sizeof(int) * (size_t)n,
@@ -679,6 +696,19 @@ gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_i
                              "batch item " + std::to_string(b) +
                                  ": an observation point lies on a prism edge (Integral.cpp:77)");
     return GRA_OK;
+}
+}  // namespace
+
+gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item *items,
+                              gra_dtype store)
+{
+    return assemble_batch(ctxs, n, items, store, false);
+}
+
+gra_status gra_assemble_batch_corners(gra_ctx *const *ctxs, int64_t n,
+                                      const gra_batch_item *items, gra_dtype store)
+{
+    return assemble_batch(ctxs, n, items, store, true);
 }
 
 gra_status gra_expand_cols(gra_ctx *ctx, const int64_t *split_cols, int64_t n_split,

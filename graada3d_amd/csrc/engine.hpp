@@ -189,6 +189,9 @@ struct BatchItem {
 };
 hipError_t launch_assemble_batch(const BatchItem *d_items, int n_items, int64_t total_blocks,
                                  gra_dtype store, int *d_bad, hipStream_t st);
+// Per-prism corner sums over the same item table (rectilinear_kernels.hip).
+hipError_t launch_cell_corner_batch(const BatchItem *d_items, int n_items, int64_t total_blocks,
+                                    gra_dtype store, hipStream_t st);
 
 // Node-sum forward on a rectilinear mesh (rectilinear_kernels.hip); w holds
 // (nx+1)(ny+1)(nz+1) node weights.

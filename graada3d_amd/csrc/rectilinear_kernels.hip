@@ -348,27 +348,16 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
 // difference order of node_assemble_kernel -- x differences first, then y,
 // then z -- so a cell of a tensor-product mesh gets the same value both ways.
 // Threads take consecutive cells of one point: consecutive G columns.
-template <typename T>
-__global__ void __launch_bounds__(kBlk)
-cell_corner_kernel(const double *__restrict__ prisms, int64_t Nm, const double *__restrict__ pts,
-                   int64_t N_obs, int64_t p0, uint32_t flag, T *__restrict__ G, int64_t ld)
+// sum[bit - 1] receives the unscaled corner sum of each requested component.
+__device__ inline void prism_corner_sums(double px, double py, double pz,
+                                         const double *__restrict__ pr, uint32_t flag,
+                                         const double *lg_G, const double *lg_T, double sum[9])
 {
-    __shared__ double lg_G[kLogTable], lg_T[kLogTable];
-    for (int q = threadIdx.x; q < kLogTable; q += kBlk) {
-        lg_G[q] = kLogG[q];
-        lg_T[q] = kLogT[q];
-    }
-    __syncthreads();
-    const int64_t ip = p0 + blockIdx.y;
-    const int64_t j = (int64_t)blockIdx.x * kBlk + threadIdx.x;
-    if (ip >= N_obs || j >= Nm) return;
-    const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
-    const double *pr = prisms + 6 * j;
     const double xv[2] = {pr[0] - px, pr[1] - px};
     const double yv[2] = {pr[2] - py, pr[3] - py};
     const double zv[2] = {pr[4] - pz, pr[5] - pz};
     const NodeTerms nt(flag);
-    double dlo[9], e0[9];
+    double e0[9];
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
 #pragma unroll
@@ -392,17 +381,78 @@ cell_corner_kernel(const double *__restrict__ prisms, int64_t Nm, const double *
 #pragma unroll
             for (int bit = 1; bit < 10; ++bit) {
                 if (!(flag & (1u << bit))) continue;
-                if (bb == 0) dlo[bit - 1] = d[bit - 1];
-                else if (kk == 0) e0[bit - 1] = d[bit - 1] - dlo[bit - 1];
-                else dlo[bit - 1] = (d[bit - 1] - dlo[bit - 1]) - e0[bit - 1];
+                if (bb == 0) sum[bit - 1] = d[bit - 1];
+                else if (kk == 0) e0[bit - 1] = d[bit - 1] - sum[bit - 1];
+                else sum[bit - 1] = (d[bit - 1] - sum[bit - 1]) - e0[bit - 1];
             }
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+cell_corner_kernel(const double *__restrict__ prisms, int64_t Nm, const double *__restrict__ pts,
+                   int64_t N_obs, int64_t p0, uint32_t flag, T *__restrict__ G, int64_t ld)
+{
+    __shared__ double lg_G[kLogTable], lg_T[kLogTable];
+    for (int q = threadIdx.x; q < kLogTable; q += kBlk) {
+        lg_G[q] = kLogG[q];
+        lg_T[q] = kLogT[q];
+    }
+    __syncthreads();
+    const int64_t ip = p0 + blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (ip >= N_obs || j >= Nm) return;
+    double sum[9];
+    prism_corner_sums(pts[3 * ip], pts[3 * ip + 1], pts[3 * ip + 2], prisms + 6 * j, flag, lg_G,
+                      lg_T, sum);
     int q = 0;
 #pragma unroll
     for (int bit = 1; bit < 10; ++bit) {
         if (!(flag & (1u << bit))) continue;
-        G[((int64_t)q * N_obs + ip) * ld + j] = store_as<T>(comp_scale(bit) * dlo[bit - 1]);
+        G[((int64_t)q * N_obs + ip) * ld + j] = store_as<T>(comp_scale(bit) * sum[bit - 1]);
+        ++q;
+    }
+}
+
+// The batched form over uneven instances: blocks are distributed over the
+// items as in the batched face/edge assembly (item = last block0 <= block;
+// within an item, blocks walk the point rows, ceil(Nm / 256) per row).
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+cell_corner_batch_kernel(const BatchItem *__restrict__ items, int n_items)
+{
+    __shared__ double lg_G[kLogTable], lg_T[kLogTable];
+    __shared__ int s_item;
+    for (int q = threadIdx.x; q < kLogTable; q += kBlk) {
+        lg_G[q] = kLogG[q];
+        lg_T[q] = kLogT[q];
+    }
+    const int64_t b = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = n_items - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].block0 <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        s_item = lo;
+    }
+    __syncthreads();
+    const BatchItem it = items[s_item];
+    const int64_t lb = b - it.block0;
+    const int64_t i = lb / it.bpr;
+    const int64_t j = (lb - i * it.bpr) * kBlk + threadIdx.x;
+    if (i >= it.N_obs || j >= it.Nm) return;
+    double sum[9];
+    prism_corner_sums(it.pts[3 * i], it.pts[3 * i + 1], it.pts[3 * i + 2], it.prisms + 6 * j,
+                      it.flag, lg_G, lg_T, sum);
+    T *G = static_cast<T *>(it.G);
+    int q = 0;
+#pragma unroll
+    for (int bit = 1; bit < 10; ++bit) {
+        if (!(it.flag & (1u << bit))) continue;
+        G[((int64_t)q * it.N_obs + i) * it.ld + j] = store_as<T>(comp_scale(bit) * sum[bit - 1]);
         ++q;
     }
 }
@@ -491,6 +541,26 @@ hipError_t launch_cell_corner(const double *prisms, int64_t Nm, const double *pt
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_cell_corner_batch(const BatchItem *d_items, int n_items, int64_t total_blocks,
+                                    gra_dtype store, hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    const dim3 grid((unsigned)total_blocks), block(kBlk);
+    switch (store) {
+    case GRA_F64:
+        hipLaunchKernelGGL(cell_corner_batch_kernel<double>, grid, block, 0, st, d_items, n_items);
+        break;
+    case GRA_F32:
+        hipLaunchKernelGGL(cell_corner_batch_kernel<float>, grid, block, 0, st, d_items, n_items);
+        break;
+    default:
+        hipLaunchKernelGGL(cell_corner_batch_kernel<__hip_bfloat16>, grid, block, 0, st, d_items,
+                           n_items);
+        break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_node_forward(const double *xs, const double *ys, const double *zs, int64_t nx,

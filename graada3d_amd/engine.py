@@ -423,9 +423,13 @@ class GaussNewtonInversion(Fwd):
         return m
 
 
-def assemble_batch(fwds, store: str = "f64"):
+def assemble_batch(fwds, store: str = "f64", method: str = "faces"):
     """Assemble the G of every Fwd in `fwds` (each with its own context, one
-    device) in a single batched launch (gra_assemble_batch)."""
+    device) in a single batched launch: the face/edge values
+    (gra_assemble_batch) or, with method "corners", per-prism corner sums
+    (gra_assemble_batch_corners; points above every prism of their item)."""
+    if method not in ("faces", "corners"):
+        raise ValueError(f"unknown assembly method {method!r}")
     n = len(fwds)
     if n == 0:
         return
@@ -434,7 +438,8 @@ def assemble_batch(fwds, store: str = "f64"):
     items = (_lib.GraBatchItem * n)()
     for b, f in enumerate(fwds):
         items[b] = _lib.GraBatchItem(_p(f.prisms), f.Nm, _p(f.points), f.N_obs, f.field_flag)
-    st = L.gra_assemble_batch(hs, n, items, _lib.DTYPES[store])
+    fn = L.gra_assemble_batch if method == "faces" else L.gra_assemble_batch_corners
+    st = fn(hs, n, items, _lib.DTYPES[store])
     fwds[0].ctx.check(st, "assemble_batch")
     for f in fwds:
         f._assembled = True

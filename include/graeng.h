@@ -138,6 +138,15 @@ gra_status gra_assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_i
                               gra_dtype store);
 
 /*
+ * gra_assemble_batch with the values of gra_assemble_corners (per-prism
+ * corner sums; no counterpart in the reference): the same items, contexts
+ * and launch distribution; every item's points must lie strictly above
+ * every prism of that item, else GRA_BAD_ARG naming the item.
+ */
+gra_status gra_assemble_batch_corners(gra_ctx *const *ctxs, int64_t n,
+                                      const gra_batch_item *items, gra_dtype store);
+
+/*
  * Replaces Fwd::compute_G_wavelet (src/Forward_modelling/Fwd.cpp:148-219,
  * compress_vec :298-371; the reference's method 2).  order[j] is the cell
  * at reordered position j (Mesh::get_reordered_id, a permutation of

@@ -118,3 +118,38 @@ def test_rejections(ctx):
     f = Fwd(prisms, np.array([[10.0, 10.0, 49.0]]), 0x2, ctx=ctx)
     f.compute_G(method="corners")
     assert np.isfinite(f.get_G()).all()
+
+
+def test_batched_corners_equal_per_instance(ctx):
+    """gra_assemble_batch_corners fills each context's G exactly as
+    gra_assemble_corners does for that instance (uneven octree instances,
+    different component sets, every storage type); the batch rejects an item
+    with a point below a prism's top."""
+    from graada3d_amd import Context
+    from graada3d_amd.engine import assemble_batch
+    from graada3d_amd.workloads import batch_instances
+    inst = batch_instances(n=5, seed=77)
+    for b in inst:
+        b.points = b.points[:: max(1, b.points.shape[0] // 60)]
+    for store in ("f64", "f32", "bf16"):
+        print(f"batched corners {store}", flush=True)
+        fw = [Fwd(b.prisms, b.points, b.flag, ctx=Context(0)) for b in inst]
+        try:
+            assemble_batch(fw, store, method="corners")
+            for b, f in zip(inst, fw):
+                single = Fwd(b.prisms, b.points, b.flag, ctx=ctx)
+                single.compute_G(store, method="corners")
+                assert np.array_equal(f.get_G(), single.get_G()), (store, b.seed)
+        finally:
+            for f in fw:
+                f.ctx.close()
+    bad = inst[1].points.copy()
+    bad[0, 2] = inst[1].prisms[:, 4].min() + 1.0
+    fw = [Fwd(inst[0].prisms, inst[0].points, inst[0].flag, ctx=Context(0)),
+          Fwd(inst[1].prisms, bad, inst[1].flag, ctx=Context(0))]
+    try:
+        with pytest.raises(GraError, match="item 1"):
+            assemble_batch(fw, "f64", method="corners")
+    finally:
+        for f in fw:
+            f.ctx.close()
