@@ -4,6 +4,7 @@
 //   graeng::Fwd                 <-> class Fwd               (src/Forward_modelling/Fwd.h)
 //     compute_G()                    Fwd::compute_G          (Fwd.cpp:47-79)
 //     compute_G_rectilinear(xs,ys,zs)  the same for tensor-product meshes (corner sums)
+//     compute_G_corners()            the same for any mesh (per-prism corner sums)
 //     G_vec_mul(vec, product)        Fwd::G_vec_mul          (Fwd.cpp:114-147)
 //     GT_vec_mul(vec, product)       Fwd::GT_vec_mul         (Fwd.cpp:80-113)
 //     compute_gobs(rho)              Fwd::compute_gobs       (Fwd.h:23-27)
@@ -125,6 +126,14 @@ public:
         check(ctx_.get(), gra_assemble(ctx_.get(), prisms_.data(), Nm, points_.data(), N_obs, flag_,
                                        store),
               "compute_G");
+    }
+    // compute_G by per-prism corner sums (gra_assemble_corners): any mesh,
+    // points strictly above every prism.
+    void compute_G_corners(gra_dtype store = GRA_F64)
+    {
+        check(ctx_.get(), gra_assemble_corners(ctx_.get(), prisms_.data(), Nm, points_.data(),
+                                               N_obs, flag_, store),
+              "compute_G_corners");
     }
     // compute_G for a tensor-product mesh whose prisms are the cells of the
     // node coordinates xs, ys, zs in Mesh::generate_regular_mesh order (the

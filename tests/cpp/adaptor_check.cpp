@@ -89,6 +89,10 @@ int main(int argc, char **argv)
         fwrite(GTy.data(), sizeof(double), GTy.size(), o);
         fwrite(dx.data(), sizeof(double), dx.size(), o);
         fwrite(norms.data(), sizeof(double), norms.size(), o);
+        std::vector<double> Gx_c;
+        inv.compute_G_corners();
+        inv.G_vec_mul(x, Gx_c);
+        fwrite(Gx_c.data(), sizeof(double), Gx_c.size(), o);
         double tail[2] = {err, (double)its};
         fwrite(tail, sizeof(double), 2, o);
         fclose(o);

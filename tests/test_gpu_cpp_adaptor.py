@@ -54,7 +54,11 @@ def test_cpp_adaptor_matches_oracle(tmp_path):
     out = np.frombuffer(fout.read_bytes(), np.float64)
     Gx, GTy, dx = out[:nd], out[nd:nd + nm], out[nd + nm:nd + 2 * nm]
     norms = out[nd + 2 * nm:nd + 3 * nm]
+    Gx_corners = out[nd + 3 * nm:2 * nd + 3 * nm]
+    assert out.size == 2 * nd + 3 * nm + 2
     err, its = out[-2], int(out[-1])
+    # the per-prism corner-sum G (compute_G_corners) times x
+    assert rel_norm(Gx_corners, G @ x) < 1e-10
     assert rel_norm(Gx, G @ x) < 1e-12
     assert rel_norm(GTy, G.T @ y) < 1e-12
     assert np.abs(norms / np.linalg.norm(G, axis=0) - 1.0).max() < 1e-14
