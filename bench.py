@@ -44,14 +44,31 @@ bounded sample.
   to end by the inversion driver; `value` is the wall time of one whole
   inversion, beside the run time the reference logged for it.
 
+Counter probes (--probe, fixed short launch sequences for rocprofv3 passes;
+no JSON line):
+* gemv: a plain 1 GiB device-to-device copy (the calibration dispatch for
+  FETCH_SIZE / WRITE_SIZE), then the 5,000 x 32,000 G of inversion_small in
+  FP64 (and FP32, BF16 with --all-dtypes) with 3 launches each of G x and
+  G^T w.
+* assemble: G of that survey (gz+Tzz), all nine components on a 20x20x10 mesh
+  at 40x40 points, the same two G by corner sums, one node-sum forward
+  (FP64 instruction counters).
+* octree: the 40x40x20 mesh refined twice at a seeded eighth of its cells
+  (158,427 cells): G by face/edge integrals and by per-prism corner sums,
+  gz+Tzz at 50x50 points and all nine at 30x30, three times each.
+* mat-mul: the matrix-core products with 16 right-hand sides on the
+  5,000 x 32,000 G in FP64, FP32 and BF16 (MFMA counters).
+
     python bench.py [--workload inversion_small|tensor_large|batch|serving|karst] [--steps K]
                     [--warmup W]
                     [--stores f64,f32,bf16] [--no-fused] [--no-cpu] [--gpus 1]
                     [--rate R] [--duration S]
+    python bench.py --probe gemv|assemble|octree|mat-mul [--all-dtypes]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -618,6 +635,101 @@ def karst_cpu_baseline(cfg, trials, final_nm, n_trials=2):
                       f"{len(trials)} trials)"}
 
 
+def device_copy(nbytes: int) -> None:
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):      # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def run_probe(args) -> None:
+    """Fixed, short launch sequences for rocprofv3 counter passes (the
+    --probe modes listed in the module docstring)."""
+    from graada3d_amd.engine import Context, Fwd, forward_rectilinear
+    from graada3d_amd.workloads import forward_problem_small
+    from graada3d_amd import mesh as M
+
+    if args.probe == "mat-mul":
+        ctx = Context(0)
+        fp = forward_problem_small()
+        f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
+        rng = np.random.default_rng(0)
+        X, R = rng.standard_normal((f.Nm, 16)), rng.standard_normal((f.Nd, 16))
+        for store in ("f64", "f32", "bf16"):
+            f.compute_G(store)
+            for _ in range(2):
+                f.G_mat_mul(X)
+                f.GT_mat_mul(R)
+            print(f"[probe] {store}: G X and G^T R with 16 right-hand sides", flush=True)
+        ctx.close()
+        return
+
+    if args.probe == "octree":
+        from graada3d_amd.octree import OctreeMesh
+        ctx = Context(0)
+        mesh = OctreeMesh((0.0, 2000.0), 40, (0.0, 2000.0), 40, (0.0, 1000.0), 20)
+        rng = np.random.default_rng(3)
+        for _ in range(2):
+            n = mesh.n_cells
+            mesh.refine(np.sort(rng.choice(n, n // 8, replace=False)))
+        pr = mesh.prisms()
+        for flag, npt in ((0x12, 50), (0x3FE, 30)):
+            pts = M.grid_points(npt, 0.0, 2000.0, npt, 0.0, 2000.0, -1.0)
+            f = Fwd(pr, pts, flag, ctx=ctx)
+            for method in ("faces", "corners"):
+                for _ in range(3):
+                    f.compute_G("f64", method=method)
+                print(f"[probe] octree {len(pr)} cells, {f.Nd} x {f.Nm} G, flag {flag:#x}, "
+                      f"{method} x3", flush=True)
+            del f
+        ctx.close()
+        return
+
+    if args.probe == "assemble":
+        ctx = Context(0)
+        fp = forward_problem_small()
+        f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
+        f.compute_G("f64")
+        print(f"[probe] assembled {f.Nd} x {f.Nm} (gz+Tzz)", flush=True)
+        pr = M.regular_mesh((0.0, 2000.0), 20, (0.0, 2000.0), 20, (0.0, 1000.0), 10)
+        pts = M.grid_points(40, 0.0, 2000.0, 40, 0.0, 2000.0, -1.0)
+        g = Fwd(pr, pts, 0x3FE, ctx=ctx)
+        g.compute_G("f64")
+        print(f"[probe] assembled {g.Nd} x {g.Nm} (9 components)", flush=True)
+        f.compute_G("f64", method="nodes")
+        g.compute_G("f64", method="nodes")
+        print("[probe] the same two G by corner sums", flush=True)
+        xs = M.linspaced(41, 0.0, 2000.0)
+        zs = M.linspaced(21, 0.0, 1000.0)
+        forward_rectilinear(xs, xs, zs, fp.rho, fp.points, 0x3FE, ctx=ctx)
+        print("[probe] node-sum forward 40x40x20 mesh, 2500 points, 9 components", flush=True)
+        ctx.close()
+        return
+
+    device_copy(1 << 30)
+    print("[probe] 1 GiB device copy done", flush=True)
+    ctx = Context(0)
+    fp = forward_problem_small()
+    f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal(f.Nm), rng.standard_normal(f.Nd)
+    for store in (("f64", "f32", "bf16") if args.all_dtypes else ("f64",)):
+        f.compute_G(store)
+        for _ in range(3):
+            f.G_vec_mul(x)
+            f.GT_vec_mul(y)
+        print(f"[probe] {store}: 3 x G x, 3 x G^T w on {f.Nd} x {f.Nm}", flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -643,11 +755,18 @@ def main():
                     help="karst: dense G (1) or wavelet-compressed G (2)")
     ap.add_argument("--karst-threshold", type=float, default=5e-4,
                     help="karst: relative threshold of the wavelet compression (method 2)")
+    ap.add_argument("--probe", default=None, choices=["gemv", "assemble", "octree", "mat-mul"],
+                    help="run a fixed launch sequence for rocprofv3 counter passes instead")
+    ap.add_argument("--all-dtypes", action="store_true",
+                    help="probe gemv: also with G stored in FP32 and BF16")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine's "
               "multi-GPU sharding is a design note, not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        run_probe(args)
+        return
     if args.workload == "serving":
         run_serving(args)
         return

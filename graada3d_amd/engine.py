@@ -13,7 +13,10 @@ CPU fallback.  Errors raise ``GraError`` (the reference asserts or aborts).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
+import weakref
 
 import numpy as np
 import scipy.sparse as sp
@@ -59,6 +62,18 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(c_dp)
 
 
+# Contexts still open; closed by an atexit hook while the GPU runtime is
+# intact (a context left to the garbage collector during interpreter teardown
+# would call into the runtime after its own exit handlers have run).
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for c in list(_LIVE):
+        c.close()
+
+
 class Context:
     """One engine context (device buffers + its own GPU stream)."""
 
@@ -69,6 +84,7 @@ class Context:
         if st != 0:
             raise GraError(st, "gra_ctx_create", "(is a GPU visible?)")
         self.h = h
+        _LIVE.add(self)
 
     def check(self, st: int, where: str):
         if st != 0:
@@ -79,8 +95,19 @@ class Context:
         if getattr(self, "h", None):
             self.L.gra_ctx_destroy(self.h)
             self.h = None
+        _LIVE.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
+        # no runtime calls once the interpreter is finalising: the atexit
+        # hook has closed every live context before that point
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
