@@ -1238,9 +1238,22 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
         c.fused_wgs = (int)value;
         break;
     case GRA_TUNE_FUSED_DEPTH:
-        if (value < 0 || value > 7)
-            return set_error(c, GRA_BAD_ARG, "fused lag variant: 0 (default) or 1..7");
+        if (value < 0 || value > 3)
+            return set_error(c, GRA_BAD_ARG, "fused lag variant: 0 (default) or 1..3");
         c.fused_depth = (int)value;
+        break;
+    case GRA_TUNE_FUSED_MATH:
+        if (value < 0 || value > 2) return set_error(c, GRA_BAD_ARG, "fused math: 0, 1 or 2");
+        c.fused_math = (int)value;
+        break;
+    case GRA_TUNE_FUSED_FAIL:
+        if (value < -1) return set_error(c, GRA_BAD_ARG, "fused failing row block: -1 or >= 0");
+        c.fused_fail_block = value;
+        break;
+    case GRA_TUNE_FUSED_FALLBACKS:
+        if (value != 0) return set_error(c, GRA_BAD_ARG, "fused fallbacks: only 0 (re-enable)");
+        c.fused_failed = false;
+        c.fused_fail_count = 0;
         break;
     case GRA_TUNE_N_BLOCK:
         if (value < -1 || value > 1) return set_error(c, GRA_BAD_ARG, "block-shared rows: -1..1");
@@ -1383,6 +1396,9 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_NONTEMPORAL: *value = s.nontemporal; break;
     case GRA_TUNE_FUSED_WGS: *value = c.fused_wgs; break;
     case GRA_TUNE_FUSED_DEPTH: *value = c.fused_depth; break;
+    case GRA_TUNE_FUSED_FAIL: *value = c.fused_fail_block; break;
+    case GRA_TUNE_FUSED_MATH: *value = c.fused_math; break;
+    case GRA_TUNE_FUSED_FALLBACKS: *value = c.fused_fail_count; break;
     case GRA_TUNE_N_BLOCK: *value = s.n_block; break;
     case GRA_TUNE_NODE_TILE: *value = c.node_tile; break;
     default: return set_error(c, GRA_BAD_ARG, "unknown tuning parameter");

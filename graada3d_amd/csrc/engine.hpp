@@ -78,6 +78,8 @@ enum TimingClass {
 // Launch plan of the single-read fused product (fused_kernels.hip).
 struct FusedPlan {
     int NT = 0, S = 0;            // column tiles, row groups (grid = NT * S)
+    int per_xcd = 0, n_xcd = 1;   // XCD-aligned workgroup mapping when per_xcd > 0
+    int grid = 0;
     int64_t rows_per_group = 0;
     int64_t nrb_group = 0;        // row blocks per group
     int RB = 0;                   // rows per block
@@ -105,7 +107,7 @@ struct Ctx {
 
     // scratch
     DevBuf part_n, part_t;     // GEMV partial sums
-    DevBuf fused_scratch, fused_coef, rloop;
+    DevBuf fused_scratch, rloop;
     DevBuf vec_nd[4], vec_nm[12], vec_row[3];
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
@@ -149,8 +151,12 @@ struct Ctx {
 
     // launch-shape overrides (0 = automatic; nontemporal -1 = automatic)
     GemvShape tune_override{0, 0, 0, 0, 0, -1, -1};
-    int fused_wgs = 0;            // 0: two per CU
+    int fused_wgs = 0;            // 0: one per CU
     int fused_depth = 0;          // 0: default (lag, prefetch) variant
+    int fused_math = 0;           // 0: default per storage type, 1: FP64, 2: FP32 products
+    int64_t fused_fail_block = -1;   // test hook: force a failed wait at this row block
+    bool fused_failed = false;    // a fused solve failed on this context: two-pass from now on
+    int fused_fail_count = 0;     // fused solves rerun with the two-pass iteration
 
     // timing
     int timing = 0;            // 0 off; n: every launch, CG-loop launches every n-th iteration
@@ -268,15 +274,17 @@ gra_status wavelet_assemble(Ctx &c, const double *d_prisms_ro, int64_t Nm, const
 hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, int64_t ld,
                            double *out, int col_major, hipStream_t st);
 
-// Fused product (fused_kernels.hip).  fused_plan returns false when the
-// cooperative grid cannot be co-resident on `device`; target_wgs <= 0 means
-// two workgroups per CU, larger values are capped at the co-resident limit.
+// Fused product (fused_kernels.hip).  fused_plan returns false when no grid
+// can be co-resident on `device` for this G; target_wgs <= 0 means one
+// workgroup per CU, larger values are capped at the co-resident limit.
+// math: 0 default for the storage type, 1 FP64 products, 2 FP32 products.
 bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, int depth,
-                FusedPlan *out);
+                int math, FusedPlan *out);
 size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld);
+// fail_block >= 0 makes the wait on that row block fail (test hook).
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
-                        const double *x, const double *Wd, double *coef, double *tdat,
-                        void *scratch, int *err, const int *done, double **zpart_out,
+                        const double *x, const double *Wd, double *tdat, void *scratch, int *err,
+                        const int *done, int64_t fail_block, double **zpart_out,
                         double **t2part_out, hipStream_t st);
 
 // CGLS (cgls.hip)

@@ -281,7 +281,9 @@ typedef struct gra_cg_args {
      * 1: single-read iteration: one fused pass computes G Pp and
      *    G^T Wd^2 G Pp together and r is updated as r + alpha A^T t (equal in
      *    exact arithmetic, different rounding).  Falls back to 0 when the
-     *    fused grid cannot be co-resident on the device.               */
+     *    fused grid cannot be co-resident on the device; when one of the
+     *    fused product's bounded waits fails, the solve is rerun with 0
+     *    from the same inputs (GRA_TUNE_FUSED_FALLBACKS counts these).  */
     int fused;
 } gra_cg_args;
 
@@ -366,12 +368,22 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
 #define GRA_TUNE_NONTEMPORAL 5  /* non-temporal loads of G, mask: 1 G*x, 2 G^T*w */
 #define GRA_TUNE_FUSED_WGS 6    /* workgroups of the fused CG product: 1..65536,
                                    capped at what can be co-resident      */
-#define GRA_TUNE_FUSED_DEPTH 7  /* fused: (lag L, prefetch PF) variant 1..7 =
-                                   (3,2) (4,2) (5,1) (6,1) (6,2) (6,3) (8,2) */
+#define GRA_TUNE_FUSED_DEPTH 7  /* fused: (lag L, prefetch PF) variant 1..3 =
+                                   (2,1) (2,2) (3,2); 0 = (2,1)            */
 #define GRA_TUNE_N_BLOCK 8      /* G*x rows owned by a wave (0) or shared by the
                                    4 waves of a workgroup (1); -1 = auto */
 #define GRA_TUNE_NODE_TILE 9    /* gra_assemble_rectilinear tile side in cell
                                    columns, 1..32; 0 = auto (LDS budget) */
+#define GRA_TUNE_FUSED_FAIL 10  /* test hook: the fused product's wait on this
+                                   row block fails (-1 = none), forcing the
+                                   two-pass rerun of the solve            */
+#define GRA_TUNE_FUSED_MATH 12  /* fused product arithmetic: 0 = default (FP32
+                                   products with FP64 sums for BF16 storage,
+                                   FP64 otherwise), 1 = FP64, 2 = FP32
+                                   products (BF16 / FP32 storage)         */
+#define GRA_TUNE_FUSED_FALLBACKS 11 /* get: fused solves rerun two-pass on this
+                                   context (it then stays two-pass); set 0:
+                                   clear and allow the fused iteration again */
 gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value);
 gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value);
 /*

@@ -27,3 +27,21 @@ extern "C" int hc_prism_field(const double *pt, const double *pr, double rho, ui
     default: return gra::prism_fields(pt, pr, rho, flag, out10);
     }
 }
+
+// The fused product's launch planner (graada3d_amd/csrc/fused_plan.hpp).
+// out: NT, S, per_xcd, rows_per_group, nrb, grid.
+#include "fused_plan.hpp"
+
+extern "C" int hc_fused_plan(int64_t Nd, int64_t ld, int TW, int RB, int cus, int n_xcd,
+                             int per_cu, int target, int max_poll, int64_t *out6)
+{
+    gra::FusedShape s;
+    if (!gra::fused_plan_shape(Nd, ld, TW, RB, cus, n_xcd, per_cu, target, max_poll, &s)) return 0;
+    out6[0] = s.NT;
+    out6[1] = s.S;
+    out6[2] = s.per_xcd;
+    out6[3] = s.rows_per_group;
+    out6[4] = s.nrb;
+    out6[5] = s.grid;
+    return 1;
+}

@@ -73,8 +73,9 @@ def test_fused_solve_cg_vs_oracle(ctx, quirks, wl, crg):
 
 @pytest.mark.parametrize("wgs", [1, 3, 7, 64, 0])
 def test_fused_grid_shapes(ctx, wgs):
-    """Row groups from 1 (one workgroup streams every row block of its tile)
-    to the default; more tiles than rows per group and the reverse."""
+    """Grids from 13 workgroups (13 row groups of one 4,096-column tile) to
+    the default (one per CU, XCD-aligned): groups of one row block and groups
+    of many."""
     inv, G, aux = small_inversion(ctx, nx=20, ny=18, nz=9, npt=25, flag=0x3FE)
     try:
         ctx.tune(fused_wgs=wgs * ((inv.Nm + 255) // 256) if wgs else 0)
@@ -87,19 +88,71 @@ def test_fused_grid_shapes(ctx, wgs):
     assert rel_norm(dx, dxo) < FUSED_TOL
 
 
+def test_fused_failed_wait_reruns_two_pass(ctx):
+    """A failed wait inside the fused product (forced from the host at row
+    block 0 of every row group) makes the solve rerun with the two-pass iteration from the same
+    inputs: the result is bit-identical to a plain two-pass solve, the
+    context counts the fallback and stays on the two-pass iteration until
+    re-enabled."""
+    inv, G, aux = small_inversion(ctx)
+    args = _cg_inputs(inv, aux)[:4]
+    (dx_2, err_2, its_2), _, _, _ = _solve(inv, aux, True, {}, fused=False, args=args)
+    try:
+        ctx.tune(fused_fail=0)
+        (dx_f, err_f, its_f), _, n_fused, _ = _solve(inv, aux, True, {}, fused=True, args=args)
+        assert ctx.tuning()["fused_fallbacks"] == 1
+        ctx.tune(fused_fail=-1)
+        (dx_g, _, _), _, n_fused2, _ = _solve(inv, aux, True, {}, fused=True, args=args)
+        assert n_fused2 == 0 and ctx.tuning()["fused_fallbacks"] == 1
+    finally:
+        ctx.tune(fused_fail=-1, fused_fallbacks=0)
+    assert np.array_equal(dx_f, dx_2) and err_f == err_2 and its_f == its_2
+    assert np.array_equal(dx_g, dx_2)
+    (dx_h, _, _), _, n_fused3, _ = _solve(inv, aux, True, {}, fused=True, args=args)
+    assert n_fused3 == 20
+
+
 @pytest.mark.parametrize("store", ["f32", "bf16"])
 def test_fused_reduced_storage_matches_two_pass(ctx, store):
-    """With G stored in FP32/BF16 both iterations read the same stored values
-    and accumulate in FP64: they agree to FP64 rounding of the recurrence."""
+    """With G stored in FP32/BF16 and FP64 products (fused_math 1) both
+    iterations read the same stored values and accumulate in FP64: they agree
+    to FP64 rounding of the recurrence."""
     inv, G, aux = small_inversion(ctx, nx=20, ny=18, nz=9, npt=15)
     inv.compute_G(store)
     try:
+        ctx.tune(fused_math=1)
         (dx_f, err_f, _), args, n_fused, _ = _solve(inv, aux, True, {}, fused=True, its=20)
         (dx_2, err_2, _), _, n_f2, _ = _solve(inv, aux, True, {}, fused=False, its=20, args=args)
     finally:
+        ctx.tune(fused_math=0)
         inv.compute_G("f64")
     assert n_fused == 20 and n_f2 == 0
     assert rel_norm(dx_f, dx_2) < REF_TOL
+
+
+@pytest.mark.parametrize("store", ["f32", "bf16"])
+def test_fused_fp32_products_match_two_pass(ctx, store):
+    """FP32 products with FP64 sums (fused_math 2, the BF16 default): a
+    thread's products of one row (8 for BF16, 8 for FP32 storage) and of one
+    column over a block's rows are summed in FP32 with x and the coefficients
+    rounded to FP32 (unit roundoff 6e-8), everything else in FP64.  After
+    three CG iterations dx agrees with the two-pass FP64-accumulating run
+    within 1e-5 (a bound 100x above the FP32 rounding of the products, far
+    below the 4e-3 relative rounding of BF16 storage itself), and the run is
+    bit-for-bit repeatable."""
+    inv, G, aux = small_inversion(ctx, nx=20, ny=18, nz=9, npt=15)
+    inv.compute_G(store)
+    try:
+        ctx.tune(fused_math=2)
+        (dx_f, _, _), args, n_fused, _ = _solve(inv, aux, True, {}, fused=True, its=3)
+        (dx_g, _, _), _, _, _ = _solve(inv, aux, True, {}, fused=True, its=3, args=args)
+        (dx_2, _, _), _, _, _ = _solve(inv, aux, True, {}, fused=False, its=3, args=args)
+    finally:
+        ctx.tune(fused_math=0)
+        inv.compute_G("f64")
+    assert n_fused == 3
+    assert np.array_equal(dx_f, dx_g)
+    assert 0 < rel_norm(dx_f, dx_2) < 1e-5
 
 
 def test_fused_tolerance_stop(ctx):
