@@ -31,12 +31,12 @@ EXPORTS = [
     "gra_forward", "gra_forward_rectilinear", "gra_set_regularization", "gra_solve_cg", "gra_barrier_P",
     "gra_barrier_update", "gra_misfit", "gra_irls_weights", "gra_gn_setup_problem",
     "gra_gn_step", "gra_gn_get_model", "gra_timing_enable", "gra_timing_reset",
-    "gra_timing_get", "gra_tune_set", "gra_tune_get", "gra_autotune",
+    "gra_timing_get", "gra_tune_set", "gra_tune_get", "gra_autotune", "gra_mixed_precision",
 ]
 
 TUNE = {"n_rows": 0, "n_unroll": 1, "n_splits": 2, "t_chunks": 3, "t_unroll": 4, "nontemporal": 5,
         "fused_wgs": 6, "fused_depth": 7, "n_block": 8, "node_tile": 9, "fused_fail": 10,
-        "fused_fallbacks": 11, "fused_math": 12}
+        "fused_fallbacks": 11, "fused_math": 12, "last_refreshes": 13}
 
 
 class GraError(RuntimeError):
@@ -137,6 +137,7 @@ def load() -> ctypes.CDLL:
         "gra_tune_set": ([vp, ctypes.c_int, ctypes.c_int64], ctypes.c_int),
         "gra_tune_get": ([vp, ctypes.c_int, c_i64p], ctypes.c_int),
         "gra_autotune": ([vp, ctypes.c_int, c_dp, c_dp], ctypes.c_int),
+        "gra_mixed_precision": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

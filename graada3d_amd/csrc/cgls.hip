@@ -65,6 +65,12 @@ struct CgDev {
     int zgroups;
     const double *t2part;     // [nt2] ||t_d||^2 partials of the row blocks
     int64_t nt2;
+    // mixed precision (inner products on a reduced-precision copy of G,
+    // FP64 residual refreshes): b kept for the refresh, and its work vectors
+    double *bvec;             // [nrow] b, or null
+    double *vmk;              // [ld]   mk + P dx  (data rows of the refresh)
+    double *vpdx;             // [nm]   P dx       (regularisation rows)
+    int w_from_t;             // two-pass recurrence: w = Wd t_d (k_t_rows), not Wd s_d
 };
 
 __device__ inline double dgv(const double *d, int64_t i) { return d ? d[i] : 1.0; }
@@ -160,6 +166,54 @@ __global__ void __launch_bounds__(kB) k_b_init(CgDev d)
         }
     }
     d.s[g] = -b;
+    if (d.bvec) d.bvec[g] = b;
+}
+
+// FP64 residual refresh of the mixed-precision iteration, part 1: the vectors
+// the true s = A_t (P dx) - b needs: mk + P dx for the data rows (one product
+// with the FP64 G) and P dx for the regularisation rows.
+__global__ void __launch_bounds__(kB) k_refresh_vec(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (j >= d.ld) return;
+    if (j < d.nm) {
+        const double pdx = d.P[j] * d.dx[j];
+        d.vpdx[j] = pdx;
+        d.vmk[j] = d.mk[j] + pdx;
+    } else {
+        d.vmk[j] = 0.0;
+    }
+}
+
+// Part 2, after y = G (mk + P dx) in d.part_n (FP64 G's split count in
+// d.splits): s = A_t (P dx) - b row by row -- data rows
+// Wd (G (mk + P dx) - dobs), regularisation rows as k_t_rows forms t from
+// P dx, minus b -- and w = Wd s_d for the G^T product of the true residual.
+// The running s and the iterate dx satisfy s = A_t (P dx) - b in exact
+// arithmetic; this replaces the recurrence's s (which carries the
+// reduced-precision products) by its FP64 value.
+__global__ void __launch_bounds__(kB) k_refresh_s(CgDev d)
+{
+    if (d.sc->done) return;
+    int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (g >= d.nrow) return;
+    double sv;
+    if (g < d.nd) {
+        double y = d.part_n[g];
+        for (int q = 1; q < d.splits; ++q) y += d.part_n[(int64_t)q * d.nd + g];
+        sv = dgv(d.Wd, g) * (y - d.dobs[g]);
+        d.w[g] = dgv(d.Wd, g) * sv;
+    } else {
+        int64_t h = g - d.nd;
+        int q = (int)(h / d.nm);
+        int64_t i = h - (int64_t)q * d.nm;
+        double pre;
+        const double *L;
+        t_row_factors(d, q, pre, L);
+        sv = row_dot(d.A[q], i, pre, L, nullptr, d.vpdx) - d.bvec[g];
+    }
+    d.s[g] = sv;
 }
 
 // r_j = P_j (G^T w)_j + sum_q (pre P_j) * AT_q row j * WL_q * seg_q   (A^T v).
@@ -358,6 +412,7 @@ __global__ void __launch_bounds__(kB) k_t_rows(CgDev d)
             t = row_dot(d.A[q], i, pre, L, nullptr, d.Pp);
         }
         d.t[g] = t;
+        if (d.w_from_t && g < d.nd) d.w[g] = dgv(d.Wd, g) * t;
         t2 = t * t;
     }
     double s = block_sum(t2);
@@ -402,7 +457,7 @@ __global__ void __launch_bounds__(kB) k_s_update(CgDev d, int64_t nblk, int it)
     if (g < d.nrow) {
         double s = d.s[g] + a * d.t[g];
         d.s[g] = s;
-        if (g < d.nd) d.w[g] = dgv(d.Wd, g) * s;
+        if (g < d.nd && !d.w_from_t) d.w[g] = dgv(d.Wd, g) * s;
     }
     if (g < d.nm) d.dx[g] = d.dx[g] + a * d.p[g];
 }
@@ -577,21 +632,40 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         return set_error(c, GRA_BAD_ARG, "cross-gradient requested but no T operators uploaded");
     const int nseg = crg ? 7 : 4;
     const int64_t nrow = nd + nseg * nm;
-    const GemvShape shape = gemv_shape(c);
-    const int splits = shape.n_splits;
-    const int chunks = shape.t_chunks;
+    // Mixed precision: the iteration's products use the reduced-precision
+    // copy of the FP64 G (gra_mixed_precision); every `refresh` iterations
+    // and after the last one the true residual is recomputed with the FP64
+    // G, and only those residuals take the stop test.
+    const bool mixed = c.mixed_refresh > 0 && c.Glo.p && c.lo_version == c.g_version && !c.wav &&
+                       c.store == GRA_F64;
+    const int refresh = mixed ? c.mixed_refresh : 0;
+    const gra_dtype st_in = mixed ? c.store_lo : c.store;
+    const GemvShape shape = gemv_shape(c);                 // FP64 (context) G
+    const GemvShape shape_in = mixed ? default_gemv_shape(c.store_lo, nd, nm) : shape;
+    const int splits = std::max(shape.n_splits, shape_in.n_splits);
+    const int chunks = std::max(shape.t_chunks, shape_in.t_chunks);
     hipError_t e;
     FusedPlan plan;
     bool fused = a.fused && !c.wav && !c.fused_failed &&
-                 fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth, c.fused_math,
+                 fused_plan(st_in, nd, ld, c.device, c.fused_wgs, c.fused_depth, c.fused_math,
                             &plan);
+    const void *G_in = mixed ? c.Glo.p : c.G.p;
+    // products of the iteration (reduced-precision copy when mixed)
+    auto prod_n = [&](const double *x, const int *dn) {
+        return mixed ? launch_gemv_n(G_in, st_in, nd, ld, shape_in, x, c.part_n.as<double>(), dn, st)
+                     : launch_G_n(c, shape, x, c.part_n.as<double>(), dn, st);
+    };
+    auto prod_t = [&](const double *w, const int *dn) {
+        return mixed ? launch_gemv_t(G_in, st_in, nd, ld, shape_in, w, c.part_t.as<double>(), dn, st)
+                     : launch_G_t(c, shape, w, c.part_t.as<double>(), dn, st);
+    };
     int *fused_err = c.status.as<int>() + 1;
     if (fused) {
         if ((e = c.fused_scratch.reserve(fused_scratch_bytes(plan, ld))) != hipSuccess)
             return hip_fail(c, e, "alloc fused scratch");
         hipLaunchKernelGGL(k_init_count, dim3(1), dim3(1), 0, st, fused_err);
     }
-    const bool need_rloop = fused && crg && a.quirk_mode;
+    const bool need_rloop = (fused || mixed) && crg && a.quirk_mode;
     if (need_rloop && (e = c.rloop.reserve(sizeof(double) * (size_t)ld)) != hipSuccess)
         return hip_fail(c, e, "alloc rloop");
 
@@ -604,6 +678,12 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         if ((e = c.vec_nm[q].reserve(vnm)) != hipSuccess) return hip_fail(c, e, "alloc vec");
     if ((e = c.vec_nd[0].reserve(sizeof(double) * (size_t)nd)) != hipSuccess)
         return hip_fail(c, e, "alloc w");
+    if (mixed) {
+        if ((e = c.vec_row[2].reserve(need_rows)) != hipSuccess) return hip_fail(c, e, "alloc b");
+        if ((e = c.vec_nm[4].reserve(vnm)) != hipSuccess ||
+            (e = c.vec_nm[5].reserve(vnm)) != hipSuccess)
+            return hip_fail(c, e, "alloc refresh vectors");
+    }
     if (have_Gmk && c.part_n.bytes < sizeof(double) * (size_t)splits * nd) have_Gmk = false;
     if ((e = c.part_n.reserve(sizeof(double) * (size_t)splits * nd)) != hipSuccess)
         return hip_fail(c, e, "alloc part_n");
@@ -670,9 +750,9 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     d.dx = d_dx;
     d.w = c.vec_nd[0].as<double>();
     d.part_n = c.part_n.as<double>();
-    d.splits = splits;
+    d.splits = shape.n_splits;
     d.part_t = c.part_t.as<double>();
-    d.chunks = chunks;
+    d.chunks = shape.t_chunks;
     d.red = c.red.as<double>();
     d.red2 = d.red + nred;
     d.sc = c.scal.as<CgScalars>();
@@ -680,6 +760,11 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     if (fused) {
         d.zgroups = plan.S;
         d.nt2 = (int64_t)plan.S * plan.nrb_group;
+    }
+    if (mixed) {
+        d.bvec = c.vec_row[2].as<double>();
+        d.vmk = c.vec_nm[4].as<double>();
+        d.vpdx = c.vec_nm[5].as<double>();
     }
 
     // zero p, Pp (incl. padding up to ld) and dx
@@ -705,53 +790,104 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg init");
 
     const int *done = &d.sc->done;
+    // the iteration's kernels read the partial sums with its own launch shape
+    CgDev di = d;
+    di.splits = shape_in.n_splits;
+    di.chunks = shape_in.t_chunks;
+    if (mixed && !fused) {
+        // Two-pass form of the recurrence: between FP64 refreshes the
+        // iteration must stay consistent with its own (reduced-precision)
+        // operator, so r <- r + alpha A^T t with G^T (Wd t_d) from the copy,
+        // as the fused iteration does; recomputing r = A^T s from an s that a
+        // refresh made exact mixes the two operators and diverges.
+        di.w_from_t = 1;
+        di.zpart = c.part_t.as<double>();
+        di.zgroups = shape_in.t_chunks;
+    }
+    int n_refresh = 0;
+    // FP64 residual refresh (mixed precision): s = A_t (P dx) - b and
+    // r = A_r^T s with the FP64 G, replacing the recurrence's values; the
+    // next p update ends the iteration from this r.
+    auto refresh_residual = [&]() -> hipError_t {
+        hipLaunchKernelGGL(k_refresh_vec, dim3(nblocks(ld)), dim3(kB), 0, st, d);
+        {
+            TimedRegion tr(c, kTimeRefresh);
+            hipError_t er = launch_G_n(c, shape, d.vmk, c.part_n.as<double>(), done, st);
+            if (er != hipSuccess) return er;
+        }
+        hipLaunchKernelGGL(k_refresh_s, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
+        {
+            TimedRegion tr(c, kTimeRefresh);
+            hipError_t er = launch_G_t(c, shape, d.w, c.part_t.as<double>(), done, st);
+            if (er != hipSuccess) return er;
+        }
+        hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st, d);
+        ++n_refresh;
+        return hipGetLastError();
+    };
+    // stop test: every iteration, or (mixed) only on refreshed residuals
+    auto tol_at = [&](int it) { return !mixed || (it > 0 && it % refresh == 0) ? a.tol : 0.0; };
+    auto refresh_after = [&](int it) { return mixed && ((it + 1) % refresh == 0 || it == a.maxit - 1); };
     if (!fused) {
         for (int it = 0; it < a.maxit; ++it) {
             TimedRegion tr_it(c, kTimeCgIter, it);
-            launch_p_update(d, nm, nblk_r, it, a.tol, st);
+            launch_p_update(di, nm, nblk_r, it, tol_at(it), st);
             {
                 TimedRegion tr(c, kTimeGemvN, it);
-                e = launch_G_n(c, shape, d.Pp, c.part_n.as<double>(), done, st);
+                e = prod_n(di.Pp, done);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
             }
-            hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, di);
             hipLaunchKernelGGL(k_s_update<false>, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0,
-                               st, d, nblk_row, it);
+                               st, di, nblk_row, it);
             {
                 TimedRegion tr(c, kTimeGemvT, it);
-                e = launch_G_t(c, shape, d.w, c.part_t.as<double>(), done, st);
+                e = prod_t(di.w, done);
                 if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
             }
-            hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st, d);
+            if (!mixed)
+                hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
+                                   di);
+            else if (it == 0)
+                hipLaunchKernelGGL((k_r_loop<true, true>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
+                                   di);
+            else
+                hipLaunchKernelGGL((k_r_loop<true, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
+                                   di);
             if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
+            if (refresh_after(it) && (e = refresh_residual()) != hipSuccess)
+                return hip_fail(c, e, "residual refresh");
         }
     } else {
         const int64_t nreg = nrow - nd;
         const int64_t nblk_reg = nblocks(nreg);
         for (int it = 0; it < a.maxit; ++it) {
             TimedRegion tr_it(c, kTimeCgIter, it);
-            launch_p_update(d, nm, nblk_r, it, a.tol, st);
+            launch_p_update(di, nm, nblk_r, it, tol_at(it), st);
             {
                 TimedRegion tr(c, kTimeFused, it);
                 double *zp = nullptr, *t2p = nullptr;
-                e = launch_fused(plan, c.G.p, nd, ld, d.Pp, dg.Wd, d.t, c.fused_scratch.p,
+                e = launch_fused(plan, G_in, nd, ld, di.Pp, dg.Wd, di.t, c.fused_scratch.p,
                                  fused_err, done, c.fused_fail_block, &zp, &t2p, st);
                 if (e != hipSuccess) return hip_fail(c, e, "fused product");
-                d.zpart = zp;
-                d.t2part = t2p;
+                di.zpart = zp;
+                di.t2part = t2p;
             }
-            hipLaunchKernelGGL(k_t_rows_reg, dim3((unsigned)nblk_reg), dim3(kB), 0, st, d);
+            hipLaunchKernelGGL(k_t_rows_reg, dim3((unsigned)nblk_reg), dim3(kB), 0, st, di);
             hipLaunchKernelGGL(k_s_update<true>, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0,
-                               st, d, nblk_reg, it);
+                               st, di, nblk_reg, it);
             if (it == 0)
                 hipLaunchKernelGGL((k_r_loop<true, true>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
-                                   d);
+                                   di);
             else
                 hipLaunchKernelGGL((k_r_loop<true, false>), dim3((unsigned)nblk_r), dim3(kB), 0,
-                                   st, d);
+                                   st, di);
             if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
+            if (refresh_after(it) && (e = refresh_residual()) != hipSuccess)
+                return hip_fail(c, e, "residual refresh");
         }
     }
+    (void)n_refresh;
     if (a.maxit > 0) {
         hipLaunchKernelGGL(k_scalar_end, dim3(1), dim3(kB), 0, st, d, nblk_r, a.maxit - 1, a.tol);
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg end");
@@ -778,7 +914,11 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         return cgls_solve(c, b, d_mk, d_mref, d_dobs, dg, d_dx, err, iterations, false);
     }
     if (a.maxit > 0 && err) *err = hs.err;
-    if (iterations) *iterations = a.maxit > 0 ? (hs.done ? hs.iterations : a.maxit) : 0;
+    const int its_done = a.maxit > 0 ? (hs.done ? hs.iterations : a.maxit) : 0;
+    if (iterations) *iterations = its_done;
+    // refreshes that ran: after every refresh-th iteration and the last one;
+    // a converged solve stops right after one
+    c.last_refreshes = refresh > 0 ? (hs.done ? its_done / refresh : (its_done + refresh - 1) / refresh) : 0;
     if (hs.breakdown)
         return set_error(c, GRA_NUMERIC_BREAKDOWN, "CG breakdown: t2 == 0 or non-finite alpha");
     return GRA_OK;

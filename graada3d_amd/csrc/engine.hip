@@ -304,6 +304,7 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
     HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream), "status");
     HIPCHK(hipStreamSynchronize(c.stream), "assemble");
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    ++c.g_version;
     if (hbad)
         return set_error(c, GRA_DEGENERATE_GEOMETRY,
                          "an observation point lies on a prism edge (s0*s1 <= 0, Integral.cpp:77)");
@@ -363,6 +364,7 @@ gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, 
     }
     HIPCHK(hipStreamSynchronize(c.stream), "assemble");
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    ++c.g_version;
     return GRA_OK;
 }
 
@@ -408,6 +410,7 @@ gra_status gra_assemble_corners(gra_ctx *ctx, const double *prisms, int64_t Nm, 
     }
     HIPCHK(hipStreamSynchronize(c.stream), "assemble");
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    ++c.g_version;
     return GRA_OK;
 }
 
@@ -472,6 +475,7 @@ gra_status gra_assemble_wavelet(gra_ctx *ctx, const double *prisms, int64_t Nm, 
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = (Nm + 63) / 64 * 64; c.flag = field_flag;
     c.store = GRA_F64;
     c.wav = true;
+    ++c.g_version;
     if (hbad)
         return set_error(c, GRA_DEGENERATE_GEOMETRY,
                          "an observation point lies on a prism edge (s0*s1 <= 0, Integral.cpp:77)");
@@ -689,6 +693,7 @@ gra_status assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item 
         cb.ld = tab[(size_t)b].ld;
         cb.flag = items[b].field_flag;
         cb.store = store;
+        ++cb.g_version;
     }
     for (int64_t b = 0; b < n; ++b)
         if (bad[(size_t)b])
@@ -764,6 +769,7 @@ gra_status gra_expand_cols(gra_ctx *ctx, const int64_t *split_cols, int64_t n_sp
     HIPCHK(hipStreamSynchronize(c.stream), "expand");
     std::swap(c.G.p, G2.p);
     std::swap(c.G.bytes, G2.bytes);
+    ++c.g_version;
     c.Nm = Nm2;
     c.ld = ld2;
     c.gn_ready = false;
@@ -972,6 +978,31 @@ gra_status gra_solve_cg(gra_ctx *ctx, const gra_cg_args *a, double *dx, double *
     HIPCHK(hipMemcpyAsync(dx, d_dx, sizeof(double) * nm, hipMemcpyDeviceToHost, c.stream), "dx");
     HIPCHK(hipStreamSynchronize(c.stream), "solve_cg");
     return s;
+}
+
+gra_status gra_mixed_precision(gra_ctx *ctx, gra_dtype inner, int refresh_every)
+{
+    CTX_GUARD(ctx);
+    if (inner == GRA_F64) {
+        c.Glo.release();
+        c.mixed_refresh = 0;
+        c.lo_version = ~0ull;
+        c.store_lo = GRA_F64;
+        return GRA_OK;
+    }
+    if (inner != GRA_F32 && inner != GRA_BF16)
+        return set_error(c, GRA_BAD_ARG, "inner precision: GRA_F64 (off), GRA_F32 or GRA_BF16");
+    if (refresh_every < 1) return set_error(c, GRA_BAD_ARG, "refresh_every must be >= 1");
+    if (!c.G.p || c.Nd == 0 || c.wav || c.store != GRA_F64)
+        return set_error(c, GRA_BAD_ARG, "mixed precision needs a dense FP64 G assembled");
+    const int64_t n = c.Nd * c.ld;
+    HIPCHK(c.Glo.reserve(dtype_size(inner) * (size_t)n), "allocate reduced-precision G");
+    HIPCHK(launch_convert_G(c.G.as<double>(), n, inner, c.Glo.p, c.stream), "convert G");
+    HIPCHK(hipStreamSynchronize(c.stream), "convert G");
+    c.store_lo = inner;
+    c.lo_version = c.g_version;
+    c.mixed_refresh = refresh_every;
+    return GRA_OK;
 }
 
 gra_status gra_barrier_P(gra_ctx *ctx, const double *m, const double *m_min, const double *m_max,
@@ -1242,6 +1273,8 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
             return set_error(c, GRA_BAD_ARG, "fused lag variant: 0 (default) or 1..3");
         c.fused_depth = (int)value;
         break;
+    case GRA_TUNE_LAST_REFRESHES:
+        return set_error(c, GRA_BAD_ARG, "last refreshes is read-only");
     case GRA_TUNE_FUSED_MATH:
         if (value < 0 || value > 2) return set_error(c, GRA_BAD_ARG, "fused math: 0, 1 or 2");
         c.fused_math = (int)value;
@@ -1398,6 +1431,7 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_FUSED_DEPTH: *value = c.fused_depth; break;
     case GRA_TUNE_FUSED_FAIL: *value = c.fused_fail_block; break;
     case GRA_TUNE_FUSED_MATH: *value = c.fused_math; break;
+    case GRA_TUNE_LAST_REFRESHES: *value = c.last_refreshes; break;
     case GRA_TUNE_FUSED_FALLBACKS: *value = c.fused_fail_count; break;
     case GRA_TUNE_N_BLOCK: *value = s.n_block; break;
     case GRA_TUNE_NODE_TILE: *value = c.node_tile; break;

@@ -72,7 +72,7 @@ struct GemvShape {
 // Kernel timing classes (gra_timing_get).
 enum TimingClass {
     kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeFused = 4,
-    kTimeMatMul = 5, kTimeClasses = 6
+    kTimeMatMul = 5, kTimeRefresh = 6, kTimeClasses = 7
 };
 
 // Launch plan of the single-read fused product (fused_kernels.hip).
@@ -100,6 +100,16 @@ struct Ctx {
     int64_t Nd = 0, Nm = 0, N_obs = 0, ld = 0;
     uint32_t flag = 0;
     gra_dtype store = GRA_F64;
+    uint64_t g_version = 0;       // bumped whenever G changes
+
+    // mixed precision (gra_mixed_precision): a reduced-precision copy of the
+    // FP64 G for the CG iteration's products, FP64 residual refreshes every
+    // mixed_refresh iterations
+    DevBuf Glo;
+    gra_dtype store_lo = GRA_F64;
+    uint64_t lo_version = ~0ull;  // g_version the copy was made from
+    int mixed_refresh = 0;        // 0: off
+    int last_refreshes = 0;       // FP64 refreshes of the last solve
 
     // regularisation operators
     DevCsr Ws, Wx, Wy, Wz, Tx0, Ty0, Tz0;
@@ -248,6 +258,8 @@ hipError_t launch_gemv_n(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, co
                          const double *x, double *part, const int *done, hipStream_t st);
 hipError_t launch_gemv_t(const void *G, gra_dtype dt, int64_t Nd, int64_t ld, const GemvShape &s,
                          const double *w, double *part, const int *done, hipStream_t st);
+// FP64 G -> FP32 / BF16 copy with the assembly's rounding (BF16 via FP32).
+hipError_t launch_convert_G(const double *G, int64_t n, gra_dtype dt, void *out, hipStream_t st);
 hipError_t launch_remap_cols(const void *G, int64_t ld, const void *Gc, int64_t ldc,
                              const int32_t *src, int64_t Nd, int64_t Nm2, void *G2, int64_t ld2,
                              int elem_bytes, hipStream_t st);

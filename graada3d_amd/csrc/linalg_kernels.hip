@@ -543,4 +543,35 @@ hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, 
     return hipGetLastError();
 }
 
+
+namespace {
+// One conversion per element, rounding as the assembly stores (prism_kernels.hip):
+// FP32 = round(double), BF16 = round(round(double) to FP32).
+template <typename T> __device__ inline T cvt_store(double v);
+template <> __device__ inline float cvt_store<float>(double v) { return (float)v; }
+template <> __device__ inline __hip_bfloat16 cvt_store<__hip_bfloat16>(double v)
+{
+    return __float2bfloat16((float)v);
+}
+template <typename T>
+__global__ void convert_G_kernel(const double *__restrict__ G, int64_t n, T *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = cvt_store<T>(__builtin_nontemporal_load(G + i));
+}
+}  // namespace
+
+hipError_t launch_convert_G(const double *G, int64_t n, gra_dtype dt, void *out, hipStream_t st)
+{
+    const unsigned grid = 4096;
+    if (dt == GRA_F32)
+        hipLaunchKernelGGL(convert_G_kernel<float>, dim3(grid), dim3(256), 0, st, G, n,
+                           static_cast<float *>(out));
+    else
+        hipLaunchKernelGGL(convert_G_kernel<__hip_bfloat16>, dim3(grid), dim3(256), 0, st, G, n,
+                           static_cast<__hip_bfloat16 *>(out));
+    return hipGetLastError();
+}
+
 }  // namespace gra

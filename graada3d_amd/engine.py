@@ -158,6 +158,7 @@ class Context:
 
 
 TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER, TIME_FUSED, TIME_MAT_MUL = 0, 1, 2, 3, 4, 5
+TIME_REFRESH = 6
 
 
 def csr_struct(m, keep: list) -> GraCsr:
@@ -386,6 +387,15 @@ class GaussNewtonInversion(Fwd):
                                      ctypes.byref(its))
         self.ctx.check(st, "solve_cg")
         return dx, err.value, its.value
+
+    def mixed_precision(self, inner: str = "bf16", refresh_every: int = 10):
+        """Mixed-precision CG on this context's FP64 G (gra_mixed_precision):
+        the iteration's products on an FP32 / BF16 copy, FP64 residual
+        refreshes every `refresh_every` iterations; inner "f64" switches it
+        off."""
+        self._need_G()
+        st = self.ctx.L.gra_mixed_precision(self.ctx.h, _lib.DTYPES[inner], int(refresh_every))
+        self.ctx.check(st, "mixed_precision")
 
     def barrier_P(self, m, m_min, m_max):
         nm = self.Nm

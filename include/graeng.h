@@ -298,6 +298,24 @@ typedef struct gra_cg_args {
 gra_status gra_solve_cg(gra_ctx *ctx, const gra_cg_args *args, double *dx, double *err,
                         int *iterations);
 
+/*
+ * Mixed-precision CG (no counterpart in the reference, whose solve_cg runs
+ * in FP64: GaussNewtonInversion.cpp:1556-1691).  Keeps a copy of the
+ * context's dense FP64 G in `inner` precision (GRA_F32 or GRA_BF16, rounded
+ * as gra_assemble stores them) beside it; from then on every gra_solve_cg
+ * and gra_gn_step on this context runs the CG iteration's G products on the
+ * copy (fused or two-pass as requested) and, every `refresh_every`
+ * iterations and after the last one, replaces the iteration's residuals by
+ * their FP64 values -- s = A (P dx) - b and r = A^T s with the FP64 G -- so
+ * that the solve converges to the FP64 least-squares solution.  The
+ * tolerance is tested on the refreshed residuals only, and the returned err
+ * is the FP64 one.  Products outside the iteration (b, the initial
+ * residual, misfits) use the FP64 G.  GRA_F64 switches it off and frees the
+ * copy; a new G (assembly, expansion) invalidates it (solves then run in
+ * FP64 until it is made again).
+ */
+gra_status gra_mixed_precision(gra_ctx *ctx, gra_dtype inner, int refresh_every);
+
 /* P_ii = (M-m)(m-m_min)/(M-m_min)  (GaussNewtonInversion.cpp:856-861). */
 gra_status gra_barrier_P(gra_ctx *ctx, const double *m, const double *m_min,
                          const double *m_max, double *P);
@@ -381,6 +399,8 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
                                    products with FP64 sums for BF16 storage,
                                    FP64 otherwise), 1 = FP64, 2 = FP32
                                    products (BF16 / FP32 storage)         */
+#define GRA_TUNE_LAST_REFRESHES 13 /* get: FP64 residual refreshes of the last
+                                   mixed-precision solve                    */
 #define GRA_TUNE_FUSED_FALLBACKS 11 /* get: fused solves rerun two-pass on this
                                    context (it then stays two-pass); set 0:
                                    clear and allow the fused iteration again */
@@ -404,7 +424,8 @@ gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t);
  * launches of kernel class `which` since the last gra_timing_reset:
  * 0 = GEMV G*x, 1 = GEMV G^T*x, 2 = G assembly, 3 = one full CG iteration,
  * 4 = fused single-read product (G Pp and G^T Wd^2 G Pp), 5 = the
- * matrix-core multi-right-hand-side product (gra_G_mat_mul / gra_GT_mat_mul).
+ * matrix-core multi-right-hand-side product (gra_G_mat_mul / gra_GT_mat_mul),
+ * 6 = the FP64 G products of the mixed-precision residual refreshes.
  * Timing events are recorded only while enabled: `on` = 0 disables; n >= 1
  * times every launch of these classes, except inside the CG loop, where only
  * iterations 0, n, 2n, ... are timed (each event is a queue packet of its

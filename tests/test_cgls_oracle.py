@@ -177,6 +177,69 @@ def test_quirks_inert_when_weights_are_identity():
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
 
 
+@pytest.mark.parametrize("crg", [False, True])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_quirk_iteration_converges_to_its_operator_pair_solution(crg, seed):
+    """With quirks on, t uses the operator A_t (t_s without WL_s, cross-gradient
+    rows with WL_z/x/y) and r the operator A_r (A^T with WL_s, WL_t*): the
+    converged iterate must solve A_r^T (A_t dx - b) = 0, the dense direct
+    solution of that square system (not the least-squares solution of either
+    operator).  The iteration reaches it to ~1e-11..1e-14 in 200 steps."""
+    p = problem(seed, crg=crg, wl=True)
+    lam = 0.3
+    A_t, b, A_r, _ = stacked(p, lam, quirks=True)
+    At, Ar = np.vstack(A_t), np.vstack(A_r)
+    sol = np.linalg.solve(Ar.T @ At, Ar.T @ b)
+    dx, err, its = run_oracle(p, lam, 200, 1e-30, quirks=True)
+    assert np.linalg.norm(dx - sol) < 1e-9 * np.linalg.norm(sol)
+    # and it is not the consistent operator's least-squares solution
+    ls = np.linalg.lstsq(At, b, rcond=None)[0]
+    assert np.linalg.norm(ls - sol) > 1e-6 * np.linalg.norm(sol)
+
+
+def _coincide(crg):
+    """Random weights where the swapped ones coincide: WL_s = 1 (quirk i,
+    GN.cpp:1573), WL_tz/tx/ty = WL_z/x/y (quirks ii and iii, :1582-1584,
+    :1508 vs :1661)."""
+    p = problem(7, crg=crg, wl=True)
+    WL = p["WL"]
+    WL["s"] = np.ones_like(WL["s"])
+    WL["tz"], WL["tx"], WL["ty"] = WL["z"].copy(), WL["x"].copy(), WL["y"].copy()
+    return p
+
+
+@pytest.mark.parametrize("crg", [False, True])
+def test_quirks_reduce_to_the_consistent_iteration_where_weights_coincide(crg):
+    """Each quirk swaps one weight for another.  Where the swapped weights
+    coincide while every other weight stays random, the quirk iteration is
+    bit-identical to the consistent one (which lstsq pins); which weight the
+    reference uses in each place is fixed only by its source text."""
+    p = _coincide(crg)
+    a = run_oracle(p, 1.0, 12, 1e-30, quirks=True)
+    b = run_oracle(p, 1.0, 12, 1e-30, quirks=False)
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+
+
+@pytest.mark.parametrize("quirk", ["i", "ii", "ii+iii"])
+def test_each_quirk_acts_when_its_weights_differ(quirk):
+    """From the coinciding weights, changing only the weights a quirk swaps
+    makes the quirk iteration differ from the consistent one (and match the
+    dense restatement of the reference's expressions)."""
+    p = _coincide(crg=quirk != "i")
+    WL = p["WL"]
+    if quirk == "i":
+        WL["s"] = WL["s"] * np.linspace(0.7, 1.3, WL["s"].size)
+    elif quirk == "ii":
+        WL["tz"] = WL["tz"] * 1.2
+    else:
+        WL["ty"] = WL["ty"] * 0.8
+    a = run_oracle(p, 1.0, 12, 1e-30, quirks=True)
+    b = run_oracle(p, 1.0, 12, 1e-30, quirks=False)
+    assert np.linalg.norm(a[0] - b[0]) > 1e-6 * np.linalg.norm(b[0])
+    d = dense_cgls(p, 1.0, 12, 1e-30, quirks=True)
+    assert np.linalg.norm(a[0] - d[0]) < 1e-10 * np.linalg.norm(d[0])
+
+
 def test_stops_at_tolerance_and_counts_iterations():
     p = problem(4)
     dx, err, its = run_oracle(p, 1.0, 200, 1e-6, quirks=True)
