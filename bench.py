@@ -2,14 +2,15 @@
 """Benchmark of the engine's hot path on one GPU.
 
 Workloads
-* inversion_small (default): the L2 Gauss-Newton inversion of the
-  40x40x20-cell, 50x50-point gz+Tzz survey -- dense G of 5,000 x 32,000
-  resident in HBM, Ws/Wx/Wy/Wz regularisation with (z+50)^-1.5 depth weighting,
-  lambda_k = 100*0.5^k, 50 CG iterations per outer iteration; default 10
-  steps = the whole inversion.
-* tensor_large: all nine components of the 64x64x32-cell model (62.5 m cells)
-  on a 100x100 grid at z = -50 m -- G of 90,000 x 131,072 (94.4 GB in FP64) --
-  L1 (IRLS, eps^2 = 1e-3) Gauss-Newton, 20 CG iterations per outer iteration.
+* tensor_large (default): all nine components of the 64x64x32-cell model
+  (62.5 m cells) on a 100x100 grid at z = -50 m -- G of 90,000 x 131,072
+  (94.4 GB in FP64) -- L1 (IRLS, eps^2 = 1e-3) Gauss-Newton, lambda_k =
+  100*0.5^k, 20 CG iterations per outer iteration; default 10 steps.
+* inversion_small: the L2 Gauss-Newton inversion of the 40x40x20-cell,
+  50x50-point gz+Tzz survey -- dense G of 5,000 x 32,000 resident in HBM,
+  Ws/Wx/Wy/Wz regularisation with (z+50)^-1.5 depth weighting, lambda_k =
+  100*0.5^k, 50 CG iterations per outer iteration; default 10 steps = the
+  whole inversion.
 
 One step = one Gauss-Newton outer iteration on the GPU (barrier P, solve_cg =
 CGLS with two passes over G per iteration, barrier model update, misfit, IRLS
@@ -18,9 +19,11 @@ shapes are chosen by gra_autotune (timed back-to-back product pairs on the
 resident G; its time is reported, not counted).  `value` is CG iterations
 per second of the plain FP64 path.  The
 same steps are then run with G stored in FP32 and BF16 (FP64 accumulation),
-and with the single-read iteration (one fused pass over G per CG iteration)
-for every storage type, and reported as speed-up and error against the plain
-FP64 run (final model and misfit).  G assembly throughput (entries/s) is
+with the single-read iteration (one fused pass over G per CG iteration) for
+every storage type, and with the single-read iteration over an FP32 / BF16
+copy of the FP64 G plus FP64 residual refreshes ("refined"), and reported
+as speed-up and error against the plain FP64 run (final model, misfit, and
+predicted data per component).  G assembly throughput (entries/s) is
 measured for every storage type.  Kernel rooflines use HIP events on the
 engine's stream around the G products of every TIMING_STRIDE-th CG
 iteration of the timed steps (an event is a queue packet of its own and
@@ -50,6 +53,9 @@ no JSON line):
   FETCH_SIZE / WRITE_SIZE), then the 5,000 x 32,000 G of inversion_small in
   FP64 (and FP32, BF16 with --all-dtypes) with 3 launches each of G x and
   G^T w.
+* gemv-large: the same copy, then the 90,000 x 131,072 FP64 G of
+  tensor_large (assembled by corner sums) with 3 launches each of G x and
+  G^T w.
 * assemble: G of that survey (gz+Tzz), all nine components on a 20x20x10 mesh
   at 40x40 points, the same two G by corner sums, one node-sum forward
   (FP64 instruction counters).
@@ -59,11 +65,11 @@ no JSON line):
 * mat-mul: the matrix-core products with 16 right-hand sides on the
   5,000 x 32,000 G in FP64, FP32 and BF16 (MFMA counters).
 
-    python bench.py [--workload inversion_small|tensor_large|batch|serving|karst] [--steps K]
+    python bench.py [--workload tensor_large|inversion_small|batch|serving|karst] [--steps K]
                     [--warmup W]
                     [--stores f64,f32,bf16] [--no-fused] [--no-cpu] [--gpus 1]
                     [--rate R] [--duration S]
-    python bench.py --probe gemv|assemble|octree|mat-mul [--all-dtypes]
+    python bench.py --probe gemv|gemv-large|assemble|octree|mat-mul [--all-dtypes]
 """
 from __future__ import annotations
 
@@ -86,6 +92,10 @@ HBM_PEAK_GBS = 8000.0
 FLOP_PER_ENTRY = {True: 1165.0, False: 740.0}
 BYTES = {"f64": 8, "f32": 4, "bf16": 2}
 TIMING_STRIDE = 10      # CG iterations 0, 10, 20, ... of each solve carry events
+REFRESH = 10            # refined variants: FP64 residual refresh every 10 CG iterations
+# field bits (gs.h:46-57) and component names, in G's row-block order
+COMPONENTS = [(1, "gz"), (2, "gx"), (3, "gy"), (4, "Tzz"), (5, "Txz"), (6, "Tyz"), (7, "Txx"),
+              (8, "Txy"), (9, "Tyy")]
 
 
 def build_problem(name):
@@ -111,7 +121,7 @@ def build_problem(name):
 
 def run(args):
     from graada3d_amd.engine import (TIME_ASSEMBLE, TIME_FUSED, TIME_GEMV_N, TIME_GEMV_T,
-                                     Context, GaussNewtonInversion)
+                                     TIME_REFRESH, Context, GaussNewtonInversion)
     from graada3d_amd.workloads import add_noise
 
     pb = build_problem(args.workload)
@@ -151,24 +161,32 @@ def run(args):
           f"(face/edge next), G x deviation {max(node_dev):.1e}", file=sys.stderr, flush=True)
 
     results = {}
-    variants = [(s, False) for s in args.stores]
+    # (storage, mode): "" two passes over G per CG iteration, "fused" one
+    # single-read pass, "refined" the single-read pass over a reduced-precision
+    # copy of the FP64 G with FP64 residual refreshes every REFRESH iterations
+    variants = [(s, "") for s in args.stores]
     if args.fused:
-        variants += [(s, True) for s in args.stores]
+        variants += [(s, "fused") for s in args.stores]
+        variants += [(s, "refined") for s in args.stores if s != "f64"]
     asm = {}
     skipped = []
-    for store, fused in variants:
-        key = store + ("_fused" if fused else "")
+    for store, mode in variants:
+        fused = mode != ""
+        key = store + ("_" + mode if mode else "")
         # the context holds one G: assemble it in this variant's storage type
-        # (timed the first time, twice on the small workload)
+        # (timed the first time, twice on the small workload); the refined
+        # variant keeps the FP64 G and makes its reduced-precision copy
         ctx.timing(True)
         ctx.timing_reset()
         reps = 2 if args.workload == "inversion_small" and store not in asm else 1
         for _ in range(reps):
-            inv.compute_G(store)
+            inv.compute_G("f64" if mode == "refined" else store)
         if store not in asm:
             asm[store], _ = ctx.timing_get(TIME_ASSEMBLE)
         ctx.timing(False)
         asm_ms = asm[store]
+        if mode == "refined":
+            inv.mixed_precision(store, REFRESH)
         inv.set_regularization(*pb["W"])
         ctx.tune(n_rows=0, n_unroll=0, n_splits=0, t_chunks=0, t_unroll=0, nontemporal=-1, n_block=-1)
         tune_s = 0.0
@@ -199,14 +217,19 @@ def run(args):
         gn_ms, gn_c = ctx.timing_get(TIME_GEMV_N)
         gt_ms, gt_c = ctx.timing_get(TIME_GEMV_T)
         fu_ms, fu_c = ctx.timing_get(TIME_FUSED)
+        rf_ms, rf_c = ctx.timing_get(TIME_REFRESH)
         ctx.timing(False)
+        if mode == "refined":
+            inv.mixed_precision("f64")
         gbytes = float(nd) * nm * BYTES[store]
 
         def kern(ms, n):
             return dict(avg_ms=ms, launches=n, GBps=gbytes / (ms * 1e-3) / 1e9 if n else None)
 
         results[key] = dict(
-            store=store, fused=fused,
+            store=store, fused=fused, mode=mode or "two-pass",
+            refresh={"every": REFRESH, "fp64_product_avg_ms": rf_ms, "fp64_products": rf_c}
+            if mode == "refined" else None,
             iter_s=its / el, ms_per_step=el / args.steps * 1e3, cg_iters=its,
             misfit=mis[-1], model=inv.gn_model(),
             asm_ms=asm_ms, entries_per_s=nd * nm / (asm_ms * 1e-3),
@@ -225,6 +248,17 @@ def run(args):
         if not fused:
             results[key]["mat_mul"] = mat_mul_rates(ctx, inv, store, gbytes, gn_ms, gt_ms)
 
+    # predicted data of every final model with the FP64 G, per component
+    inv.compute_G("f64")
+    comps = [n for b, n in COMPONENTS if fp.flag & (1 << b)]
+    y_ref = inv.G_vec_mul(results["f64"]["model"])
+    nobs = fp.points.shape[0]
+    for key, r in results.items():
+        y = inv.G_vec_mul(r["model"])
+        r["data_rel_err"] = {c: float(np.linalg.norm(y[k * nobs:(k + 1) * nobs]
+                                                     - y_ref[k * nobs:(k + 1) * nobs])
+                                      / np.linalg.norm(y_ref[k * nobs:(k + 1) * nobs]))
+                             for k, c in enumerate(comps)}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, inv, pb, dobs, Wd)
@@ -672,6 +706,25 @@ def run_probe(args) -> None:
         ctx.close()
         return
 
+    if args.probe == "gemv-large":
+        # the 90,000 x 131,072 FP64 G of tensor_large (assembled by corner
+        # sums, which is quick; the products do not depend on the method)
+        device_copy(1 << 30)
+        print("[probe] 1 GiB device copy done", flush=True)
+        from graada3d_amd.workloads import tensor_problem_large
+        ctx = Context(0)
+        fp = tensor_problem_large()
+        f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
+        f.compute_G("f64", method="nodes")
+        rng = np.random.default_rng(0)
+        x, y = rng.standard_normal(f.Nm), rng.standard_normal(f.Nd)
+        for _ in range(3):
+            f.G_vec_mul(x)
+            f.GT_vec_mul(y)
+        print(f"[probe] f64: 3 x G x, 3 x G^T w on {f.Nd} x {f.Nm}", flush=True)
+        ctx.close()
+        return
+
     if args.probe == "octree":
         from graada3d_amd.octree import OctreeMesh
         ctx = Context(0)
@@ -730,10 +783,25 @@ def run_probe(args) -> None:
     ctx.close()
 
 
+def mixed_summary(mixed, bar=1e-6):
+    """The fastest variant whose final-model and misfit errors are within
+    `bar` of the plain FP64 run, and the fastest variant overall."""
+    ok = {k: v for k, v in mixed.items()
+          if v["model_rel_err"] <= bar and v["misfit_rel_err"] <= bar}
+    pick = lambda d: max(d, key=lambda k: d[k]["speedup_vs_f64"]) if d else None
+    out = {}
+    for name, d in (("fastest_within_1e-6", ok), ("fastest", mixed)):
+        k = pick(d)
+        out[name] = None if k is None else {
+            "variant": k, "speedup_vs_f64": d[k]["speedup_vs_f64"], "iter_s": d[k]["iter_s"],
+            "model_rel_err": d[k]["model_rel_err"], "misfit_rel_err": d[k]["misfit_rel_err"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="inversion_small",
+    ap.add_argument("--workload", default="tensor_large",
                     choices=["inversion_small", "tensor_large", "batch", "serving", "karst"])
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: arrival window, s")
@@ -755,7 +823,8 @@ def main():
                     help="karst: dense G (1) or wavelet-compressed G (2)")
     ap.add_argument("--karst-threshold", type=float, default=5e-4,
                     help="karst: relative threshold of the wavelet compression (method 2)")
-    ap.add_argument("--probe", default=None, choices=["gemv", "assemble", "octree", "mat-mul"],
+    ap.add_argument("--probe", default=None,
+                    choices=["gemv", "gemv-large", "assemble", "octree", "mat-mul"],
                     help="run a fixed launch sequence for rocprofv3 counter passes instead")
     ap.add_argument("--all-dtypes", action="store_true",
                     help="probe gemv: also with G stored in FP32 and BF16")
@@ -778,7 +847,7 @@ def main():
         return
     small = args.workload == "inversion_small"
     if args.steps is None:
-        args.steps = 10 if small else 3
+        args.steps = 10
     if args.warmup is None:
         args.warmup = 2 if small else 1
     args.stores = [s for s in args.stores.split(",") if s]
@@ -798,7 +867,10 @@ def main():
                     "model_rel_err": float(np.linalg.norm(r["model"] - ref["model"])
                                            / np.linalg.norm(ref["model"])),
                     "misfit_rel_err": abs(r["misfit"] - ref["misfit"]) / ref["misfit"],
-                    "g_entries_per_s": r["entries_per_s"]}
+                    "data_rel_err_per_component": r["data_rel_err"],
+                    "mode": r["mode"], "g_entries_per_s": r["entries_per_s"]}
+        if r["refresh"]:
+            mixed[s]["fp64_refresh"] = r["refresh"]
         if r["fused"]:
             mixed[s]["fused_GBps"] = r["fused_product"]["GBps"]
             mixed[s]["fused_avg_ms"] = r["fused_product"]["avg_ms"]
@@ -830,9 +902,10 @@ def main():
             "workload": ("inversion_small: L2 Gauss-Newton on the 40x40x20 mesh, 50x50 gz+Tzz "
                          "points, dense FP64 G 5000x32000 resident in HBM, 50 CGLS iterations per "
                          "step (plain FP64, 2 passes over G per iteration)") if small else
-                        ("tensor_large: L1 IRLS Gauss-Newton on the 64x64x32 mesh, 100x100 points "
-                         "at z=-50 m, all 9 components, dense FP64 G 90000x131072 (94.4 GB) "
-                         "resident in HBM, 20 CGLS iterations per step"),
+                        ("tensor_large: L1 IRLS (eps^2 = 1e-3) Gauss-Newton on the 64x64x32 mesh "
+                         "(62.5 m cells, 12 seeded random boxes), 100x100 points at z=-50 m, all 9 "
+                         "components, dense FP64 G 90000x131072 (94.4 GB) resident in HBM, 20 "
+                         "CGLS iterations per step (plain FP64, 2 passes over G per iteration)"),
             "Nd": nd, "Nm": nm, "cg_iters_per_step": pb["cg_maxit"],
             "lambda_schedule": "100*0.5^k"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -856,9 +929,15 @@ def main():
         "mixed": mixed,
         "mixed_not_run": {k: "fused grid not co-resident for this G (two-pass iteration ran)"
                           for k in skipped},
-        "mixed_error_definition": "||m - m_f64|| / ||m_f64|| of the final model and |misfit - "
-                                  "misfit_f64| / misfit_f64 after the same steps, against the "
-                                  "plain FP64 GPU run (itself checked against the CPU oracle)",
+        "mixed_summary": mixed_summary(mixed),
+        "mixed_error_definition": "||m - m_f64|| / ||m_f64|| of the final model, |misfit - "
+                                  "misfit_f64| / misfit_f64, and per component ||G m - G m_f64|| "
+                                  "/ ||G m_f64|| with the FP64 G, after the same steps, against "
+                                  "the plain FP64 GPU run (itself checked against the CPU "
+                                  "oracle); modes: two-pass (G stored in that type), fused (one "
+                                  "single-read pass per iteration), refined (fused pass over an "
+                                  "FP32/BF16 copy of the FP64 G, FP64 residual refresh every "
+                                  f"{REFRESH} iterations and after the last)",
         "tuning": ref["tuning"],
         "autotune_s": ref["autotune_s"],
         "cpu_baseline": cpu,

@@ -420,20 +420,17 @@ __global__ void __launch_bounds__(kB) k_t_rows(CgDev d)
 }
 
 // alpha = r2 / t2   (GN.cpp:1588-1603) in every thread; block 0 records it.
-// t2 sums the t_rows partials, or for the single-read iteration
-// ||t_d||^2 + ||t_reg||^2, each sum in a fixed order.
+// t2 sums the t_rows partials (for the single-read iteration those of
+// k_t_rows_reg, which fold in the fused product's ||t_d||^2 partials), in a
+// fixed order.
 template <bool FUSED>
 __device__ inline double cg_alpha(const CgDev &d, int64_t nblk, int it)
 {
     __shared__ double a_sh;
-    double t2;
-    if (FUSED) {
-        double td2 = sum_partials(d.t2part, d.nt2);
-        double tr2 = sum_partials(d.red, nblk);
-        t2 = td2 + tr2;
-    } else {
-        t2 = sum_partials(d.red, nblk);
-    }
+    // the single-read iteration's partials already hold ||t_d||^2 slices
+    // (k_t_rows_reg); both forms sum one partial per block of the t kernel
+    (void)FUSED;
+    const double t2 = sum_partials(d.red, nblk);
     if (threadIdx.x == 0) {
         double a = d.sc->r2[it & 1] / t2;
         if (blockIdx.x == 0) {
@@ -490,7 +487,17 @@ __global__ void __launch_bounds__(kB) k_t_rows_reg(CgDev d)
         t2 = t * t;
     }
     double s = block_sum(t2);
-    if (threadIdx.x == 0) d.red[blockIdx.x] = s;
+    // this block's slice of the fused product's ||t_d||^2 partials, so that
+    // alpha sums one partial per block instead of every block re-reading all
+    // of them (fixed slices and order: repeatable)
+    const int64_t per = (d.nt2 + gridDim.x - 1) / gridDim.x;
+    const int64_t k0 = (int64_t)blockIdx.x * per;
+    int64_t k1 = k0 + per;
+    if (k1 > d.nt2) k1 = d.nt2;
+    double v = 0.0;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += kB) v += d.t2part[k];
+    double sd = block_sum(v);
+    if (threadIdx.x == 0) d.red[blockIdx.x] = s + sd;
 }
 
 // T*_scaled = (a*S_i) * T0_ij, and for the transpose (a*S_j) * T0_ji.
