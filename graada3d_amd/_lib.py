@@ -36,7 +36,7 @@ EXPORTS = [
 
 TUNE = {"n_rows": 0, "n_unroll": 1, "n_splits": 2, "t_chunks": 3, "t_unroll": 4, "nontemporal": 5,
         "fused_wgs": 6, "fused_depth": 7, "n_block": 8, "node_tile": 9, "fused_fail": 10,
-        "fused_fallbacks": 11, "fused_math": 12, "last_refreshes": 13}
+        "fused_fallbacks": 11, "fused_math": 12, "last_refreshes": 13, "cg_graph": 14}
 
 
 class GraError(RuntimeError):

@@ -71,6 +71,9 @@ struct CgDev {
     double *vmk;              // [ld]   mk + P dx  (data rows of the refresh)
     double *vpdx;             // [nm]   P dx       (regularisation rows)
     int w_from_t;             // two-pass recurrence: w = Wd t_d (k_t_rows), not Wd s_d
+    // iteration index from sc->it_dev instead of the launch argument (set
+    // for iterations replayed from a captured graph); k_r_loop advances it
+    int dev_it;
 };
 
 __device__ inline double dgv(const double *d, int64_t i) { return d ? d[i] : 1.0; }
@@ -313,6 +316,9 @@ __global__ void __launch_bounds__(kB) k_r_loop(CgDev d)
     }
     double s = block_sum(r2);
     if (threadIdx.x == 0) d.red[blockIdx.x] = s;
+    // the next iteration's kernels read the advanced index (no block of this
+    // kernel reads it)
+    if (d.dev_it && blockIdx.x == 0 && threadIdx.x == 0) d.sc->it_dev += 1;
 }
 
 // After the initial residual: r2, ATb2.
@@ -375,6 +381,7 @@ template <bool FIRST>
 __global__ void __launch_bounds__(kB) k_p_update(CgDev d, int64_t nblk, int it, double tol)
 {
     if (cg_done_block(d)) return;
+    if (d.dev_it) it = d.sc->it_dev;
     double beta;
     if (FIRST) {
         beta = d.sc->beta;
@@ -449,6 +456,7 @@ template <bool FUSED>
 __global__ void __launch_bounds__(kB) k_s_update(CgDev d, int64_t nblk, int it)
 {
     if (d.sc->done) return;
+    if (d.dev_it) it = d.sc->it_dev;
     double a = cg_alpha<FUSED>(d, nblk, it);
     int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (g < d.nrow) {
@@ -836,21 +844,20 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     auto tol_at = [&](int it) { return !mixed || (it > 0 && it % refresh == 0) ? a.tol : 0.0; };
     auto refresh_after = [&](int it) { return mixed && ((it + 1) % refresh == 0 || it == a.maxit - 1); };
     if (!fused) {
-        for (int it = 0; it < a.maxit; ++it) {
-            TimedRegion tr_it(c, kTimeCgIter, it);
+        // one iteration of the two-pass form (timed launches when `timed`)
+        auto two_pass_iteration = [&](int it, bool timed) -> hipError_t {
+            hipError_t er;
             launch_p_update(di, nm, nblk_r, it, tol_at(it), st);
             {
-                TimedRegion tr(c, kTimeGemvN, it);
-                e = prod_n(di.Pp, done);
-                if (e != hipSuccess) return hip_fail(c, e, "gemv_n");
+                TimedRegion tr(c, timed ? kTimeGemvN : -1, it);
+                if ((er = prod_n(di.Pp, done)) != hipSuccess) return er;
             }
             hipLaunchKernelGGL(k_t_rows, dim3(nblocks(nrow)), dim3(kB), 0, st, di);
             hipLaunchKernelGGL(k_s_update<false>, dim3(nblocks(nrow > nm ? nrow : nm)), dim3(kB), 0,
                                st, di, nblk_row, it);
             {
-                TimedRegion tr(c, kTimeGemvT, it);
-                e = prod_t(di.w, done);
-                if (e != hipSuccess) return hip_fail(c, e, "gemv_t");
+                TimedRegion tr(c, timed ? kTimeGemvT : -1, it);
+                if ((er = prod_t(di.w, done)) != hipSuccess) return er;
             }
             if (!mixed)
                 hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
@@ -861,10 +868,54 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
             else
                 hipLaunchKernelGGL((k_r_loop<true, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
                                    di);
-            if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg iteration");
-            if (refresh_after(it) && (e = refresh_residual()) != hipSuccess)
-                return hip_fail(c, e, "residual refresh");
+            return hipGetLastError();
+        };
+        // Captured iterations (plain two-pass form): iteration 0 runs as
+        // launched, then a graph of cg_graph iterations is replayed; the
+        // kernels take the iteration index from device memory, so every
+        // replay continues where the previous one stopped.
+        const int gk = (!mixed && c.cg_graph > 0) ? c.cg_graph : 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        int it0 = 0;
+        if (gk > 0 && a.maxit > gk) {
+            di.dev_it = 1;
+            {
+                TimedRegion tr_it(c, kTimeCgIter, 0);
+                if ((e = two_pass_iteration(0, true)) != hipSuccess) return hip_fail(c, e, "cg iteration");
+            }
+            it0 = 1;
+            const int reps = (a.maxit - 1) / gk;
+            bool captured = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (captured) {
+                hipError_t ec = hipSuccess;
+                for (int j = 0; j < gk && ec == hipSuccess; ++j) ec = two_pass_iteration(1 + j, false);
+                hipError_t ee = hipStreamEndCapture(st, &graph);
+                captured = ec == hipSuccess && ee == hipSuccess && graph &&
+                           hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) == hipSuccess;
+            }
+            if (captured) {
+                for (int r = 0; r < reps; ++r) {
+                    TimedRegion tr(c, kTimeCgGraph);
+                    if ((e = hipGraphLaunch(gexec, st)) != hipSuccess) break;
+                }
+                it0 = 1 + reps * gk;
+            }
+            (void)hipGetLastError();
         }
+        for (int it = it0; it < a.maxit && e == hipSuccess; ++it) {
+            TimedRegion tr_it(c, kTimeCgIter, it);
+            if ((e = two_pass_iteration(it, true)) != hipSuccess) break;
+            if (refresh_after(it) && (e = refresh_residual()) != hipSuccess) break;
+        }
+        if (gexec || graph) {
+            // the replays must finish before the graph is destroyed
+            hipError_t es = hipStreamSynchronize(st);
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (es != hipSuccess && e == hipSuccess) e = es;
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "cg iteration");
     } else {
         const int64_t nreg = nrow - nd;
         const int64_t nblk_reg = nblocks(nreg);

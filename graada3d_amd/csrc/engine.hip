@@ -35,7 +35,7 @@ gra_status hip_fail(Ctx &c, hipError_t e, const char *what)
 
 TimedRegion::TimedRegion(Ctx &ctx, int klass, int iter) : c(ctx), cls(klass)
 {
-    if (c.timing <= 0 || iter % c.timing != 0) return;
+    if (klass < 0 || c.timing <= 0 || iter % c.timing != 0) return;   // -1: untimed
     while (c.ev_used + 2 > c.ev_pool.size()) {
         hipEvent_t ev;
         // timing only: read after a stream synchronisation, so the record
@@ -1275,6 +1275,11 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
         break;
     case GRA_TUNE_LAST_REFRESHES:
         return set_error(c, GRA_BAD_ARG, "last refreshes is read-only");
+    case GRA_TUNE_CG_GRAPH:
+        if (value < 0 || value > 1000)
+            return set_error(c, GRA_BAD_ARG, "CG iterations per graph: 0 (off) .. 1000");
+        c.cg_graph = (int)value;
+        break;
     case GRA_TUNE_FUSED_MATH:
         if (value < 0 || value > 2) return set_error(c, GRA_BAD_ARG, "fused math: 0, 1 or 2");
         c.fused_math = (int)value;
@@ -1432,6 +1437,7 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_FUSED_FAIL: *value = c.fused_fail_block; break;
     case GRA_TUNE_FUSED_MATH: *value = c.fused_math; break;
     case GRA_TUNE_LAST_REFRESHES: *value = c.last_refreshes; break;
+    case GRA_TUNE_CG_GRAPH: *value = c.cg_graph; break;
     case GRA_TUNE_FUSED_FALLBACKS: *value = c.fused_fail_count; break;
     case GRA_TUNE_N_BLOCK: *value = s.n_block; break;
     case GRA_TUNE_NODE_TILE: *value = c.node_tile; break;

@@ -55,7 +55,7 @@ struct CgScalars {
     int done;
     int iterations;
     int breakdown;
-    int pad;
+    int it_dev;       // iteration index for graph-replayed iterations (dev_it)
 };
 
 // Launch shape of the two G products (see linalg_kernels.hip).
@@ -72,7 +72,7 @@ struct GemvShape {
 // Kernel timing classes (gra_timing_get).
 enum TimingClass {
     kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeFused = 4,
-    kTimeMatMul = 5, kTimeRefresh = 6, kTimeClasses = 7
+    kTimeMatMul = 5, kTimeRefresh = 6, kTimeCgGraph = 7, kTimeClasses = 8
 };
 
 // Launch plan of the single-read fused product (fused_kernels.hip).
@@ -164,6 +164,7 @@ struct Ctx {
     int fused_wgs = 0;            // 0: one per CU
     int fused_depth = 0;          // 0: default (lag, prefetch) variant
     int fused_math = 0;           // 0: default per storage type, 1: FP64, 2: FP32 products
+    int cg_graph = 0;             // two-pass CG iterations per captured hipGraph (0: none)
     int64_t fused_fail_block = -1;   // test hook: force a failed wait at this row block
     bool fused_failed = false;    // a fused solve failed on this context: two-pass from now on
     int fused_fail_count = 0;     // fused solves rerun with the two-pass iteration

@@ -401,6 +401,11 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
                                    products (BF16 / FP32 storage)         */
 #define GRA_TUNE_LAST_REFRESHES 13 /* get: FP64 residual refreshes of the last
                                    mixed-precision solve                    */
+#define GRA_TUNE_CG_GRAPH 14    /* two-pass CG iterations captured per hipGraph
+                                   (after the first iteration, the rest are
+                                   graph replays); 0 = every kernel launched
+                                   on its own.  Not used by the fused or the
+                                   mixed-precision iteration.             */
 #define GRA_TUNE_FUSED_FALLBACKS 11 /* get: fused solves rerun two-pass on this
                                    context (it then stays two-pass); set 0:
                                    clear and allow the fused iteration again */
@@ -425,7 +430,8 @@ gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t);
  * 0 = GEMV G*x, 1 = GEMV G^T*x, 2 = G assembly, 3 = one full CG iteration,
  * 4 = fused single-read product (G Pp and G^T Wd^2 G Pp), 5 = the
  * matrix-core multi-right-hand-side product (gra_G_mat_mul / gra_GT_mat_mul),
- * 6 = the FP64 G products of the mixed-precision residual refreshes.
+ * 6 = the FP64 G products of the mixed-precision residual refreshes,
+ * 7 = one replay of the captured CG iterations (GRA_TUNE_CG_GRAPH).
  * Timing events are recorded only while enabled: `on` = 0 disables; n >= 1
  * times every launch of these classes, except inside the CG loop, where only
  * iterations 0, n, 2n, ... are timed (each event is a queue packet of its
