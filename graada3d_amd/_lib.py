@@ -32,6 +32,7 @@ EXPORTS = [
     "gra_barrier_update", "gra_misfit", "gra_irls_weights", "gra_gn_setup_problem",
     "gra_gn_step", "gra_gn_get_model", "gra_timing_enable", "gra_timing_reset",
     "gra_timing_get", "gra_tune_set", "gra_tune_get", "gra_autotune", "gra_mixed_precision",
+    "gra_G_vec_mul_noalloc",
 ]
 
 TUNE = {"n_rows": 0, "n_unroll": 1, "n_splits": 2, "t_chunks": 3, "t_unroll": 4, "nontemporal": 5,
@@ -138,6 +139,7 @@ def load() -> ctypes.CDLL:
         "gra_tune_get": ([vp, ctypes.c_int, c_i64p], ctypes.c_int),
         "gra_autotune": ([vp, ctypes.c_int, c_dp, c_dp], ctypes.c_int),
         "gra_mixed_precision": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "gra_G_vec_mul_noalloc": ([vp, vp, vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -216,6 +216,45 @@ const char *gra_status_string(int s)
     }
 }
 
+// ------------------------------------------- allocation-free G x (scratch)
+// gra_G_vec_mul_noalloc stages x and keeps the column-split partial sums of
+// G x in this array instead of context buffers.  Each GPU holds its own copy
+// (32 MiB) from the module load until the process ends; host code reaches it
+// only through the device address hipGetSymbolAddress returns for the
+// current device, resolved once per device when a context is created.
+namespace gra {
+namespace {
+constexpr int64_t kScratchDoubles = int64_t(4) << 20;     // 32 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex scratch_mu;
+double *scratch_dev[64] = {};      // per device ordinal, null until resolved
+
+// With `device` current: the device address of its `scratch`.
+hipError_t scratch_address(int device, double **out)
+{
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_dev[device] = static_cast<double *>(p);
+    }
+    *out = scratch_dev[device];
+    return hipSuccess;
+}
+
+// out[i] = x[i] for i < n, 0 up to ld (the G x kernels read ld entries).
+__global__ void k_stage_padded(const double *__restrict__ x, int64_t n, int64_t ld,
+                               double *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ld) out[i] = i < n ? x[i] : 0.0;
+}
+}  // namespace
+}  // namespace gra
+
 const char *gra_last_error(const gra_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : ""; }
 
 gra_status gra_ctx_create(int device, gra_ctx **out)
@@ -238,6 +277,14 @@ gra_status gra_ctx_create(int device, gra_ctx **out)
         delete ctx;
         return GRA_OOM;
     }
+    // the allocation-free entry point's scratch on this device (see above)
+    if (scratch_address(device, &ctx->c.noalloc_scratch) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return GRA_HIP_ERROR;
+    }
+    // error messages of short length then need no allocation
+    ctx->c.last_error.reserve(256);
     *out = ctx;
     return GRA_OK;
 }
@@ -807,6 +854,27 @@ gra_status gra_col_norms(gra_ctx *ctx, double *norms)
 gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product)
 {
     return gemv_host(ctx, 0, vec, product);
+}
+
+gra_status gra_G_vec_mul_noalloc(gra_ctx *ctx, const double *d_x, double *d_y)
+{
+    CTX_GUARD(ctx);
+    if (!d_x || !d_y) return set_error(c, GRA_BAD_ARG, "null vector");
+    if (c.wav || !c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no dense G assembled");
+    const GemvShape shape = gemv_shape(c);
+    // staged x (ld) + the split partial sums (n_splits x Nd)
+    const int64_t need = c.ld + (int64_t)shape.n_splits * c.Nd;
+    if (need > kScratchDoubles)
+        return set_error(c, GRA_BAD_ARG, "G too large for the allocation-free product (needs "
+                                         "ld + splits*Nd <= 4194304 doubles of scratch)");
+    double *xs = c.noalloc_scratch, *part = c.noalloc_scratch + c.ld;
+    hipLaunchKernelGGL(k_stage_padded, dim3((unsigned)((c.ld + 255) / 256)), dim3(256), 0,
+                       c.stream, d_x, c.Nm, c.ld, xs);
+    HIPCHK(hipGetLastError(), "stage x");
+    HIPCHK(launch_gemv_n(c.G.p, c.store, c.Nd, c.ld, shape, xs, part, nullptr, c.stream), "G x");
+    HIPCHK(launch_reduce_parts(part, shape.n_splits, c.Nd, c.Nd, nullptr, d_y, c.stream),
+           "reduce");
+    return GRA_OK;
 }
 
 gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product)

@@ -129,6 +129,7 @@ struct Ctx {
     DevBuf red;                // block partial sums for reductions
     DevBuf scal;               // CgScalars
     DevBuf status;             // int flags written by kernels
+    double *noalloc_scratch = nullptr;   // this device's `scratch` (gra_G_vec_mul_noalloc)
     DevBuf host_io;            // unused placeholder
 
     // wavelet-compressed G (gra_assemble_wavelet): replaces the dense G while

@@ -184,6 +184,21 @@ gra_status gra_copy_G(gra_ctx *ctx, double *out, int col_major);
 gra_status gra_G_vec_mul(gra_ctx *ctx, const double *vec, double *product);
 /* Replaces Fwd::GT_vec_mul (Fwd.cpp:80-113): product[Nm] = G^T * vec[Nd]. */
 gra_status gra_GT_vec_mul(gra_ctx *ctx, const double *vec, double *product);
+/*
+ * No counterpart in the reference: G x for callers that cannot allocate
+ * device memory (code under hipGraph stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and launch shape as
+ * gra_G_vec_mul, but device pointers in and out (d_x: Nm doubles, d_y: Nd
+ * doubles) and no allocation of any kind: x is staged and the partial sums
+ * kept in a fixed 32 MiB per-GPU scratch array inside the library, so G must
+ * satisfy ld + splits * Nd <= 4,194,304 (ld = Nm rounded up to 64; splits as
+ * gra_tune_get(GRA_TUNE_N_SPLITS) reports), else GRA_BAD_ARG.  Dense G only.
+ * Asynchronous on the context's stream (gra_sync waits for d_y).  Every call
+ * on one GPU uses that GPU's single scratch array: calls that overlap on one
+ * GPU (contexts on different streams, or host threads) race and must be
+ * serialised by the caller; calls on one context are ordered by its stream.
+ */
+gra_status gra_G_vec_mul_noalloc(gra_ctx *ctx, const double *d_x, double *d_y);
 
 /*
  * k right-hand sides at once on the matrix cores: Y = G X (X: Nm x k, Y:
