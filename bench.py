@@ -880,11 +880,12 @@ def main():
     dom = "gemv_t" if ref["gemv_t"]["avg_ms"] >= ref["gemv_n"]["avg_ms"] else "gemv_n"
     achieved = ref[dom]["GBps"]
     # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs
-    # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
-    # measured for the 5000 x 32000 FP64 G only.
-    traffic = None
+    # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy;
+    # counter passes over --probe gemv and --probe gemv-large)
     if small:
         traffic = {"gemv_t": 1280.6e6 + 0.8e6, "gemv_n": 1282.1e6 + 0.04e6}[dom] / 1e9
+    else:
+        traffic = {"gemv_t": 94.384e9, "gemv_n": 94.382e9}[dom] / 1e9
     out = {
         "metric": METRIC,
         "value": ref["iter_s"],
