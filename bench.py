@@ -68,7 +68,8 @@ no JSON line):
     python bench.py [--workload tensor_large|inversion_small|batch|serving|karst] [--steps K]
                     [--warmup W]
                     [--stores f64,f32,bf16] [--no-fused] [--no-cpu] [--gpus 1]
-                    [--rate R] [--duration S]
+                    [--rate R] [--duration S] [--budget-gb B] [--policy gdsf|lru]
+                    [--max-batch K] [--workers W]
     python bench.py --probe gemv|gemv-large|assemble|octree|mat-mul [--all-dtypes]
 """
 from __future__ import annotations
@@ -93,6 +94,7 @@ FLOP_PER_ENTRY = {True: 1165.0, False: 740.0}
 BYTES = {"f64": 8, "f32": 4, "bf16": 2}
 TIMING_STRIDE = 10      # CG iterations 0, 10, 20, ... of each solve carry events
 REFRESH = 10            # refined variants: FP64 residual refresh every 10 CG iterations
+BATCH_MIN_BLOCKS = 24   # batch: single-read CG where each row group gets >= 24 row blocks
 # field bits (gs.h:46-57) and component names, in G's row-block order
 COMPONENTS = [(1, "gz"), (2, "gx"), (3, "gy"), (4, "Tzz"), (5, "Txz"), (6, "Tyz"), (7, "Txx"),
               (8, "Txy"), (9, "Tyy")]
@@ -404,32 +406,64 @@ def run_batch(args):
     # batched inversion: per instance, L0 (IRLS, eps^2 = 1e-2) or L2 Gauss-Newton
     # on its octree operators, half of them with the cross-gradient term; the
     # host-side operator build and uploads are not timed
+    from graada3d_amd.engine import gn_step_batch
     gn_steps, cg_its = 3, 20
     c0 = time.perf_counter()
+    setups = []
     for b, f in zip(inst, fwds):
         W, T = batch_regularisation(b)
         f.set_regularization(*W, *(T or ()))
         ncomp = f.Nd // f.N_obs
         dobs, sigma = add_noise(f.G_vec_mul(b.rho), f.N_obs, ncomp, b.seed)
         nm = f.Nm
-        f.gn_setup(dobs, np.repeat(1.0 / sigma, f.N_obs), np.full(nm, -1000.0),
-                   np.full(nm, 1000.0), np.zeros(nm), np.zeros(nm), lp=b.lp,
-                   eps2=1e-2 if b.lp != 2 else 0.0, cross_gradient=b.cross_gradient, a_crg=1.0)
+        setups.append((dobs, np.repeat(1.0 / sigma, f.N_obs), np.full(nm, -1000.0),
+                       np.full(nm, 1000.0), np.zeros(nm), np.zeros(nm)))
     setup_s = time.perf_counter() - c0
-    for f in fwds:
-        f.ctx.sync()
-    c0 = time.perf_counter()
-    total_its, mis_drop = 0, []
-    for b, f in zip(inst, fwds):
-        first = None
-        for k in range(gn_steps):
-            mis, err, its = f.gn_step(100.0 * 0.5 ** k, 0.0, cg_its)
-            total_its += its
-            first = mis if first is None else first
-        mis_drop.append(mis / first)
-    t_inv = time.perf_counter() - c0
-    print(f"[bench] batch inversion: {total_its} CG iterations in {t_inv:.2f} s "
-          f"(operator setup {setup_s:.1f} s)", file=sys.stderr, flush=True)
+
+    def gn_setup_all(fused):
+        for b, f, a in zip(inst, fwds, setups):
+            f.gn_setup(*a, lp=b.lp, eps2=1e-2 if b.lp != 2 else 0.0, fused=fused,
+                       cross_gradient=b.cross_gradient, a_crg=1.0)
+        for f in fwds:
+            f.ctx.sync()
+
+    lams = [[100.0 * 0.5 ** k] * len(fwds) for k in range(gn_steps)]
+    runs = {}
+    # (a) instance after instance, one host synchronisation per step
+    # (b) gra_gn_step_batch: each step of all instances in one submission,
+    #     two-pass CG (instances overlap) and single-read CG (chained)
+    for name, fused, batched in (("sequential_two_pass", False, False),
+                                 ("batched_two_pass", False, True),
+                                 ("batched_single_read", True, True)):
+        gn_setup_all(fused)
+        for f in fwds:
+            # single-read only where each row group gets enough row blocks
+            # (small instances stay two-pass)
+            f.ctx.tune(fused_min_blocks=BATCH_MIN_BLOCKS if fused else 0)
+        c0 = time.perf_counter()
+        total_its, models = 0, None
+        if batched:
+            for k in range(gn_steps):
+                mis, err, its = gn_step_batch(fwds, lams[k], 0.0, cg_its)
+                total_its += int(its.sum())
+        else:
+            for f in fwds:
+                for k in range(gn_steps):
+                    total_its += f.gn_step(lams[k][0], 0.0, cg_its)[2]
+        t_run = time.perf_counter() - c0
+        runs[name] = {"seconds": t_run, "cg_iterations": total_its,
+                      "cg_iter_per_s": total_its / t_run,
+                      "G_TBps": gn_steps * cg_its * (1 if fused else 2) *
+                      sum(f.Nd * f.Nm * 8.0 for f in fwds) / t_run / 1e12}
+        if name == "sequential_two_pass":
+            ref_models = [f.gn_model() for f in fwds]
+        elif name == "batched_two_pass":
+            runs[name]["bit_identical_to_sequential"] = all(
+                np.array_equal(f.gn_model(), m) for f, m in zip(fwds, ref_models))
+        print(f"[bench] batch inversion {name}: {total_its} CG iterations in {t_run:.2f} s",
+              file=sys.stderr, flush=True)
+    t_inv = runs["sequential_two_pass"]["seconds"]
+    total_its = runs["sequential_two_pass"]["cg_iterations"]
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -468,8 +502,7 @@ def run_batch(args):
                             gn_steps * cg_its / t_inv,
                             "L0": int(sum(b.lp == 0 for b in inst)),
                             "cross_gradient": int(sum(b.cross_gradient for b in inst)),
-                            "misfit_ratio_last_first_median": float(np.median(mis_drop)),
-                            "host_setup_s": setup_s},
+                            "host_setup_s": setup_s, "runs": runs},
         "roofline": {"bound": "fp64-valu", "achieved": None, "peak": 78.6, "unit": "TFLOP/s",
                      "frac": None, "traffic": None,
                      "note": "transcendental-bound FP64 evaluation; HBM writes are "
@@ -485,10 +518,12 @@ def run_serving(args):
     from graada3d_amd import serving as S
     pool = S.default_cg_pool()
     reqs = S.request_stream(args.rate, args.duration, args.seed, pool)
-    print(f"[bench] serving: {len(reqs)} requests over {args.duration} s at {args.rate}/s",
-          file=sys.stderr, flush=True)
-    res = S.serve(reqs, pool, progress=50, forward_method=args.forward_method,
-                  workers=args.workers)
+    print(f"[bench] serving: {len(reqs)} requests over {args.duration} s at {args.rate}/s, "
+          f"{len(pool)} CG problems ({sum(S.ResidentCache.g_bytes(p) for p in pool) / 1e9:.0f} "
+          f"GB of G) under a {args.budget_gb:g} GB budget", file=sys.stderr, flush=True)
+    res = S.serve(reqs, pool, budget_bytes=args.budget_gb * 1e9,
+                  progress=max(50, int(args.rate * 10)), forward_method=args.forward_method,
+                  workers=args.workers, max_batch=args.max_batch, policy=args.policy)
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -510,10 +545,11 @@ def run_serving(args):
                          f"{pts.shape[0]} points ({c1 - c0:.1f} s), extrapolated to the whole "
                          f"request; GPU: {res['by_kind']['forward']['service_ms']['mean']:.1f} ms "
                          f"mean forward service time", "all_cores": ac}
+    fwd = res["by_kind"]["forward"]["latency_ms"]
     out = {
         "metric": METRIC,
-        "value": res["latency_ms"]["p99"],
-        "unit": "ms (p99 request latency)",
+        "value": fwd["p99"],
+        "unit": "ms (p99 forward request latency)",
         "n_gpus": 1, "steps": res["requests"], "warmup": 0,
         "ms_per_step": res["service_ms"]["mean"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -521,18 +557,34 @@ def run_serving(args):
         "config": {"workload": f"serving: Poisson {args.rate:g} req/s for {args.duration:g} s "
                                f"(seed {args.seed}), 80% matrix-free forward (5k-130k cells, "
                                f"1k-20k points, 1-9 comps, FP32 out, {args.forward_method} "
-                               f"form), 20% 20-iteration CGLS on "
-                               f"resident G (pool: " + ", ".join(p.name for p in pool) + "), "
+                               f"form, up to {args.max_batch} per launch), 20% 20-iteration "
+                               f"CGLS on resident G (pool: inversion_small 45%, the 64 octree "
+                               f"instances 54%, tensor_large FP32 single-read 1%; "
+                               f"{res['cache']['pool_gb']:.0f} GB of G under a "
+                               f"{args.budget_gb:g} GB {args.policy} budget), "
                                f"{args.workers} server streams"},
         "latency_ms": res["latency_ms"], "service_ms": res["service_ms"],
         "by_kind": res["by_kind"], "cg_warm_service_ms": res["cg_warm_service_ms"],
+        "cg_by_problem": {k: v for k, v in res["cg_by_problem"].items()
+                          if k in {"inversion_small", "tensor_large"} | _serving_extremes(pool)},
+        "throughput_req_per_s": res["achieved_rate"],
         "offered_rate": res["offered_rate"], "achieved_rate": res["achieved_rate"],
-        "sustainable_rate": res["sustainable_rate"], "utilization": res["utilization"],
+        "utilization": res["utilization"],
         "mean_in_service": res["mean_in_service"], "workers": res["workers"],
-        "cache": res["cache"],
+        "forward_batches": res["forward_batches"], "cache": res["cache"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _serving_extremes(pool):
+    """Names of the smallest and largest octree instances of the pool."""
+    from graada3d_amd import serving as S
+    inst = [p for p in pool if p.instance >= 0]
+    if not inst:
+        return set()
+    by = sorted(inst, key=S.ResidentCache.g_bytes)
+    return {by[0].name, by[len(by) // 2].name, by[-1].name}
 
 
 def stage_karst(root, key="interpolated"):
@@ -808,6 +860,12 @@ def main():
     ap.add_argument("--seed", type=int, default=20240605, help="serving: stream seed")
     ap.add_argument("--workers", type=int, default=3,
                     help="serving: server threads, each with its own stream")
+    ap.add_argument("--budget-gb", type=float, default=100.0,
+                    help="serving: bytes of resident G the cache may hold (GB)")
+    ap.add_argument("--policy", default="gdsf", choices=["gdsf", "lru"],
+                    help="serving: eviction policy of the resident-G cache")
+    ap.add_argument("--max-batch", type=int, default=8,
+                    help="serving: forward requests per shared launch (1: no batching)")
     ap.add_argument("--forward-method", default="nodes", choices=["nodes", "prisms"],
                     help="serving: node-sum (rectilinear) or per-prism face/edge forward")
     ap.add_argument("--steps", type=int, default=None)

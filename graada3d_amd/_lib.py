@@ -24,6 +24,7 @@ DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
 
 EXPORTS = [
     "gra_status_string", "gra_last_error", "gra_ctx_create", "gra_ctx_destroy", "gra_sync",
+    "gra_ctx_set_stream", "gra_ctx_stream",
     "gra_assemble", "gra_get_dims", "gra_copy_G", "gra_G_vec_mul", "gra_GT_vec_mul",
     "gra_col_norms", "gra_expand_cols", "gra_assemble_batch", "gra_assemble_batch_corners",
     "gra_assemble_wavelet",
@@ -32,12 +33,12 @@ EXPORTS = [
     "gra_barrier_update", "gra_misfit", "gra_irls_weights", "gra_gn_setup_problem",
     "gra_gn_step", "gra_gn_get_model", "gra_timing_enable", "gra_timing_reset",
     "gra_timing_get", "gra_tune_set", "gra_tune_get", "gra_autotune", "gra_mixed_precision",
-    "gra_G_vec_mul_noalloc",
+    "gra_G_vec_mul_noalloc", "gra_gn_step_batch", "gra_forward_rectilinear_batch",
 ]
 
 TUNE = {"n_rows": 0, "n_unroll": 1, "n_splits": 2, "t_chunks": 3, "t_unroll": 4, "nontemporal": 5,
         "fused_wgs": 6, "fused_depth": 7, "n_block": 8, "node_tile": 9, "fused_fail": 10,
-        "fused_fallbacks": 11, "fused_math": 12, "last_refreshes": 13, "cg_graph": 14}
+        "fused_fallbacks": 11, "fused_math": 12, "last_refreshes": 13, "cg_graph": 14, "fused_min_blocks": 15}
 
 
 class GraError(RuntimeError):
@@ -54,6 +55,12 @@ class GraCsr(ctypes.Structure):
 class GraBatchItem(ctypes.Structure):
     _fields_ = [("prisms", c_dp), ("Nm", ctypes.c_int64), ("pts", c_dp), ("N_obs", ctypes.c_int64),
                 ("field_flag", ctypes.c_uint32)]
+
+
+class GraFwdRectItem(ctypes.Structure):
+    _fields_ = [("xs", c_dp), ("nx", ctypes.c_int64), ("ys", c_dp), ("ny", ctypes.c_int64),
+                ("zs", c_dp), ("nz", ctypes.c_int64), ("rho", c_dp), ("pts", c_dp),
+                ("N_obs", ctypes.c_int64), ("field_flag", ctypes.c_uint32), ("out", c_dp)]
 
 
 class GraCgArgs(ctypes.Structure):
@@ -91,6 +98,8 @@ def load() -> ctypes.CDLL:
         "gra_ctx_create": ([ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
         "gra_ctx_destroy": ([vp], None),
         "gra_sync": ([vp], ctypes.c_int),
+        "gra_ctx_set_stream": ([vp, vp], ctypes.c_int),
+        "gra_ctx_stream": ([vp], vp),
         "gra_assemble": ([vp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64, ctypes.c_uint32,
                           ctypes.c_int], ctypes.c_int),
         "gra_assemble_wavelet": ([vp, c_dp, ctypes.c_int64, c_dp, ctypes.c_int64,
@@ -140,6 +149,11 @@ def load() -> ctypes.CDLL:
         "gra_autotune": ([vp, ctypes.c_int, c_dp, c_dp], ctypes.c_int),
         "gra_mixed_precision": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "gra_G_vec_mul_noalloc": ([vp, vp, vp], ctypes.c_int),
+        "gra_forward_rectilinear_batch": ([vp, ctypes.c_int64, ctypes.POINTER(GraFwdRectItem)],
+                                          ctypes.c_int),
+        "gra_gn_step_batch": ([ctypes.POINTER(vp), ctypes.c_int64, c_dp, ctypes.c_double,
+                               ctypes.c_int, c_dp, c_dp, ctypes.POINTER(ctypes.c_int)],
+                              ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

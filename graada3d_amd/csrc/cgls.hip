@@ -638,7 +638,7 @@ inline void launch_p_update(const CgDev &d, int64_t nm, int64_t nblk_r, int it, 
 
 gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const double *d_mref,
                       const double *d_dobs, const CgDiag &dg, double *d_dx, double *err,
-                      int *iterations, bool have_Gmk)
+                      int *iterations, bool have_Gmk, bool defer)
 {
     hipStream_t st = c.stream;
     const int64_t nd = c.Nd, nm = c.Nm, ld = c.ld;
@@ -663,7 +663,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     FusedPlan plan;
     bool fused = a.fused && !c.wav && !c.fused_failed &&
                  fused_plan(st_in, nd, ld, c.device, c.fused_wgs, c.fused_depth, c.fused_math,
-                            &plan);
+                            c.fused_min_blocks, &plan);
     const void *G_in = mixed ? c.Glo.p : c.G.p;
     // products of the iteration (reduced-precision copy when mixed)
     auto prod_n = [&](const double *x, const int *dn) {
@@ -951,28 +951,49 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg end");
     }
 
-    CgScalars hs;
-    int ferr = 0;
-    if ((e = hipMemcpyAsync(&hs, d.sc, sizeof(hs), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    // scalars (and the fused product's status) to pinned host staging
+    HostStage *hst = c.host_stage;
+    if ((e = hipMemcpyAsync(&hst->sc, d.sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st)) !=
+        hipSuccess)
         return hip_fail(c, e, "copy scalars");
-    if (fused && (e = hipMemcpyAsync(&ferr, fused_err, sizeof(int), hipMemcpyDeviceToHost, st)) !=
-                     hipSuccess)
+    hst->ferr = 0;
+    if (fused && (e = hipMemcpyAsync(&hst->ferr, fused_err, sizeof(int), hipMemcpyDeviceToHost,
+                                     st)) != hipSuccess)
         return hip_fail(c, e, "copy status");
+    c.pend_fused = fused;
+    c.pend_maxit = a.maxit;
+    c.pend_refresh = refresh;
+    if (defer) return GRA_OK;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(c, e, "cg sync");
-    c.last_solve_fused = fused;
-    if (ferr) {
+    bool rerun = false;
+    gra_status s = cgls_collect(c, err, iterations, &rerun);
+    if (rerun) {
         // A wait of the fused product failed (bounded wait or the test hook):
         // its result is discarded and the same solve reruns with the two-pass
         // iteration from the unchanged inputs; the context stays on the
         // two-pass iteration from now on.
-        c.fused_failed = true;
-        c.fused_fail_count += 1;
         gra_cg_args b = a;
         b.fused = 0;
-        return cgls_solve(c, b, d_mk, d_mref, d_dobs, dg, d_dx, err, iterations, false);
+        return cgls_solve(c, b, d_mk, d_mref, d_dobs, dg, d_dx, err, iterations, false, false);
     }
-    if (a.maxit > 0 && err) *err = hs.err;
-    const int its_done = a.maxit > 0 ? (hs.done ? hs.iterations : a.maxit) : 0;
+    return s;
+}
+
+gra_status cgls_collect(Ctx &c, double *err, int *iterations, bool *rerun)
+{
+    const HostStage *hst = c.host_stage;
+    const CgScalars &hs = hst->sc;
+    c.last_solve_fused = c.pend_fused;
+    *rerun = false;
+    if (c.pend_fused && hst->ferr) {
+        c.fused_failed = true;
+        c.fused_fail_count += 1;
+        *rerun = true;
+        return GRA_OK;
+    }
+    const int maxit = c.pend_maxit, refresh = c.pend_refresh;
+    if (maxit > 0 && err) *err = hs.err;
+    const int its_done = maxit > 0 ? (hs.done ? hs.iterations : maxit) : 0;
     if (iterations) *iterations = its_done;
     // refreshes that ran: after every refresh-th iteration and the last one;
     // a converged solve stops right after one

@@ -268,23 +268,32 @@ gra_status gra_ctx_create(int device, gra_ctx **out)
     gra_ctx *ctx = new (std::nothrow) gra_ctx();
     if (!ctx) return GRA_OOM;
     ctx->c.device = device;
-    if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&ctx->c.own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return GRA_HIP_ERROR;
     }
+    ctx->c.stream = ctx->c.own_stream;
     if (ctx->c.status.reserve(sizeof(int) * 4) != hipSuccess) {
-        (void)hipStreamDestroy(ctx->c.stream);
+        (void)hipStreamDestroy(ctx->c.own_stream);
         delete ctx;
         return GRA_OOM;
     }
     // the allocation-free entry point's scratch on this device (see above)
     if (scratch_address(device, &ctx->c.noalloc_scratch) != hipSuccess) {
-        (void)hipStreamDestroy(ctx->c.stream);
+        (void)hipStreamDestroy(ctx->c.own_stream);
         delete ctx;
         return GRA_HIP_ERROR;
     }
     // error messages of short length then need no allocation
     ctx->c.last_error.reserve(256);
+    // per-solve results come back through pinned memory, so that result
+    // copies queued on the stream do not block the host
+    if (hipHostMalloc((void **)&ctx->c.host_stage, sizeof(HostStage), hipHostMallocDefault) !=
+        hipSuccess) {
+        (void)hipStreamDestroy(ctx->c.own_stream);
+        delete ctx;
+        return GRA_OOM;
+    }
     *out = ctx;
     return GRA_OK;
 }
@@ -298,10 +307,22 @@ void gra_ctx_destroy(gra_ctx *ctx)
         (void)hipStreamSynchronize(c.stream);
         for (hipEvent_t ev : c.ev_pool) (void)hipEventDestroy(ev);
         c.ev_pool.clear();
-        (void)hipStreamDestroy(c.stream);
+        if (c.batch_event) (void)hipEventDestroy(c.batch_event);
+        if (c.host_stage) (void)hipHostFree(c.host_stage);
+        (void)hipStreamDestroy(c.own_stream);
     }
     delete ctx;
 }
+
+gra_status gra_ctx_set_stream(gra_ctx *ctx, void *stream)
+{
+    CTX_GUARD(ctx);
+    HIPCHK(hipStreamSynchronize(c.stream), "sync before a stream change");
+    c.stream = stream ? static_cast<hipStream_t>(stream) : c.own_stream;
+    return GRA_OK;
+}
+
+void *gra_ctx_stream(const gra_ctx *ctx) { return ctx ? ctx->c.stream : nullptr; }
 
 gra_status gra_sync(gra_ctx *ctx)
 {
@@ -969,6 +990,101 @@ gra_status gra_forward_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, c
     return GRA_OK;
 }
 
+gra_status gra_forward_rectilinear_batch(gra_ctx *ctx, int64_t n, const gra_fwd_rect_item *items)
+{
+    CTX_GUARD(ctx);
+    if (!items || n <= 0 || n > 4096) return set_error(c, GRA_BAD_ARG, "items: 1..4096");
+    int64_t n_axes = 0, n_rho = 0, n_pts = 0, n_w = 0, n_out = 0, blocks = 0;
+    for (int64_t b = 0; b < n; ++b) {
+        const gra_fwd_rect_item &it = items[b];
+        const std::string tag = "item " + std::to_string(b) + ": ";
+        if (!it.xs || !it.ys || !it.zs || !it.rho || !it.pts || !it.out || it.nx <= 0 ||
+            it.ny <= 0 || it.nz <= 0 || it.N_obs <= 0)
+            return set_error(c, GRA_BAD_ARG, tag + "empty input");
+        const uint32_t f = it.field_flag;
+        if ((f & 1u) || (f & ~0x3FFu) || !(f & 0x3FEu))
+            return set_error(c, GRA_BAD_ARG, tag + "field_flag must select components among bits 1..9");
+        if (!increasing(it.xs, it.nx + 1) || !increasing(it.ys, it.ny + 1) ||
+            !increasing(it.zs, it.nz + 1))
+            return set_error(c, GRA_BAD_ARG, tag + "node coordinates must be finite and strictly increasing");
+        if ((it.nx + 1) * (it.ny + 1) > (int64_t)INT32_MAX / (it.nz + 1))
+            return set_error(c, GRA_BAD_ARG, tag + "mesh too large (more than 2^31 nodes)");
+        if (!points_valid(it.pts, it.N_obs)) return set_error(c, GRA_BAD_ARG, tag + "non-finite point");
+        for (int64_t i = 0; i < it.N_obs; ++i)
+            if (!(it.pts[3 * i + 2] < it.zs[0]))
+                return set_error(c, GRA_BAD_ARG,
+                                 tag + "points must lie strictly above the mesh (z < zs[0], z down)");
+        n_axes += it.nx + it.ny + it.nz + 3;
+        n_rho += it.nx * it.ny * it.nz;
+        n_pts += 3 * it.N_obs;
+        n_w += (it.nx + 1) * (it.ny + 1) * (it.nz + 1);
+        n_out += it.N_obs * ncomp(f);
+        blocks += it.N_obs;
+    }
+    if (blocks > (int64_t)INT32_MAX) return set_error(c, GRA_BAD_ARG, "too many points for one launch");
+    // one host arena per array kind, uploaded once
+    std::vector<double> axes((size_t)n_axes), rho((size_t)n_rho), pts((size_t)n_pts);
+    std::vector<FwdItem> tab((size_t)n);
+    int64_t oa = 0, orh = 0, op = 0, ow = 0, oo = 0, ob = 0;
+    for (int64_t b = 0; b < n; ++b) {
+        const gra_fwd_rect_item &it = items[b];
+        double *a = axes.data() + oa;
+        std::copy(it.xs, it.xs + it.nx + 1, a);
+        std::copy(it.ys, it.ys + it.ny + 1, a + it.nx + 1);
+        std::copy(it.zs, it.zs + it.nz + 1, a + it.nx + it.ny + 2);
+        std::copy(it.rho, it.rho + it.nx * it.ny * it.nz, rho.data() + orh);
+        std::copy(it.pts, it.pts + 3 * it.N_obs, pts.data() + op);
+        FwdItem &t = tab[(size_t)b];
+        // offsets now, device addresses once the arenas exist
+        t.xs = reinterpret_cast<const double *>(oa);
+        t.rho = reinterpret_cast<const double *>(orh);
+        t.pts = reinterpret_cast<const double *>(op);
+        t.w = reinterpret_cast<double *>(ow);
+        t.out = reinterpret_cast<double *>(oo);
+        t.nx = it.nx; t.ny = it.ny; t.nz = it.nz; t.N_obs = it.N_obs; t.block0 = ob;
+        t.flag = it.field_flag;
+        oa += it.nx + it.ny + it.nz + 3;
+        orh += it.nx * it.ny * it.nz;
+        op += 3 * it.N_obs;
+        ow += (it.nx + 1) * (it.ny + 1) * (it.nz + 1);
+        oo += it.N_obs * ncomp(it.field_flag);
+        ob += it.N_obs;
+    }
+    HIPCHK(put(c.fwd_axes, axes.data(), axes.size(), c.stream), "upload axes");
+    HIPCHK(put(c.fwd_rho, rho.data(), rho.size(), c.stream), "upload rho");
+    HIPCHK(put(c.fwd_pts, pts.data(), pts.size(), c.stream), "upload points");
+    HIPCHK(c.fwd_w.reserve(sizeof(double) * (size_t)n_w), "alloc node weights");
+    HIPCHK(c.fwd_out.reserve(sizeof(double) * (size_t)n_out), "alloc out");
+    for (FwdItem &t : tab) {
+        const int64_t a = reinterpret_cast<int64_t>(t.xs);
+        t.xs = c.fwd_axes.as<double>() + a;
+        t.ys = t.xs + t.nx + 1;
+        t.zs = t.ys + t.ny + 1;
+        t.rho = c.fwd_rho.as<double>() + reinterpret_cast<int64_t>(t.rho);
+        t.pts = c.fwd_pts.as<double>() + reinterpret_cast<int64_t>(t.pts);
+        t.w = c.fwd_w.as<double>() + reinterpret_cast<int64_t>(t.w);
+        t.out = c.fwd_out.as<double>() + reinterpret_cast<int64_t>(t.out);
+    }
+    HIPCHK(c.fwd_items.reserve(sizeof(FwdItem) * (size_t)n), "alloc item table");
+    HIPCHK(hipMemcpyAsync(c.fwd_items.p, tab.data(), sizeof(FwdItem) * (size_t)n,
+                          hipMemcpyHostToDevice, c.stream),
+           "upload item table");
+    HIPCHK(launch_node_forward_batch(tab.data(), c.fwd_items.as<FwdItem>(), (int)n, blocks, c.stream),
+           "batched node forward");
+    std::vector<double> out((size_t)n_out);
+    HIPCHK(hipMemcpyAsync(out.data(), c.fwd_out.p, sizeof(double) * (size_t)n_out,
+                          hipMemcpyDeviceToHost, c.stream),
+           "download");
+    HIPCHK(hipStreamSynchronize(c.stream), "forward_rectilinear_batch");
+    oo = 0;
+    for (int64_t b = 0; b < n; ++b) {
+        const int64_t m = items[b].N_obs * ncomp(items[b].field_flag);
+        std::copy(out.begin() + oo, out.begin() + oo + m, items[b].out);
+        oo += m;
+    }
+    return GRA_OK;
+}
+
 gra_status gra_set_regularization(gra_ctx *ctx, const gra_csr *Ws, const gra_csr *Wx,
                                   const gra_csr *Wy, const gra_csr *Wz, const gra_csr *Tx0,
                                   const gra_csr *Ty0, const gra_csr *Tz0)
@@ -1213,14 +1329,33 @@ gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
     return GRA_OK;
 }
 
-gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit, double *misfit,
-                       double *cg_err, int *cg_iterations)
+namespace gra {
+namespace {
+// One Gauss-Newton step queued on the context stream (no host
+// synchronisation): its results land in c.host_stage.  When the step may use
+// the fused product, the model and the IRLS weights are first copied aside so
+// that a failed fused wait can be undone (gn_collect).
+gra_status gn_enqueue(Ctx &c, double lambda, double cg_tol, int cg_maxit)
 {
-    CTX_GUARD(ctx);
     if (!c.gn_ready) return set_error(c, GRA_BAD_ARG, "gra_gn_setup_problem not called");
     if (!c.Ws.present || c.Ws.n != c.Nm)
         return set_error(c, GRA_BAD_ARG, "regularisation operators not uploaded for this Nm");
     if (cg_maxit < 0 || !(lambda >= 0)) return set_error(c, GRA_BAD_ARG, "maxit/lambda");
+    c.gn_backup = c.gn_fused && !c.fused_failed;
+    if (c.gn_backup) {
+        const size_t bm = sizeof(double) * (size_t)c.ld;
+        HIPCHK(c.gn_m_prev.reserve(bm), "alloc model backup");
+        HIPCHK(hipMemcpyAsync(c.gn_m_prev.p, c.gn_m.p, bm, hipMemcpyDeviceToDevice, c.stream),
+               "model backup");
+        if (c.gn_have_wl) {
+            const size_t bw = sizeof(double) * 4 * (size_t)c.Nm;
+            HIPCHK(c.gn_WL_prev.reserve(bw), "alloc weight backup");
+            HIPCHK(hipMemcpyAsync(c.gn_WL_prev.p, c.gn_WL.p, bw, hipMemcpyDeviceToDevice, c.stream),
+                   "weight backup");
+        }
+        c.gn_backup_wl = c.gn_have_wl;
+        c.gn_backup_partn = c.partn_holds_gn_model;
+    }
     const int64_t nm = c.Nm, nd = c.Nd, ld = c.ld;
     double *m = c.gn_m.as<double>(), *mt = c.gn_mtrial.as<double>();
     double *scr = c.gn_one.as<double>();
@@ -1250,10 +1385,9 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
         dg.WLy = wl + 2 * nm;
         dg.WLz = wl + 3 * nm;
     }
-    double err = NAN;
-    int its = 0;
     gra_status s = cgls_solve(c, a, m, c.gn_mref.as<double>(), c.gn_dobs.as<double>(), dg,
-                              c.gn_dx.as<double>(), &err, &its, c.partn_holds_gn_model);
+                              c.gn_dx.as<double>(), nullptr, nullptr, c.partn_holds_gn_model,
+                              true);
     if (s != GRA_OK) return s;
     HIPCHK(launch_barrier_update(m, c.gn_dx.as<double>(), c.gn_mmin.as<double>(),
                                  c.gn_mmax.as<double>(), nm, mt, scr, flags, c.stream),
@@ -1280,13 +1414,119 @@ gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
                    "irls");
         c.gn_have_wl = true;
     }
-    double hm = NAN;
-    HIPCHK(hipMemcpyAsync(&hm, dmis, sizeof(double), hipMemcpyDeviceToHost, c.stream), "misfit");
-    HIPCHK(hipStreamSynchronize(c.stream), "gn step");
-    if (misfit) *misfit = hm;
+    HIPCHK(hipMemcpyAsync(&c.host_stage->misfit, dmis, sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream),
+           "misfit");
+    c.gn_args = {lambda, cg_tol, cg_maxit};
+    return GRA_OK;
+}
+
+// After the stream has been synchronised: the step's results; a failed fused
+// wait restores the model and weights and redoes the step two-pass (the
+// context stays two-pass from then on).
+gra_status gn_collect(Ctx &c, double *misfit, double *cg_err, int *cg_iterations)
+{
+    double err = NAN;
+    int its = 0;
+    bool rerun = false;
+    gra_status s = cgls_collect(c, &err, &its, &rerun);
+    if (rerun) {
+        const size_t bm = sizeof(double) * (size_t)c.ld;
+        HIPCHK(hipMemcpyAsync(c.gn_m.p, c.gn_m_prev.p, bm, hipMemcpyDeviceToDevice, c.stream),
+               "model restore");
+        if (c.gn_backup_wl)
+            HIPCHK(hipMemcpyAsync(c.gn_WL.p, c.gn_WL_prev.p, sizeof(double) * 4 * (size_t)c.Nm,
+                                  hipMemcpyDeviceToDevice, c.stream),
+                   "weight restore");
+        c.gn_have_wl = c.gn_backup_wl;
+        // part_n was overwritten by the misfit product of the failed step
+        c.partn_holds_gn_model = false;
+        (void)c.gn_backup_partn;
+        if ((s = gn_enqueue(c, c.gn_args.lambda, c.gn_args.tol, c.gn_args.maxit)) != GRA_OK)
+            return s;
+        HIPCHK(hipStreamSynchronize(c.stream), "gn step");
+        return gn_collect(c, misfit, cg_err, cg_iterations);
+    }
+    if (s != GRA_OK) return s;
+    if (misfit) *misfit = c.host_stage->misfit;
     if (cg_err) *cg_err = err;
     if (cg_iterations) *cg_iterations = its;
     return GRA_OK;
+}
+}  // namespace
+}  // namespace gra
+
+gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit, double *misfit,
+                       double *cg_err, int *cg_iterations)
+{
+    CTX_GUARD(ctx);
+    gra_status s = gn_enqueue(c, lambda, cg_tol, cg_maxit);
+    if (s != GRA_OK) return s;
+    HIPCHK(hipStreamSynchronize(c.stream), "gn step");
+    return gn_collect(c, misfit, cg_err, cg_iterations);
+}
+
+gra_status gra_gn_step_batch(gra_ctx *const *ctxs, int64_t n, const double *lambdas,
+                             double cg_tol, int cg_maxit, double *misfits, double *cg_errs,
+                             int *cg_iterations)
+{
+    if (!ctxs || n <= 0 || !lambdas) return GRA_BAD_ARG;
+    for (int64_t b = 0; b < n; ++b)
+        if (!ctxs[b]) return GRA_BAD_ARG;
+    // Every instance's step is queued on its own stream before any is waited
+    // for.  Steps whose CG may use the fused product (a cooperatively
+    // launched grid that needs the whole device) are chained by events, so
+    // that no two such grids are in flight together; two-pass steps run
+    // concurrently with everything.
+    hipEvent_t prev_fused = nullptr;
+    gra_status first = GRA_OK;
+    int64_t queued = 0;
+    for (int64_t b = 0; b < n && first == GRA_OK; ++b) {
+        Ctx &c = ctxs[b]->c;
+        std::lock_guard<std::mutex> lock(c.mu);
+        c.last_error.clear();
+        if (hipSetDevice(c.device) != hipSuccess) {
+            first = set_error(c, GRA_HIP_ERROR, "hipSetDevice failed");
+            break;
+        }
+        const bool coop = c.gn_fused && !c.fused_failed;
+        if (coop && prev_fused && hipStreamWaitEvent(c.stream, prev_fused, 0) != hipSuccess) {
+            first = set_error(c, GRA_HIP_ERROR, "stream wait");
+            break;
+        }
+        gra_status s = gn_enqueue(c, lambdas[b], cg_tol, cg_maxit);
+        if (s != GRA_OK) {
+            first = s;
+            break;
+        }
+        if (coop) {
+            if (!c.batch_event &&
+                hipEventCreateWithFlags(&c.batch_event, hipEventDisableTiming) != hipSuccess) {
+                first = set_error(c, GRA_HIP_ERROR, "event");
+                break;
+            }
+            if (hipEventRecord(c.batch_event, c.stream) != hipSuccess) {
+                first = set_error(c, GRA_HIP_ERROR, "event record");
+                break;
+            }
+            prev_fused = c.batch_event;
+        }
+        ++queued;
+    }
+    for (int64_t b = 0; b < queued; ++b) {
+        Ctx &c = ctxs[b]->c;
+        std::lock_guard<std::mutex> lock(c.mu);
+        if (hipSetDevice(c.device) != hipSuccess || hipStreamSynchronize(c.stream) != hipSuccess) {
+            if (first == GRA_OK) first = set_error(c, GRA_HIP_ERROR, "batch sync");
+            continue;
+        }
+        if (first != GRA_OK) continue;
+        gra_status s = gn_collect(c, misfits ? misfits + b : nullptr,
+                                  cg_errs ? cg_errs + b : nullptr,
+                                  cg_iterations ? cg_iterations + b : nullptr);
+        if (s != GRA_OK) first = s;
+    }
+    return first;
 }
 
 gra_status gra_gn_get_model(gra_ctx *ctx, double *m)
@@ -1343,6 +1583,11 @@ gra_status gra_tune_set(gra_ctx *ctx, int param, int64_t value)
         break;
     case GRA_TUNE_LAST_REFRESHES:
         return set_error(c, GRA_BAD_ARG, "last refreshes is read-only");
+    case GRA_TUNE_FUSED_MIN_BLOCKS:
+        if (value < 0 || value > 1000000)
+            return set_error(c, GRA_BAD_ARG, "fused minimum row blocks: 0..1000000");
+        c.fused_min_blocks = (int)value;
+        break;
     case GRA_TUNE_CG_GRAPH:
         if (value < 0 || value > 1000)
             return set_error(c, GRA_BAD_ARG, "CG iterations per graph: 0 (off) .. 1000");
@@ -1506,6 +1751,7 @@ gra_status gra_tune_get(gra_ctx *ctx, int param, int64_t *value)
     case GRA_TUNE_FUSED_MATH: *value = c.fused_math; break;
     case GRA_TUNE_LAST_REFRESHES: *value = c.last_refreshes; break;
     case GRA_TUNE_CG_GRAPH: *value = c.cg_graph; break;
+    case GRA_TUNE_FUSED_MIN_BLOCKS: *value = c.fused_min_blocks; break;
     case GRA_TUNE_FUSED_FALLBACKS: *value = c.fused_fail_count; break;
     case GRA_TUNE_N_BLOCK: *value = s.n_block; break;
     case GRA_TUNE_NODE_TILE: *value = c.node_tile; break;

@@ -89,9 +89,17 @@ struct FusedPlan {
     const void *kernel = nullptr;
 };
 
+// Pinned host staging of a context's per-solve results (read after a sync).
+struct HostStage {
+    CgScalars sc;
+    int ferr;
+    double misfit;
+};
+
 struct Ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the context's work is queued
+    hipStream_t own_stream = nullptr;  // created with the context
     std::mutex mu;
     std::string last_error;
 
@@ -121,6 +129,7 @@ struct Ctx {
     DevBuf vec_nd[4], vec_nm[12], vec_row[3];
     DevBuf fwd_prisms, fwd_rho, fwd_pts, fwd_out;   // gra_forward inputs/outputs (grow-only)
     DevBuf fwd_axes, fwd_w;                          // gra_forward_rectilinear
+    DevBuf fwd_items;                                // gra_forward_rectilinear_batch
     DevBuf rect_axes;                                // gra_assemble_rectilinear
     DevBuf mm_in, mm_pack, mm_part, mm_out;         // gra_G_mat_mul / gra_GT_mat_mul
     int node_tile = 0;                               // its tile side (0: by LDS budget)
@@ -130,6 +139,15 @@ struct Ctx {
     DevBuf scal;               // CgScalars
     DevBuf status;             // int flags written by kernels
     double *noalloc_scratch = nullptr;   // this device's `scratch` (gra_G_vec_mul_noalloc)
+    HostStage *host_stage = nullptr;     // pinned (hipHostMalloc at context creation)
+    // the solve / Gauss-Newton step whose results host_stage will hold
+    bool pend_fused = false;
+    int pend_maxit = 0, pend_refresh = 0;
+    bool gn_backup = false;              // gn_m_prev / gn_WL_prev hold the step's inputs
+    bool gn_backup_wl = false, gn_backup_partn = false;
+    DevBuf gn_m_prev, gn_WL_prev;
+    struct { double lambda, tol; int maxit; } gn_args{0.0, 0.0, 0};   // the queued step
+    hipEvent_t batch_event = nullptr;    // gra_gn_step_batch ordering of fused steps
     DevBuf host_io;            // unused placeholder
 
     // wavelet-compressed G (gra_assemble_wavelet): replaces the dense G while
@@ -166,6 +184,7 @@ struct Ctx {
     int fused_depth = 0;          // 0: default (lag, prefetch) variant
     int fused_math = 0;           // 0: default per storage type, 1: FP64, 2: FP32 products
     int cg_graph = 0;             // two-pass CG iterations per captured hipGraph (0: none)
+    int fused_min_blocks = 0;     // fused only with at least this many row blocks per group
     int64_t fused_fail_block = -1;   // test hook: force a failed wait at this row block
     bool fused_failed = false;    // a fused solve failed on this context: two-pass from now on
     int fused_fail_count = 0;     // fused solves rerun with the two-pass iteration
@@ -211,6 +230,16 @@ hipError_t launch_assemble_batch(const BatchItem *d_items, int n_items, int64_t 
 hipError_t launch_cell_corner_batch(const BatchItem *d_items, int n_items, int64_t total_blocks,
                                     gra_dtype store, hipStream_t st);
 
+// One request of a batched node-sum forward (device pointers; items sorted
+// by block0, the first point's workgroup in the launch).
+struct FwdItem {
+    const double *xs, *ys, *zs, *rho, *pts;
+    double *w, *out;
+    int64_t nx, ny, nz, N_obs, block0;
+    uint32_t flag;
+};
+hipError_t launch_node_forward_batch(const FwdItem *h_items, const FwdItem *d_items, int n_items,
+                                     int64_t total_blocks, hipStream_t st);
 // Node-sum forward on a rectilinear mesh (rectilinear_kernels.hip); w holds
 // (nx+1)(ny+1)(nz+1) node weights.
 // Multi-right-hand-side products on the matrix cores (gemm_kernels.hip).
@@ -293,7 +322,7 @@ hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, 
 // workgroup per CU, larger values are capped at the co-resident limit.
 // math: 0 default for the storage type, 1 FP64 products, 2 FP32 products.
 bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, int depth,
-                int math, FusedPlan *out);
+                int math, int min_blocks, FusedPlan *out);
 size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld);
 // fail_block >= 0 makes the wait on that row block fail (test hook).
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
@@ -311,9 +340,13 @@ struct CgDiag {
 // shape), so the initial product is not recomputed.  a.fused selects the
 // single-read iteration (falls back to the two-pass one when the fused grid
 // cannot be co-resident).
+// defer: enqueue only (results to c.host_stage); cgls_collect after a sync
+// reads them (rerun: the fused product failed, the solve must be redone
+// with the two-pass iteration).
 gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const double *d_mref,
                       const double *d_dobs, const CgDiag &dg, double *d_dx, double *err,
-                      int *iterations, bool have_Gmk = false);
+                      int *iterations, bool have_Gmk = false, bool defer = false);
+gra_status cgls_collect(Ctx &c, double *err, int *iterations, bool *rerun);
 hipError_t launch_barrier_P(const double *m, const double *mmin, const double *mmax, int64_t n,
                             double *P, hipStream_t st);
 hipError_t launch_barrier_update(const double *m, const double *dx, const double *mmin,

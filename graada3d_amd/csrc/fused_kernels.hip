@@ -471,7 +471,7 @@ using CfgBf16 = FusedCfg<__hip_bfloat16, 1, 8>;
 }  // namespace
 
 bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, int depth,
-                int math, FusedPlan *out)
+                int math, int min_blocks, FusedPlan *out)
 {
     int cus = 0, coop = 0, clock_khz = 0, n_xcd = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
@@ -502,7 +502,8 @@ bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, 0) != hipSuccess)
         return false;
     FusedShape s;
-    if (!fused_plan_shape(Nd, ld, TW, RB, cus, n_xcd, per_cu, target_wgs, kMaxPoll, &s))
+    if (!fused_plan_shape(Nd, ld, TW, RB, cus, n_xcd, per_cu, target_wgs, kMaxPoll, &s,
+                          min_blocks))
         return false;
     FusedPlan p;
     p.NT = s.NT;

@@ -30,8 +30,12 @@ struct FusedShape {
 // NT, workgroup b is mapped to the share of XCD b mod n_xcd (workgroups are
 // dispatched to the XCDs round robin), so a row group's partials stay in one
 // XCD; the protocol is correct for any mapping.
+// min_blocks > 0: refuse a shape whose groups hold fewer row blocks (the
+// pipeline's fill and drain, L + PF steps, would dominate; the two-pass
+// iteration is faster there).
 inline bool fused_plan_shape(int64_t Nd, int64_t ld, int TW, int RB, int cus, int n_xcd,
-                             int per_cu, int target, int max_poll, FusedShape *out)
+                             int per_cu, int target, int max_poll, FusedShape *out,
+                             int min_blocks = 0)
 {
     if (Nd < 1 || ld < 1 || TW < 1 || RB < 1 || cus < 1 || per_cu < 1) return false;
     if (n_xcd < 1) n_xcd = 1;
@@ -64,6 +68,7 @@ inline bool fused_plan_shape(int64_t Nd, int64_t ld, int TW, int RB, int cus, in
     s.S = (int)S;
     s.per_xcd = per_xcd;
     s.nrb = s.rows_per_group / RB;
+    if (min_blocks > 0 && s.nrb < min_blocks) return false;
     s.grid = s.NT * s.S;
     *out = s;
     return true;

@@ -168,21 +168,23 @@ __device__ inline double comp_scale(int bit)
     }
 }
 
-__global__ void __launch_bounds__(kBlk)
-node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys,
-                    const double *__restrict__ zs, int64_t nx, int64_t ny, int64_t nz,
-                    const double *__restrict__ w, const double *__restrict__ pts, int64_t N_obs,
-                    uint32_t flag, double *__restrict__ out)
+// The forward sum of one observation point (one workgroup): every mesh node's
+// weight times the corner function, summed over the nodes column by column
+// by the workgroup's threads, then reduced in a fixed order.  Block-uniform
+// arguments (it synchronises the workgroup).
+__device__ inline void node_forward_point(const double *__restrict__ xs,
+                                          const double *__restrict__ ys,
+                                          const double *__restrict__ zs, int64_t nx, int64_t ny,
+                                          int64_t nz, const double *__restrict__ w, double px,
+                                          double py, double pz, uint32_t flag,
+                                          double *__restrict__ out_row)
 {
     __shared__ double zl[kMaxLevels];
     __shared__ double lg_G[kLogTable], lg_T[kLogTable];
-    const int64_t ip = blockIdx.x;
-    if (ip >= N_obs) return;
     for (int t = threadIdx.x; t < kLogTable; t += kBlk) {
         lg_G[t] = kLogG[t];
         lg_T[t] = kLogT[t];
     }
-    const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
     const int NY = (int)ny + 1, NZ = (int)nz + 1, NCOL = ((int)nx + 1) * NY;
     const bool lds_levels = NZ <= kMaxLevels;
     if (lds_levels) {
@@ -222,17 +224,50 @@ node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int nc = 0;
-        for (int bit = 1; bit < 10; ++bit) nc += (flag >> bit) & 1u;
         int col = 0;
         for (int bit = 1; bit < 10; ++bit) {
             if (!(flag & (1u << bit))) continue;
             double s = red[0][bit];
             for (int q = 1; q < kBlk / 64; ++q) s += red[q][bit];
-            out[ip * nc + col] = comp_scale(bit) * s;
+            out_row[col] = comp_scale(bit) * s;
             ++col;
         }
     }
+}
+
+__global__ void __launch_bounds__(kBlk)
+node_forward_kernel(const double *__restrict__ xs, const double *__restrict__ ys,
+                    const double *__restrict__ zs, int64_t nx, int64_t ny, int64_t nz,
+                    const double *__restrict__ w, const double *__restrict__ pts, int64_t N_obs,
+                    uint32_t flag, double *__restrict__ out)
+{
+    const int64_t ip = blockIdx.x;
+    if (ip >= N_obs) return;
+    int nc = 0;
+    for (int bit = 1; bit < 10; ++bit) nc += (flag >> bit) & 1u;
+    node_forward_point(xs, ys, zs, nx, ny, nz, w, pts[3 * ip], pts[3 * ip + 1], pts[3 * ip + 2],
+                       flag, out + ip * nc);
+}
+
+// Several forward requests in one launch: workgroup b takes point
+// b - block0 of the item whose block range holds b (items sorted by block0).
+__global__ void __launch_bounds__(kBlk)
+node_forward_batch_kernel(const FwdItem *__restrict__ items, int n_items)
+{
+    int lo = 0, hi = n_items - 1;
+    const int64_t b = blockIdx.x;
+    while (lo < hi) {                       // last item with block0 <= b
+        const int mid = (lo + hi + 1) / 2;
+        if (items[mid].block0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const FwdItem &it = items[lo];
+    const int64_t ip = b - it.block0;
+    if (ip >= it.N_obs) return;
+    int nc = 0;
+    for (int bit = 1; bit < 10; ++bit) nc += (it.flag >> bit) & 1u;
+    node_forward_point(it.xs, it.ys, it.zs, it.nx, it.ny, it.nz, it.w, it.pts[3 * ip],
+                       it.pts[3 * ip + 1], it.pts[3 * ip + 2], it.flag, it.out + ip * nc);
 }
 
 // (a, b, k) of the flat index i = (a * nb + b) * nk + k, advanced by kBlk per
@@ -560,6 +595,22 @@ hipError_t launch_cell_corner_batch(const BatchItem *d_items, int n_items, int64
                            n_items);
         break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_node_forward_batch(const FwdItem *h_items, const FwdItem *d_items, int n_items,
+                                     int64_t total_blocks, hipStream_t st)
+{
+    for (int q = 0; q < n_items; ++q) {
+        const FwdItem &it = h_items[q];
+        const int64_t NN = (it.nx + 1) * (it.ny + 1) * (it.nz + 1);
+        hipLaunchKernelGGL(node_weights_kernel, dim3((unsigned)((NN + kBlk - 1) / kBlk)),
+                           dim3(kBlk), 0, st, it.rho, it.nx, it.ny, it.nz, it.w);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || total_blocks == 0) return e;
+    hipLaunchKernelGGL(node_forward_batch_kernel, dim3((unsigned)total_blocks), dim3(kBlk), 0, st,
+                       d_items, n_items);
     return hipGetLastError();
 }
 

@@ -116,6 +116,13 @@ class Context:
     def sync(self):
         self.check(self.L.gra_sync(self.h), "gra_sync")
 
+    def use_stream_of(self, other: "Context | None"):
+        """Queue this context's later work on `other`'s stream (None: back on
+        its own).  Waits for the work queued on the stream in use first.
+        `other` must stay open while this context uses its stream."""
+        s = self.L.gra_ctx_stream(other.h) if other is not None else None
+        self.check(self.L.gra_ctx_set_stream(self.h, s), "gra_ctx_set_stream")
+
     # timing (GPU events on the context stream)
     def timing(self, on):
         """False/True, or a stride n: CG-loop launches timed on every n-th
@@ -460,6 +467,26 @@ class GaussNewtonInversion(Fwd):
         return m
 
 
+def gn_step_batch(invs, lambdas, cg_tol: float, cg_maxit: int):
+    """One resident Gauss-Newton step on each inversion (each with its own
+    context, one device) in a single batched submission
+    (gra_gn_step_batch): returns (misfits, cg_errs, cg_iterations)."""
+    n = len(invs)
+    hs = (ctypes.c_void_p * n)(*[f.ctx.h.value for f in invs])
+    lam = np.ascontiguousarray(lambdas, dtype=np.float64)
+    if lam.size != n:
+        raise ValueError("one lambda per inversion")
+    mis, err = np.empty(n), np.empty(n)
+    its = (ctypes.c_int * n)()
+    L = invs[0].ctx.L
+    st = L.gra_gn_step_batch(hs, n, _p(lam), float(cg_tol), int(cg_maxit), _p(mis), _p(err), its)
+    if st != 0:
+        msg = "; ".join(L.gra_last_error(f.ctx.h).decode(errors="replace") for f in invs
+                        if L.gra_last_error(f.ctx.h))
+        raise GraError(st, "gn_step_batch", msg)
+    return mis, err, np.array(its[:], dtype=int)
+
+
 def assemble_batch(fwds, store: str = "f64", method: str = "faces"):
     """Assemble the G of every Fwd in `fwds` (each with its own context, one
     device) in a single batched launch: the face/edge values
@@ -497,6 +524,29 @@ def forward_rectilinear(xs, ys, zs, rho, points, field_flag: int, ctx: Context |
                                        _p(points), points.shape[0], int(field_flag), _p(out))
     ctx.check(st, "forward_rectilinear")
     return out
+
+
+def forward_rectilinear_batch(requests, ctx: Context | None = None):
+    """Several forward_rectilinear requests in one launch; `requests` is a
+    list of (xs, ys, zs, rho, points, field_flag); returns one (N_obs, n_comp)
+    array per request, bit-identical to forward_rectilinear of each."""
+    ctx = ctx or Context()
+    n = len(requests)
+    items = (_lib.GraFwdRectItem * n)()
+    keep, outs = [], []
+    for it, (xs, ys, zs, rho, points, flag) in zip(items, requests):
+        xs, ys, zs = _f64(xs), _f64(ys), _f64(zs)
+        nx, ny, nz = xs.size - 1, ys.size - 1, zs.size - 1
+        points = _f64(points).reshape(-1, 3)
+        rho = _f64(rho, nx * ny * nz, "rho")
+        out = np.empty((points.shape[0], len(component_bits(flag))))
+        keep += [xs, ys, zs, points, rho]
+        outs.append(out)
+        it.xs, it.nx, it.ys, it.ny, it.zs, it.nz = _p(xs), nx, _p(ys), ny, _p(zs), nz
+        it.rho, it.pts, it.N_obs, it.field_flag, it.out = (_p(rho), _p(points), points.shape[0],
+                                                           int(flag), _p(out))
+    ctx.check(ctx.L.gra_forward_rectilinear_batch(ctx.h, n, items), "forward_rectilinear_batch")
+    return outs
 
 
 def forward(prisms, rho, points, field_flag: int, ctx: Context | None = None):

@@ -16,20 +16,22 @@ Stream (seeded numpy PCG64):
   gra_forward_rectilinear; `forward_method="prisms"` uses the per-prism
   face/edge kernel, gra_forward) and returned as FP32;
 * 20 % inversion requests: 20 CGLS iterations (L2, lambda log-uniform in
-  [0.1, 100]) on one of a pool of problems whose dense G stays resident in
-  HBM between requests.  Residency is a byte-budgeted LRU of engine
-  contexts: a request for a problem that is not resident assembles its G
-  (admission, counted in that request's latency), evicting least recently
-  used problems until the budget holds.
+  [0.1, 100]) on one of a pool of problems (default_cg_pool: the 5,000 x
+  32,000 survey, the 64 uneven octree instances, the 90,000 x 131,072
+  nine-component survey) whose G stays resident in HBM between requests
+  while a byte budget allows (ResidentCache: admission on demand, eviction
+  of idle problems by a size- and cost-aware policy).
 
 Request payloads are built ahead by a producer thread (the client's work);
 a dispatcher releases each request at its arrival time and `workers`
 server threads (default 3, each with its own context and stream) take them
-in that order, so up to `workers` requests are on the GPU at once.
+in that order, so up to `workers` requests are on the GPU at once; a CG
+solve runs on the worker's stream, borrowing the resident problem's context.
 """
 from __future__ import annotations
 
 import collections
+import functools
 import threading
 import time
 from dataclasses import dataclass, field
@@ -37,7 +39,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import mesh as M
-from .engine import Context, GaussNewtonInversion, forward, forward_rectilinear
+from .engine import (Context, GaussNewtonInversion, forward, forward_rectilinear,
+                     forward_rectilinear_batch)
 from .operators import depth_weights_offset, weighting_operators
 from .workloads import add_noise
 
@@ -55,25 +58,71 @@ class Request:
 @dataclass
 class CgProblem:
     name: str
-    shape: tuple                   # (nx, ny, nz)
+    shape: tuple                   # (nx, ny, nz); unused for a batch instance
     extent: tuple                  # (x1, y1, z1), origin 0
     n_pts: int                     # points per side
     flag: int
     weight: float
+    store: str = "f64"             # storage of the resident G
+    fused: bool = False            # single-read CG iteration
+    instance: int = -1             # >= 0: index into workloads.batch_instances()
 
 
-def default_cg_pool(include_large: bool = False):
-    pool = [
-        CgProblem("inversion_small", (40, 40, 20), (2000.0, 2000.0, 1000.0), 50, 0x12, 0.5),
-        CgProblem("medium_5comp", (30, 30, 15), EXTENT, 40, 0x2 | 0x10 | 0x20 | 0x40 | 0x80, 0.3),
-        CgProblem("small_9comp", (24, 24, 12), EXTENT, 30, 0x3FE, 0.2),
-    ]
-    if include_large:
-        pool.append(CgProblem("tensor_large", (64, 64, 32), EXTENT, 100, 0x3FE, 0.05))
-    w = np.array([p.weight for p in pool])
-    for p, q in zip(pool, w / w.sum()):
-        p.weight = float(q)
+def default_cg_pool():
+    """The CG problems of the serving mix at the benchmark workloads' sizes
+    (161 GB of G in all, more than a serving budget holds):
+    * the 5,000 x 32,000 survey (FP64, two passes; 1.3 GB), 45 % of the solves;
+    * each of the 64 uneven octree instances (FP64, two passes; 0.03-23 GB,
+      114 GB in all), 54 % of the solves spread evenly over them;
+    * the 90,000 x 131,072 nine-component survey (FP32 storage, single-read
+      iteration; 47 GB), 1 %."""
+    n_inst = len(_instances())
+    pool = [CgProblem("inversion_small", (40, 40, 20), (2000.0, 2000.0, 1000.0), 50, 0x12, 0.45)]
+    for k in range(n_inst):
+        pool.append(CgProblem(f"batch_{k:02d}", (0, 0, 0), EXTENT, 0, 0, 0.54 / n_inst,
+                              instance=k))
+    pool.append(CgProblem("tensor_large", (64, 64, 32), EXTENT, 100, 0x3FE, 0.01,
+                          store="f32", fused=True))
     return pool
+
+
+_HOST = {}
+_HOST_LOCK = threading.Lock()
+
+
+@functools.lru_cache(maxsize=1)
+def _instances():
+    """workloads.batch_instances(), generated once per process."""
+    from .workloads import batch_instances
+    return batch_instances()
+
+
+def problem_inputs(prob: CgProblem):
+    """Host-side inputs of a CG problem (built once per process): prisms,
+    points, component flag, true model, and the regularisation operators."""
+    with _HOST_LOCK:
+        got = _HOST.get(prob.name)
+        if got is not None:
+            return got
+    if prob.instance >= 0:
+        from .workloads import batch_regularisation
+        b = _instances()[prob.instance]
+        W, _ = batch_regularisation(b)
+        out = dict(prisms=b.prisms, pts=b.points, flag=b.flag, rho=b.rho, W=W)
+    else:
+        x1, y1, z1 = prob.extent
+        nx, ny, nz = prob.shape
+        prisms = M.regular_mesh((0.0, x1), nx, (0.0, y1), ny, (0.0, z1), nz)
+        pts = M.grid_points(prob.n_pts, 0.0, x1, prob.n_pts, 0.0, y1, -1.0)
+        Z = depth_weights_offset(prisms, 50.0, 3.0)
+        W = weighting_operators(prisms, nx, ny, nz, Z, a_s=1e-3)
+        c = (0.5 * x1, 0.5 * y1, 0.3 * z1)
+        rho = M.model_from_boxes(prisms, [(c[0] - 0.1 * x1, c[0] + 0.1 * x1, c[1] - 0.1 * y1,
+                                           c[1] + 0.1 * y1, c[2] - 0.1 * z1, c[2] + 0.1 * z1,
+                                           500.0)])
+        out = dict(prisms=prisms, pts=pts, flag=prob.flag, rho=rho, W=W)
+    with _HOST_LOCK:
+        return _HOST.setdefault(prob.name, out)
 
 
 def request_stream(rate: float, duration: float, seed: int, pool, p_forward: float = 0.8):
@@ -115,27 +164,62 @@ def forward_payload(p: dict):
 
 
 class ResidentCache:
-    """Byte-budgeted LRU of contexts, each holding one problem's G."""
+    """Byte-budgeted residency of contexts, each holding one problem's G.
 
-    def __init__(self, pool, budget_bytes: float, device: int = 0):
-        self.pool, self.budget, self.device = pool, budget_bytes, device
+    A request for a problem that is not resident admits it (assembles its G,
+    counted in that request's latency), first evicting idle problems until
+    the budget holds.  Victims, by `policy`:
+    * "gdsf" (default; greedy-dual-size-frequency): the idle problem with
+      the lowest H = L + hits * cost / bytes, where cost is the problem's
+      measured admission time and L (the clock) rises to each victim's H, so
+      that a large problem that is expensive to rebuild and asked for often
+      stays, and problems not asked for lately age out;
+    * "lru": the least recently used idle problem."""
+
+    def __init__(self, pool, budget_bytes: float, device: int = 0, policy: str = "gdsf"):
+        if policy not in ("gdsf", "lru"):
+            raise ValueError(f"unknown eviction policy {policy!r}")
+        self.pool, self.budget, self.device, self.policy = pool, budget_bytes, device, policy
         self.entries = collections.OrderedDict()
         self.used = 0.0
         self.admissions = 0
         self.evictions = 0
+        self.admit_s = 0.0
+        self.clock = 0.0
+        self.cost = {}                  # pid -> last measured admission time (s)
         self.lock = threading.Lock()
         self.cv = threading.Condition(self.lock)
 
     @staticmethod
     def g_bytes(prob: CgProblem) -> float:
+        b = {"f64": 8.0, "f32": 4.0, "bf16": 2.0}[prob.store]
+        if prob.instance >= 0:
+            inst = _instances()[prob.instance]
+            return b * inst.prisms.shape[0] * inst.points.shape[0] * bin(inst.flag).count("1")
         nx, ny, nz = prob.shape
-        return 8.0 * nx * ny * nz * prob.n_pts ** 2 * bin(prob.flag).count("1")
+        return b * nx * ny * nz * prob.n_pts ** 2 * bin(prob.flag).count("1")
 
-    def get(self, pid: int, info: dict | None = None):
+    def prepare(self):
+        """Builds every problem's host-side inputs (before a replay)."""
+        for p in self.pool:
+            problem_inputs(p)
+
+    def _value(self, pid, e):
+        cost = self.cost.get(pid, e["bytes"] * 1e-11)     # ~ assembly time before measured
+        return self.clock + e["hits"] * cost / e["bytes"]
+
+    def _victims(self):
+        idle = [(pid, e) for pid, e in self.entries.items() if e.get("ready") and e["busy"] == 0]
+        if self.policy == "gdsf":
+            idle.sort(key=lambda t: t[1]["H"])
+        return idle
+
+    def get(self, pid: int, info: dict | None = None, stream_of=None):
         """The resident entry of problem pid, admitted if needed; the caller
         holds it (it is not evicted) until release().  Safe to call from
         several threads: one admits, the others wait for it.  info["admitted"]
-        tells whether this call admitted the problem."""
+        tells whether this call admitted the problem.  An admission queues
+        its GPU work on `stream_of`'s stream (a Context) if given."""
         if info is not None:
             info["admitted"] = False
         with self.lock:
@@ -146,39 +230,47 @@ class ResidentCache:
                 if e.get("ready"):
                     self.entries.move_to_end(pid)
                     e["busy"] += 1
+                    e["hits"] += 1
+                    e["H"] = self._value(pid, e)
                     return e
                 self.cv.wait()
             prob = self.pool[pid]
             need = self.g_bytes(prob)
             if need > self.budget:
                 raise MemoryError(f"problem {prob.name} ({need / 1e9:.1f} GB) exceeds the budget")
-            # evict idle least-recently-used problems; in-use ones stay (the
-            # budget may be exceeded until they are released)
-            for old_pid in list(self.entries):
+            # evict idle problems; in-use ones stay (the budget may be
+            # exceeded until they are released)
+            for old_pid, old in self._victims():
                 if self.used + need <= self.budget:
                     break
-                old = self.entries[old_pid]
-                if old.get("ready") and old["busy"] == 0:
-                    del self.entries[old_pid]
-                    old["ctx"].close()
-                    self.used -= old["bytes"]
-                    self.evictions += 1
-            placeholder = dict(ready=False, busy=1, bytes=need)
+                del self.entries[old_pid]
+                old["ctx"].close()
+                self.used -= old["bytes"]
+                self.evictions += 1
+                if self.policy == "gdsf":
+                    self.clock = max(self.clock, old["H"])
+            placeholder = dict(ready=False, busy=1, bytes=need, hits=1, H=0.0,
+                               lock=threading.Lock())
             if info is not None:
                 info["admitted"] = True
             self.entries[pid] = placeholder
             self.used += need
             self.admissions += 1
+        t0 = time.perf_counter()
         try:
-            e = self._admit(prob, need)
+            e = self._admit(prob, need, stream_of)
         except BaseException:
             with self.lock:
                 del self.entries[pid]
                 self.used -= need
                 self.cv.notify_all()
             raise
+        dt = time.perf_counter() - t0
         with self.lock:
+            self.admit_s += dt
+            self.cost[pid] = dt
             placeholder.update(e, ready=True)
+            placeholder["H"] = self._value(pid, placeholder)
             self.cv.notify_all()
         return placeholder
 
@@ -186,30 +278,29 @@ class ResidentCache:
         with self.lock:
             e["busy"] -= 1
 
-    def _admit(self, prob: CgProblem, need: float):
-        nx, ny, nz = prob.shape
-        x1, y1, z1 = prob.extent
-        prisms = M.regular_mesh((0.0, x1), nx, (0.0, y1), ny, (0.0, z1), nz)
-        pts = M.grid_points(prob.n_pts, 0.0, x1, prob.n_pts, 0.0, y1, -1.0)
+    def _admit(self, prob: CgProblem, need: float, stream_of=None):
+        h = problem_inputs(prob)
         ctx = Context(self.device)
-        inv = GaussNewtonInversion(prisms, pts, prob.flag, ctx=ctx)
-        inv.compute_G("f64")
-        Z = depth_weights_offset(prisms, 50.0, 3.0)
-        inv.set_regularization(*weighting_operators(prisms, nx, ny, nz, Z, a_s=1e-3))
-        c = (0.5 * x1, 0.5 * y1, 0.3 * z1)
-        rho = M.model_from_boxes(prisms, [(c[0] - 0.1 * x1, c[0] + 0.1 * x1, c[1] - 0.1 * y1,
-                                            c[1] + 0.1 * y1, c[2] - 0.1 * z1, c[2] + 0.1 * z1,
-                                            500.0)])
-        ncomp = bin(prob.flag).count("1")
-        dobs, sigma = add_noise(inv.G_vec_mul(rho), pts.shape[0], ncomp, 7)
-        if not np.all(sigma > 0):
+        try:
+            if stream_of is not None:
+                ctx.use_stream_of(stream_of)
+            inv = GaussNewtonInversion(h["prisms"], h["pts"], h["flag"], ctx=ctx)
+            inv.compute_G(prob.store)
+            inv.set_regularization(*h["W"])
+            nobs = h["pts"].shape[0]
+            dobs, sigma = add_noise(inv.G_vec_mul(h["rho"]), nobs, bin(h["flag"]).count("1"), 7)
+            if not np.all(sigma > 0):
+                raise ValueError(f"problem {prob.name}: the synthetic anomaly holds no cell "
+                                 "(zero data, no noise scale)")
+            if stream_of is not None:
+                ctx.use_stream_of(None)
+        except BaseException:
             ctx.close()
-            raise ValueError(f"problem {prob.name}: the synthetic anomaly holds no cell "
-                             "(zero data, no noise scale)")
-        Wd = np.repeat(1.0 / sigma, pts.shape[0])
-        nm = prisms.shape[0]
+            raise
+        Wd = np.repeat(1.0 / sigma, nobs)
+        nm = h["prisms"].shape[0]
         P = np.full(nm, (1500.0 * 1500.0) / 3000.0)     # barrier P at m = 0, bounds +-1500
-        return dict(ctx=ctx, inv=inv, bytes=need, dobs=dobs, Wd=Wd, P=P, nm=nm)
+        return dict(ctx=ctx, inv=inv, bytes=need, dobs=dobs, Wd=Wd, P=P, nm=nm, fused=prob.fused)
 
     def close(self):
         for e in self.entries.values():
@@ -224,13 +315,13 @@ def _percentiles(x):
     if x.size == 0:
         return None
     return {"p50": float(np.percentile(x, 50)), "p90": float(np.percentile(x, 90)),
-            "p99": float(np.percentile(x, 99)), "max": float(x.max()), "mean": float(x.mean()),
-            "n": int(x.size)}
+            "p95": float(np.percentile(x, 95)), "p99": float(np.percentile(x, 99)),
+            "max": float(x.max()), "mean": float(x.mean()), "n": int(x.size)}
 
 
 def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahead: int = 32,
           progress=None, forward_method: str = "nodes", records: bool = False,
-          workers: int = 3, outputs: bool = False):
+          workers: int = 3, outputs: bool = False, max_batch: int = 8, policy: str = "gdsf"):
     """Replays `requests` in arrival order; returns a summary dict (times in ms;
     with `records`, also the per-request arrival / start / finish times; with
     `outputs`, each request's result by request id: the FP32 forward array,
@@ -239,35 +330,59 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
     `workers` server threads, each with its own engine context (stream),
     take the released requests in arrival order: while one request's kernel
     runs, the next one's inputs are uploaded, and a short request can run
-    beside a long one instead of waiting for it to finish."""
-    import queue as _queue
+    beside a long one instead of waiting for it to finish.  A worker that
+    takes a forward request also takes the forward requests queued right
+    behind it whose inputs are ready (up to `max_batch` in all) and serves
+    them in one launch (gra_forward_rectilinear_batch; the node-sum form
+    only), each result bit-identical to its own launch."""
     payloads, cv = {}, threading.Condition()
     stop = [False]
+    prod_err = []
 
     def producer():
-        for r in requests:
-            if r.kind != "forward":
-                continue
+        try:
+            for r in requests:
+                if r.kind != "forward":
+                    continue
+                with cv:
+                    while not stop[0] and len(payloads) >= lookahead:
+                        cv.wait()
+                    if stop[0]:
+                        return
+                item = forward_payload(r.params)
+                with cv:
+                    payloads[r.rid] = item
+                    cv.notify_all()
+        except BaseException as ex:                 # workers waiting on it must not hang
             with cv:
-                while not stop[0] and len(payloads) >= lookahead:
-                    cv.wait()
-                if stop[0]:
-                    return
-            item = forward_payload(r.params)
-            with cv:
-                payloads[r.rid] = item
+                prod_err.append(ex)
+                stop[0] = True
                 cv.notify_all()
 
+    def take_payload(rid):
+        with cv:
+            while rid not in payloads:
+                if prod_err:
+                    raise RuntimeError("request producer failed") from prod_err[0]
+                cv.wait()
+            q = payloads.pop(rid)
+            cv.notify_all()
+            return q
+
     workers = max(1, int(workers))
+    batching = forward_method == "nodes" and max_batch > 1
     fctxs = [Context(device) for _ in range(workers)]
-    cache = ResidentCache(pool, budget_bytes, device)
+    cache = ResidentCache(pool, budget_bytes, device, policy)
+    cache.prepare()
     th = threading.Thread(target=producer, daemon=True)
     th.start()
     rec, rec_lock = [], threading.Lock()
     results = {}
-    work = _queue.Queue()
+    pending, qcv = collections.deque(), threading.Condition()
+    closing = [False]
     errors = []
     t_start = [0.0]
+    n_batched = [0, 0]        # launches with more than one request, requests in them
 
     def run_forward(fctx, q, flag):
         if forward_method == "nodes":
@@ -275,48 +390,85 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
                                        ctx=fctx)
         return forward(q["prisms"], q["rho"], q["pts"], flag, ctx=fctx)
 
+    def next_work():
+        """The head request and, for a forward one, the forward requests
+        queued right behind it with ready inputs."""
+        with qcv:
+            while not pending and not closing[0]:
+                qcv.wait()
+            if not pending:
+                return None
+            group = [pending.popleft()]
+            if group[0].kind == "forward" and batching:
+                with cv:
+                    while (pending and len(group) < max_batch and pending[0].kind == "forward"
+                           and pending[0].rid in payloads):
+                        group.append(pending.popleft())
+            return group
+
+    def record(r, t0, t1, cold, size, result):
+        with rec_lock:
+            rec.append(dict(rid=r.rid, kind=r.kind, arrival=r.t_arrival, start=t0, finish=t1,
+                            cold=cold, **size))
+            if outputs:
+                results[r.rid] = result
+            n = len(rec)
+        if progress and n % progress == 0:
+            print(f"[serve] {n}/{len(requests)} requests, t = {t1:.1f} s", flush=True)
+
     def worker(fctx):
         while True:
-            r = work.get()
-            if r is None:
+            group = next_work()
+            if group is None:
                 return
             try:
+                r = group[0]
                 if r.kind == "forward":
-                    with cv:
-                        while r.rid not in payloads:
-                            cv.wait()
-                        q = payloads.pop(r.rid)
-                        cv.notify_all()
+                    qs = [take_payload(g.rid) for g in group]
                     t0 = time.perf_counter() - t_start[0]
-                    out = run_forward(fctx, q, r.params["flag"]).astype(np.float32)
-                    size = dict(cells=q["prisms"].shape[0], points=q["pts"].shape[0],
-                                comps=out.shape[1])
-                    cold = False
-                    result = out
+                    if len(group) == 1:
+                        outs = [run_forward(fctx, qs[0], r.params["flag"])]
+                    else:
+                        outs = forward_rectilinear_batch(
+                            [(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"], g.params["flag"])
+                             for q, g in zip(qs, group)], ctx=fctx)
+                        with rec_lock:
+                            n_batched[0] += 1
+                            n_batched[1] += len(group)
+                    t1 = time.perf_counter() - t_start[0]
+                    for g, q, out in zip(group, qs, outs):
+                        out = out.astype(np.float32)
+                        record(g, t0, t1, False, dict(cells=q["prisms"].shape[0],
+                                                      points=q["pts"].shape[0],
+                                                      comps=out.shape[1], batch=len(group)), out)
                 else:
                     t0 = time.perf_counter() - t_start[0]
                     info = {}
-                    e = cache.get(r.params["problem"], info)
+                    e = cache.get(r.params["problem"], info, stream_of=fctx)
                     try:
                         cold = info["admitted"]
                         zero = np.zeros(e["nm"])
-                        dx, err, its = e["inv"].solve_cg(0.0, 20, r.params["lam"], zero, zero,
-                                                         e["Wd"], e["dobs"], e["P"])
+                        # the solve runs on this worker's stream (one stream per
+                        # worker, whatever the number of resident problems)
+                        with e["lock"]:
+                            e["ctx"].use_stream_of(fctx)
+                            try:
+                                dx, err, its = e["inv"].solve_cg(
+                                    0.0, 20, r.params["lam"], zero, zero, e["Wd"], e["dobs"],
+                                    e["P"], fused=e["fused"])
+                            finally:
+                                e["ctx"].use_stream_of(None)
                     finally:
                         cache.release(e)
-                    result = dict(dx=dx, err=err, iterations=its)
-                    size = dict(problem=pool[r.params["problem"]].name)
-                t1 = time.perf_counter() - t_start[0]
-                with rec_lock:
-                    rec.append(dict(rid=r.rid, kind=r.kind, arrival=r.t_arrival, start=t0,
-                                    finish=t1, cold=cold, **size))
-                    if outputs:
-                        results[r.rid] = result
-                    n = len(rec)
-                if progress and n % progress == 0:
-                    print(f"[serve] {n}/{len(requests)} requests, t = {t1:.1f} s", flush=True)
+                    t1 = time.perf_counter() - t_start[0]
+                    record(r, t0, t1, cold, dict(problem=pool[r.params["problem"]].name),
+                           dict(dx=dx, err=err, iterations=its))
             except BaseException as ex:          # reported after the replay
                 errors.append(ex)
+                with qcv:                        # stop the replay: drain and close
+                    closing[0] = True
+                    pending.clear()
+                    qcv.notify_all()
 
     threads = []
     try:
@@ -324,18 +476,26 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         warm = forward_payload(dict(cells=64, npts=16, flag=0x3FE, height=10.0, seed=1))
         for fctx in fctxs:
             run_forward(fctx, warm, 0x3FE)
+            if batching:
+                forward_rectilinear_batch([(warm["xs"], warm["ys"], warm["zs"], warm["rho"],
+                                            warm["pts"], 0x3FE)] * 2, ctx=fctx)
         t_start[0] = time.perf_counter()
         threads = [threading.Thread(target=worker, args=(f,), daemon=True) for f in fctxs]
         for t in threads:
             t.start()
         for r in requests:
+            if errors:
+                break
             now = time.perf_counter() - t_start[0]
             if now < r.t_arrival:
                 time.sleep(r.t_arrival - now)
-            work.put(r)
+            with qcv:
+                pending.append(r)
+                qcv.notify()
     finally:
-        for _ in threads:
-            work.put(None)
+        with qcv:
+            closing[0] = True
+            qcv.notify_all()
         for t in threads:
             t.join()
         with cv:
@@ -343,12 +503,16 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
             cv.notify_all()
         th.join(timeout=10)
         cache_stats = dict(admissions=cache.admissions, evictions=cache.evictions,
-                           resident_gb=cache.used / 1e9)
+                           admission_s=cache.admit_s, resident_gb=cache.used / 1e9,
+                           budget_gb=cache.budget / 1e9, policy=cache.policy,
+                           pool_gb=sum(cache.g_bytes(p) for p in pool) / 1e9)
         cache.close()
         for fctx in fctxs:
             fctx.close()
     if errors:
         raise errors[0]
+    if prod_err:
+        raise RuntimeError("request producer failed") from prod_err[0]
     rec.sort(key=lambda x: x["arrival"])
     lat = [1e3 * (x["finish"] - x["arrival"]) for x in rec]
     svc = [1e3 * (x["finish"] - x["start"]) for x in rec]
@@ -359,6 +523,14 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         by[kind] = dict(
             latency_ms=_percentiles([1e3 * (x["finish"] - x["arrival"]) for x in sel]),
             service_ms=_percentiles([1e3 * (x["finish"] - x["start"]) for x in sel]))
+    by_problem = {}
+    for p in pool:
+        sel = [x for x in rec if x["kind"] == "cg" and x.get("problem") == p.name]
+        by_problem[p.name] = dict(
+            requests=len(sel), cold=sum(x["cold"] for x in sel),
+            warm_service_ms=_percentiles([1e3 * (x["finish"] - x["start"]) for x in sel
+                                          if not x["cold"]]),
+            latency_ms=_percentiles([1e3 * (x["finish"] - x["arrival"]) for x in sel]))
     warm_cg = [1e3 * (x["finish"] - x["start"]) for x in rec if x["kind"] == "cg" and not x["cold"]]
     return dict(
         requests=len(rec), makespan_s=makespan,
@@ -369,8 +541,9 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         utilization=sum(svc) / 1e3 / makespan / workers if makespan else 0.0,
         sustainable_rate=(1e3 / np.mean(svc) if svc else 0.0) if workers == 1 else None,
         latency_ms=_percentiles(lat), service_ms=_percentiles(svc), by_kind=by,
-        cg_warm_service_ms=_percentiles(warm_cg), cache=cache_stats,
+        cg_by_problem=by_problem, cg_warm_service_ms=_percentiles(warm_cg), cache=cache_stats,
         max_in_service=_max_overlap(rec),
+        forward_batches=dict(launches=n_batched[0], requests=n_batched[1]),
         **(dict(records=rec) if records else {}),
         **(dict(outputs=results) if outputs else {}))
 

@@ -70,6 +70,20 @@ gra_status gra_ctx_create(int device, gra_ctx **out);
 void gra_ctx_destroy(gra_ctx *ctx);
 /* Waits for all work queued on the context's stream. */
 gra_status gra_sync(gra_ctx *ctx);
+/*
+ * No counterpart in the reference (its products run on the host).  For
+ * callers that run several contexts' work from one thread of control on one
+ * stream, such as a server whose worker streams borrow the context of a
+ * resident problem: the context's later calls queue their work on `stream`
+ * (a hipStream_t of the context's device, owned by the caller) instead of
+ * its own; NULL returns it to its own stream.  The call first waits for the
+ * work queued on the stream in use.  Calls that return results wait for the
+ * stream they used; gra_G_vec_mul_noalloc is ordered on it only.  The caller
+ * keeps `stream` alive while the context uses it.
+ */
+gra_status gra_ctx_set_stream(gra_ctx *ctx, void *stream);
+/* The stream the context's work is queued on (a hipStream_t). */
+void *gra_ctx_stream(const gra_ctx *ctx);
 
 /* ---------------------------------------------------------- forward model */
 /*
@@ -261,6 +275,24 @@ gra_status gra_forward_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, c
                                    const double *pts, int64_t N_obs, uint32_t field_flag,
                                    double *out);
 
+/*
+ * Several gra_forward_rectilinear requests in one launch (no counterpart in
+ * the reference): the node weights of each item, then one workgroup per
+ * observation point of every item.  Each item's output is bit-identical to
+ * its own gra_forward_rectilinear call; the same argument rules per item.
+ */
+typedef struct gra_fwd_rect_item {
+    const double *xs; int64_t nx;
+    const double *ys; int64_t ny;
+    const double *zs; int64_t nz;
+    const double *rho;       /* [nx*ny*nz] */
+    const double *pts;       /* [N_obs*3]  */
+    int64_t N_obs;
+    uint32_t field_flag;
+    double *out;             /* [N_obs*n_comp] */
+} gra_fwd_rect_item;
+gra_status gra_forward_rectilinear_batch(gra_ctx *ctx, int64_t n, const gra_fwd_rect_item *items);
+
 /* ------------------------------------------------------------- inversion */
 /*
  * Uploads the regularisation operators used by gra_solve_cg: Ws, Wx, Wy, Wz
@@ -382,6 +414,21 @@ typedef struct gra_gn_setup {
 gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *setup);
 gra_status gra_gn_step(gra_ctx *ctx, double lambda, double cg_tol, int cg_maxit,
                        double *misfit, double *cg_err, int *cg_iterations);
+/*
+ * One gra_gn_step on each of n contexts (independent inversions of uneven
+ * sizes on one device, e.g. the lambda trials or instances of a batch;
+ * lambdas[b] for context b): every step is queued on its context's stream
+ * before any is waited for, so the instances' kernels overlap and no host
+ * synchronisation separates them; steps whose CG uses the single-read
+ * product run one after another (each needs the whole device).  Each
+ * context's results are bit-identical to its own gra_gn_step.  Outputs per
+ * context (may be null).  Returns the first failure (its context's
+ * gra_last_error says why).  No counterpart in the reference (its
+ * instances run one after another).
+ */
+gra_status gra_gn_step_batch(gra_ctx *const *ctxs, int64_t n, const double *lambdas,
+                             double cg_tol, int cg_maxit, double *misfits, double *cg_errs,
+                             int *cg_iterations);
 gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
 
 /* --------------------------------------------------------------- tuning */
@@ -421,6 +468,9 @@ gra_status gra_gn_get_model(gra_ctx *ctx, double *m);
                                    graph replays); 0 = every kernel launched
                                    on its own.  Not used by the fused or the
                                    mixed-precision iteration.             */
+#define GRA_TUNE_FUSED_MIN_BLOCKS 15 /* fused CG only where each row group of
+                                   the single-read grid holds at least this
+                                   many row blocks (else two-pass); 0 = any */
 #define GRA_TUNE_FUSED_FALLBACKS 11 /* get: fused solves rerun two-pass on this
                                    context (it then stays two-pass); set 0:
                                    clear and allow the fused iteration again */

@@ -33,10 +33,12 @@ extern "C" int hc_prism_field(const double *pt, const double *pr, double rho, ui
 #include "fused_plan.hpp"
 
 extern "C" int hc_fused_plan(int64_t Nd, int64_t ld, int TW, int RB, int cus, int n_xcd,
-                             int per_cu, int target, int max_poll, int64_t *out6)
+                             int per_cu, int target, int max_poll, int min_blocks, int64_t *out6)
 {
     gra::FusedShape s;
-    if (!gra::fused_plan_shape(Nd, ld, TW, RB, cus, n_xcd, per_cu, target, max_poll, &s)) return 0;
+    if (!gra::fused_plan_shape(Nd, ld, TW, RB, cus, n_xcd, per_cu, target, max_poll, &s,
+                               min_blocks))
+        return 0;
     out6[0] = s.NT;
     out6[1] = s.S;
     out6[2] = s.per_xcd;

@@ -23,15 +23,15 @@ def hc():
         build_hostcheck()
     lib = ctypes.CDLL(LIB)
     lib.hc_fused_plan.restype = ctypes.c_int
-    lib.hc_fused_plan.argtypes = [ctypes.c_int64, ctypes.c_int64] + [ctypes.c_int] * 7 + \
+    lib.hc_fused_plan.argtypes = [ctypes.c_int64, ctypes.c_int64] + [ctypes.c_int] * 8 + \
         [ctypes.POINTER(ctypes.c_int64)]
     return lib
 
 
-def plan(hc, nd, ld, store, cus=256, n_xcd=8, per_cu=1, target=0):
+def plan(hc, nd, ld, store, cus=256, n_xcd=8, per_cu=1, target=0, min_blocks=0):
     tw, rb = KERNELS[store]
     out = (ctypes.c_int64 * 6)()
-    ok = hc.hc_fused_plan(nd, ld, tw, rb, cus, n_xcd, per_cu, target, MAX_POLL, out)
+    ok = hc.hc_fused_plan(nd, ld, tw, rb, cus, n_xcd, per_cu, target, MAX_POLL, min_blocks, out)
     if not ok:
         return None
     return dict(zip(("NT", "S", "per_xcd", "rows", "nrb", "grid"), list(out))) | {"RB": rb, "TW": tw}
@@ -89,3 +89,12 @@ def test_plan_refuses_what_cannot_be_resident(hc):
     # a requested grid is capped at the co-resident limit
     p = plan(hc, 90000, 131072, "f64", target=100000)
     assert p["grid"] <= 256
+
+
+def test_plan_minimum_row_blocks(hc):
+    """With a minimum of row blocks per group the planner refuses short
+    groups (the iteration then runs two-pass) and keeps long ones."""
+    assert plan(hc, 5000, 32000, "bf16")["nrb"] == 20
+    assert plan(hc, 5000, 32000, "bf16", min_blocks=24) is None
+    assert plan(hc, 5000, 32000, "f64", min_blocks=24)["nrb"] == 79
+    assert plan(hc, 90000, 131072, "bf16", min_blocks=24)["nrb"] == 1407

@@ -1,5 +1,7 @@
-"""The serving harness: seeded request stream, payload shapes, residency LRU
-(CPU), and one short replay on the GPU checked against the oracle."""
+"""The serving harness: seeded request stream, payload shapes, the default
+problem pool, residency with LRU and GDSF eviction (CPU); short replays on
+the GPU checked against the oracle and against each request served alone,
+and batched forward launches against single ones."""
 import numpy as np
 import pytest
 
@@ -50,17 +52,40 @@ class _FakeCtx:
         self.log.append(self.name)
 
 
-def test_resident_cache_lru_admission_and_eviction():
+class _Sized(S.ResidentCache):
+    """Cache over fake problems of given sizes and admission times."""
+
+    def __init__(self, sizes, budget, policy, admit_s=None, closed=None):
+        super().__init__([S.CgProblem(f"p{k}", (1, 1, 1), (1.0, 1.0, 1.0), 1, 0x2, 1.0)
+                          for k in range(len(sizes))], budget, policy=policy)
+        self.sizes, self.admit_s_of = sizes, admit_s or [0.0] * len(sizes)
+        self.closed = [] if closed is None else closed
+        self.admitted = []
+
+    def g_bytes(self, prob):
+        return float(self.sizes[int(prob.name[1:])])
+
+    def _admit(self, prob, need, stream_of=None):
+        import time
+        k = int(prob.name[1:])
+        self.admitted.append(k)
+        time.sleep(self.admit_s_of[k])
+        return dict(ctx=_FakeCtx(self.closed, prob.name), bytes=need)
+
+
+def test_default_pool_covers_the_benchmark_sizes():
     pool = S.default_cg_pool()
-    closed = []
+    names = [p.name for p in pool]
+    assert names[0] == "inversion_small" and names[-1] == "tensor_large"
+    assert len(pool) == 66 and abs(sum(p.weight for p in pool) - 1.0) < 1e-12
+    gb = [S.ResidentCache.g_bytes(p) / 1e9 for p in pool]
+    assert abs(gb[0] - 1.28) < 1e-9 and abs(gb[-1] - 47.18592) < 1e-9
+    assert 113 < sum(gb[1:-1]) < 115                  # the 64 octree instances
+    assert pool[-1].store == "f32" and pool[-1].fused
 
-    class Cache(S.ResidentCache):
-        def _admit(self, prob, need):
-            return dict(ctx=_FakeCtx(closed, prob.name), bytes=need)
 
-    sizes = [S.ResidentCache.g_bytes(p) for p in pool]
-    budget = sizes[0] + sizes[1] + 1.0           # room for problems 0 and 1 only
-    c = Cache(pool, budget)
+def test_resident_cache_lru_admission_and_eviction():
+    c = _Sized([3.0, 4.0, 5.0], 3.0 + 4.0 + 1.0, "lru")    # room for 0 and 1 only
     info = {}
     c.release(c.get(0, info))
     assert info["admitted"]
@@ -68,43 +93,55 @@ def test_resident_cache_lru_admission_and_eviction():
     c.release(c.get(0, info))                    # hit; 1 is now least recent
     assert not info["admitted"]
     assert c.admissions == 2 and c.evictions == 0
-    c.release(c.get(2))                          # evicts 1 (LRU), maybe 0 too
-    assert closed[0] == pool[1].name
-    assert c.used <= budget
-    assert 2 in c.entries
+    c.release(c.get(2))                          # evicts 1 (LRU): 3 + 5 fits
+    assert c.closed == ["p1"]
+    assert c.used <= c.budget and list(c.entries) == [0, 2]
+    c.release(c.get(1))                          # evicts 0, then 2
+    assert c.closed == ["p1", "p0", "p2"] and list(c.entries) == [1]
     with pytest.raises(MemoryError):
-        S.ResidentCache(pool, 1.0).get(0)
+        _Sized([3.0], 1.0, "lru").get(0)
+    with pytest.raises(ValueError):
+        S.ResidentCache([], 1.0, policy="fifo")
+
+
+def test_gdsf_keeps_a_frequent_problem_that_is_expensive_to_rebuild():
+    """A 10-unit problem that takes 40x longer per byte to admit, asked for
+    every fifth request between four small ones, with room for it and three
+    small ones: LRU evicts it each cycle (it is the least recent when the
+    fourth small one comes), GDSF keeps it and rotates the small ones."""
+    sizes = [10.0, 1.0, 1.0, 1.0, 1.0]
+    admit_s = [0.2, 0.005, 0.005, 0.005, 0.005]
+    seq = [0, 1, 2, 3, 4] * 3
+    runs = {}
+    for policy in ("lru", "gdsf"):
+        c = _Sized(sizes, 13.0, policy, admit_s)
+        for pid in seq:
+            c.release(c.get(pid))
+            assert c.used <= c.budget
+        runs[policy] = c.admitted
+    assert runs["lru"].count(0) == 3
+    assert runs["gdsf"].count(0) == 1
+    assert len(runs["gdsf"]) < len(runs["lru"])
 
 
 def test_resident_cache_threads_share_one_admission_and_keep_busy_entries():
     """Two threads asking for the same absent problem admit it once; an
     entry in use is not evicted (the budget is exceeded until release)."""
     import threading
-    import time
-    pool = S.default_cg_pool()
-    closed, admitted = [], []
-
-    class Cache(S.ResidentCache):
-        def _admit(self, prob, need):
-            admitted.append(prob.name)
-            time.sleep(0.05)
-            return dict(ctx=_FakeCtx(closed, prob.name), bytes=need)
-
-    sizes = [S.ResidentCache.g_bytes(p) for p in pool]
-    c = Cache(pool, max(sizes) + 1.0)            # room for one problem at a time
+    c = _Sized([2.0, 2.0, 2.0], 3.0, "gdsf", [0.05, 0.0, 0.0])    # one problem at a time
     got = []
     ts = [threading.Thread(target=lambda: got.append(c.get(0))) for _ in range(2)]
     for t in ts:
         t.start()
     for t in ts:
         t.join()
-    assert admitted == [pool[0].name] and got[0] is got[1] and got[0]["busy"] == 2
+    assert c.admitted == [0] and got[0] is got[1] and got[0]["busy"] == 2
     e1 = c.get(1)                                # 0 is busy: kept, budget exceeded
-    assert closed == [] and 0 in c.entries and 1 in c.entries
+    assert c.closed == [] and 0 in c.entries and 1 in c.entries
     for e in got + [e1]:
         c.release(e)
     c.release(c.get(2))                          # now the idle 0 and 1 go
-    assert set(closed) == {pool[0].name, pool[1].name} and c.used <= c.budget
+    assert set(c.closed) == {"p0", "p1"} and c.used <= c.budget
 
 
 @pytest.mark.gpu
@@ -140,21 +177,34 @@ def test_short_replay_on_gpu():
 @pytest.mark.gpu
 def test_results_alone_equal_in_the_mix():
     """Each request's result in the concurrent mix (3 server streams with
-    requests overlapping on the device, problems admitted as memory allows)
-    equals its result served alone (one stream, one request on the GPU at a
-    time), bit for bit: the forward arrays and the CG solves' dx, error and
-    iteration count."""
-    pool = [S.CgProblem("tiny", (10, 10, 5), (1000.0, 1000.0, 500.0), 8, 0x12, 0.5),
-            S.CgProblem("tiny9", (10, 10, 5), (1000.0, 1000.0, 500.0), 6, 0x3FE, 0.5)]
-    reqs = S.request_stream(400.0, 0.1, 11, pool, p_forward=0.6)    # a burst
+    requests overlapping on the device, forward requests batched into shared
+    launches, problems admitted and evicted under a budget smaller than the
+    pool) equals its result served alone (one stream, one request on the GPU
+    at a time, no batching), bit for bit: the forward arrays and the CG
+    solves' dx, error and iteration count."""
+    from graada3d_amd.serving import _instances
+    inst = _instances()
+    size = [b.prisms.shape[0] * b.points.shape[0] * bin(b.flag).count("1") for b in inst]
+    two = [int(k) for k in np.argsort(size)[:2]]
+    pool = [S.CgProblem("tiny", (10, 10, 5), (1000.0, 1000.0, 500.0), 8, 0x12, 0.3),
+            S.CgProblem("tiny9", (10, 10, 5), (1000.0, 1000.0, 500.0), 6, 0x3FE, 0.2)]
+    pool += [S.CgProblem(f"batch_{k:02d}", (0, 0, 0), S.EXTENT, 0, 0, 0.25, instance=k)
+             for k in two]
+    gb = [S.ResidentCache.g_bytes(p) for p in pool]
+    budget = sum(gb) - 0.5 * min(gb[2:])          # the two instances do not fit together
+    reqs = S.request_stream(400.0, 0.15, 11, pool, p_forward=0.6)    # a burst
     for r in reqs:
         if r.kind == "forward":
             r.params.update(cells=min(r.params["cells"], 6000), npts=min(r.params["npts"], 1500))
     kinds = {r.rid: r.kind for r in reqs}
-    assert {"forward", "cg"} <= set(kinds.values())
-    mix = S.serve(reqs, pool, budget_bytes=1e9, workers=3, outputs=True)
-    alone = S.serve(reqs, pool, budget_bytes=1e9, workers=1, outputs=True)
+    probs = {r.params["problem"] for r in reqs if r.kind == "cg"}
+    assert {"forward", "cg"} <= set(kinds.values()) and {2, 3} <= probs
+    mix = S.serve(reqs, pool, budget_bytes=budget, workers=3, outputs=True)
+    alone = S.serve(reqs, pool, budget_bytes=budget, workers=1, max_batch=1, outputs=True)
     assert mix["max_in_service"] >= 2 and alone["max_in_service"] == 1
+    assert alone["forward_batches"]["launches"] == 0
+    print("mix:", mix["forward_batches"], mix["cache"])
+    assert mix["cache"]["evictions"] >= 1 and alone["cache"]["evictions"] >= 1
     assert set(mix["outputs"]) == set(alone["outputs"]) == set(kinds)
     for rid, a in alone["outputs"].items():
         b = mix["outputs"][rid]
@@ -163,3 +213,53 @@ def test_results_alone_equal_in_the_mix():
         else:
             assert np.array_equal(a["dx"], b["dx"]), rid
             assert a["err"] == b["err"] and a["iterations"] == b["iterations"] == 20, rid
+
+
+@pytest.mark.gpu
+def test_batched_forward_equals_single_launches():
+    """gra_forward_rectilinear_batch: uneven meshes, point sets and component
+    sets in one launch, each result bit-identical to its own launch."""
+    from graada3d_amd.engine import Context, forward_rectilinear, forward_rectilinear_batch
+    rng = np.random.default_rng(3)
+    qs, flags = [], []
+    for k in range(6):
+        p = dict(cells=int(rng.integers(500, 9000)), npts=int(rng.integers(20, 700)),
+                 flag=int(sum(1 << int(b) for b in rng.choice(np.arange(1, 10), 1 + k % 9,
+                                                               replace=False))),
+                 height=float(rng.uniform(1, 100)), seed=k)
+        qs.append(S.forward_payload(p))
+        flags.append(p["flag"])
+    with Context(0) as ctx:
+        single = [forward_rectilinear(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"], f, ctx=ctx)
+                  for q, f in zip(qs, flags)]
+        batched = forward_rectilinear_batch(
+            [(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"], f) for q, f in zip(qs, flags)],
+            ctx=ctx)
+        again = forward_rectilinear_batch(
+            [(q["xs"], q["ys"], q["zs"], q["rho"], q["pts"], f) for q, f in zip(qs[:2], flags[:2])],
+            ctx=ctx)
+    for a, b in zip(single, batched):
+        assert a.shape == b.shape and np.array_equal(a, b)
+    for a, b in zip(single, again):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_solves_on_a_borrowed_stream_are_bit_identical():
+    """gra_ctx_set_stream: a context's solves queued on another context's
+    stream give its own stream's results (two-pass and single-read)."""
+    from graada3d_amd.engine import Context
+    from test_gpu_cg import _cg_inputs, small_inversion
+    with Context(0) as ctx, Context(0) as other:
+        inv, G, aux = small_inversion(ctx)
+        args = _cg_inputs(inv, aux)[:4]
+        for fused in (False, True):
+            ref = inv.solve_cg(0.0, 30, 2.0, *args[:3], aux["dobs"], args[3], fused=fused)
+            ctx.use_stream_of(other)
+            try:
+                got = inv.solve_cg(0.0, 30, 2.0, *args[:3], aux["dobs"], args[3], fused=fused)
+                y = inv.G_vec_mul(np.ones(inv.Nm))
+            finally:
+                ctx.use_stream_of(None)
+            assert np.array_equal(got[0], ref[0]) and got[1:] == ref[1:], fused
+            assert np.array_equal(y, inv.G_vec_mul(np.ones(inv.Nm)))
