@@ -523,7 +523,8 @@ def run_serving(args):
           f"GB of G) under a {args.budget_gb:g} GB budget", file=sys.stderr, flush=True)
     res = S.serve(reqs, pool, budget_bytes=args.budget_gb * 1e9,
                   progress=max(50, int(args.rate * 10)), forward_method=args.forward_method,
-                  workers=args.workers, max_batch=args.max_batch, policy=args.policy)
+                  workers=args.workers, max_batch=args.max_batch, policy=args.policy,
+                  cg_workers=args.cg_workers)
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -546,6 +547,13 @@ def run_serving(args):
                          f"request; GPU: {res['by_kind']['forward']['service_ms']['mean']:.1f} ms "
                          f"mean forward service time", "all_cores": ac}
     fwd = res["by_kind"]["forward"]["latency_ms"]
+    # the served solves' error from FP32 storage, against FP64 two-pass
+    prec = {}
+    for name in ["inversion_small"] + sorted(_serving_extremes(pool)) + ["tensor_large"]:
+        prob = next(p for p in pool if p.name == name)
+        prec[name] = S.store_error(prob)
+        print(f"[bench] serving precision {name}: dx {prec[name]['dx_rel_err']:.2e}",
+              file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
         "value": fwd["p99"],
@@ -558,11 +566,13 @@ def run_serving(args):
                                f"(seed {args.seed}), 80% matrix-free forward (5k-130k cells, "
                                f"1k-20k points, 1-9 comps, FP32 out, {args.forward_method} "
                                f"form, up to {args.max_batch} per launch), 20% 20-iteration "
-                               f"CGLS on resident G (pool: inversion_small 45%, the 64 octree "
-                               f"instances 54%, tensor_large FP32 single-read 1%; "
+                               f"CGLS on resident FP32 G, single-read products (pool: "
+                               f"inversion_small 45%, the 64 octree instances 54%, "
+                               f"tensor_large 1%; "
                                f"{res['cache']['pool_gb']:.0f} GB of G under a "
-                               f"{args.budget_gb:g} GB {args.policy} budget), "
-                               f"{args.workers} server streams"},
+                               f"{args.budget_gb:g} GB {args.policy} budget, tensor_large "
+                               f"pinned), {args.workers} forward + {args.cg_workers} CG server "
+                               f"streams"},
         "latency_ms": res["latency_ms"], "service_ms": res["service_ms"],
         "by_kind": res["by_kind"], "cg_warm_service_ms": res["cg_warm_service_ms"],
         "cg_by_problem": {k: v for k, v in res["cg_by_problem"].items()
@@ -571,7 +581,10 @@ def run_serving(args):
         "offered_rate": res["offered_rate"], "achieved_rate": res["achieved_rate"],
         "utilization": res["utilization"],
         "mean_in_service": res["mean_in_service"], "workers": res["workers"],
+        "cg_workers": res["cg_workers"],
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
         "forward_batches": res["forward_batches"], "cache": res["cache"],
+        "cg_precision_vs_f64": prec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -858,9 +871,11 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: arrival window, s")
     ap.add_argument("--seed", type=int, default=20240605, help="serving: stream seed")
-    ap.add_argument("--workers", type=int, default=3,
-                    help="serving: server threads, each with its own stream")
-    ap.add_argument("--budget-gb", type=float, default=100.0,
+    ap.add_argument("--workers", type=int, default=4,
+                    help="serving: forward server threads, each with its own stream")
+    ap.add_argument("--cg-workers", type=int, default=1,
+                    help="serving: CG server threads (0: the forward threads take both kinds)")
+    ap.add_argument("--budget-gb", type=float, default=80.0,
                     help="serving: bytes of resident G the cache may hold (GB)")
     ap.add_argument("--policy", default="gdsf", choices=["gdsf", "lru"],
                     help="serving: eviction policy of the resident-G cache")
@@ -895,6 +910,12 @@ def main():
         run_probe(args)
         return
     if args.workload == "serving":
+        # one hardware queue per server stream (five by default; HIP's default
+        # is four, and streams that share a queue serialise their work: with
+        # four queues the forward p99 tripled); set before the runtime starts
+        streams = args.workers + args.cg_workers
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < streams:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(8, streams)))
         run_serving(args)
         return
     if args.workload == "batch":

@@ -51,6 +51,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "engine.hpp"
@@ -529,6 +530,19 @@ size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld)
            sizeof(double) * (size_t)p.S * (size_t)ld;               // zpart
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+// The last cooperative grid launched on each device (event recorded after
+// it, and the stream it was queued on); created on first use, kept until the
+// process ends.
+struct CoopOrder {
+    hipEvent_t last = nullptr;
+    hipStream_t stream = nullptr;
+};
+std::mutex coop_mu;
+CoopOrder coop_order[kMaxDevices];
+}  // namespace
+
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
                         const double *x, const double *Wd, double *tdat, void *scratch, int *err,
                         const int *done, int64_t fail_block, double **zpart_out,
@@ -561,12 +575,29 @@ hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t l
     const int64_t npart = nrb * p.NT * p.RB;
     hipLaunchKernelGGL(fill_u64, dim3((unsigned)((npart + 255) / 256)), dim3(256), 0, st,
                        reinterpret_cast<uint64_t *>(a.partial), npart, kSentinel);
-    void *args[] = {&a};
-    hipError_t e = hipLaunchCooperativeKernel(p.kernel, dim3((unsigned)p.grid), dim3(p.threads),
-                                              args, 0, st);
     if (zpart_out) *zpart_out = a.zpart;
     if (t2part_out) *t2part_out = a.t2part;
-    return e;
+    // One cooperative grid at a time per device, in launch order, whatever
+    // stream it is queued on: each grid is planned to hold every CU, so two
+    // in flight at once could keep each other's workgroups from becoming
+    // resident until their bounded waits fail.
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(coop_mu);
+    CoopOrder &o = coop_order[dev];
+    if (!o.last) {
+        if ((e = hipEventCreateWithFlags(&o.last, hipEventDisableTiming)) != hipSuccess) return e;
+    } else if (o.stream != st) {
+        if ((e = hipStreamWaitEvent(st, o.last, 0)) != hipSuccess) return e;
+    }
+    void *args[] = {&a};
+    e = hipLaunchCooperativeKernel(p.kernel, dim3((unsigned)p.grid), dim3(p.threads), args, 0, st);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventRecord(o.last, st)) != hipSuccess) return e;
+    o.stream = st;
+    return hipSuccess;
 }
 
 }  // namespace gra

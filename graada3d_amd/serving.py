@@ -23,10 +23,12 @@ Stream (seeded numpy PCG64):
   of idle problems by a size- and cost-aware policy).
 
 Request payloads are built ahead by a producer thread (the client's work);
-a dispatcher releases each request at its arrival time and `workers`
-server threads (default 3, each with its own context and stream) take them
-in that order, so up to `workers` requests are on the GPU at once; a CG
-solve runs on the worker's stream, borrowing the resident problem's context.
+a dispatcher releases each request at its arrival time into the queue of
+its kind; `workers` server threads (default 3) take the forward requests
+and `cg_workers` (default 1) the CG requests, each thread with its own
+context and stream, so up to workers + cg_workers requests are on the GPU at
+once; a CG solve runs on its worker's stream, borrowing the resident
+problem's context.
 """
 from __future__ import annotations
 
@@ -66,24 +68,38 @@ class CgProblem:
     store: str = "f64"             # storage of the resident G
     fused: bool = False            # single-read CG iteration
     instance: int = -1             # >= 0: index into workloads.batch_instances()
+    method: str = "faces"          # assembly: "faces", "nodes" or "corners" (compute_G)
+    pinned: bool = False           # once admitted, never evicted (warm() admits it first)
 
 
 def default_cg_pool():
-    """The CG problems of the serving mix at the benchmark workloads' sizes
-    (161 GB of G in all, more than a serving budget holds):
-    * the 5,000 x 32,000 survey (FP64, two passes; 1.3 GB), 45 % of the solves;
-    * each of the 64 uneven octree instances (FP64, two passes; 0.03-23 GB,
-      114 GB in all), 54 % of the solves spread evenly over them;
-    * the 90,000 x 131,072 nine-component survey (FP32 storage, single-read
-      iteration; 47 GB), 1 %."""
+    """The CG problems of the serving mix at the benchmark workloads' sizes,
+    each G stored in FP32 (105 GB of G in all, more than the default serving
+    budget holds) and assembled by corner sums (the node form on the two
+    regular meshes, per-prism corner sums on the octrees):
+    * the 5,000 x 32,000 survey (0.64 GB), 45 % of the solves;
+    * each of the 64 uneven octree instances (0.02-11.5 GB, 57 GB in all),
+      54 % of the solves spread evenly over them;
+    * the 90,000 x 131,072 nine-component survey (47 GB), 1 %, pinned:
+      admitting or evicting it maps or unmaps 47 GB, which stalled the
+      concurrent forward requests for hundreds of milliseconds.
+    The solves use the two-pass iteration: the single-read one is a
+    cooperative grid that holds every CU for each iteration, which stalls the
+    concurrent forward requests."""
     n_inst = len(_instances())
-    pool = [CgProblem("inversion_small", (40, 40, 20), (2000.0, 2000.0, 1000.0), 50, 0x12, 0.45)]
+    pool = [CgProblem("inversion_small", (40, 40, 20), (2000.0, 2000.0, 1000.0), 50, 0x12, 0.45,
+                      store="f32", method="nodes")]
     for k in range(n_inst):
         pool.append(CgProblem(f"batch_{k:02d}", (0, 0, 0), EXTENT, 0, 0, 0.54 / n_inst,
-                              instance=k))
+                              store="f32", instance=k, method="corners"))
     pool.append(CgProblem("tensor_large", (64, 64, 32), EXTENT, 100, 0x3FE, 0.01,
-                          store="f32", fused=True))
+                          store="f32", method="nodes", pinned=True))
     return pool
+
+
+# single-read solves only where each row group holds at least this many
+# row blocks (shorter groups run the two-pass iteration faster)
+FUSED_MIN_BLOCKS = 24
 
 
 _HOST = {}
@@ -151,7 +167,8 @@ def request_stream(rate: float, duration: float, seed: int, pool, p_forward: flo
 
 
 def forward_payload(p: dict):
-    """dict(xs, ys, zs, prisms, rho, pts) of a forward request."""
+    """dict(xs, ys, zs, prisms, rho, pts, evals) of a forward request (evals:
+    (node, point) pairs of its node-sum forward)."""
     cells, npts = p["cells"], p["npts"]
     nz = max(1, int(round((cells / 4.0) ** (1.0 / 3.0))))
     nx = max(1, int(round(np.sqrt(cells / nz))))
@@ -160,30 +177,45 @@ def forward_payload(p: dict):
     pts = M.grid_points(n, 0.0, EXTENT[0], n, 0.0, EXTENT[1], -p["height"])
     rho = np.random.default_rng(p["seed"]).uniform(-500.0, 500.0, prisms.shape[0])
     return dict(xs=M.linspaced(nx + 1, 0.0, EXTENT[0]), ys=M.linspaced(nx + 1, 0.0, EXTENT[1]),
-                zs=M.linspaced(nz + 1, 0.0, EXTENT[2]), prisms=prisms, rho=rho, pts=pts)
+                zs=M.linspaced(nz + 1, 0.0, EXTENT[2]), prisms=prisms, rho=rho, pts=pts,
+                evals=float((nx + 1) * (nx + 1) * (nz + 1) * pts.shape[0]))
 
 
 class ResidentCache:
     """Byte-budgeted residency of contexts, each holding one problem's G.
 
     A request for a problem that is not resident admits it (assembles its G,
-    counted in that request's latency), first evicting idle problems until
-    the budget holds.  Victims, by `policy`:
+    counted in that request's latency), first evicting idle problems that
+    are not pinned as the budget requires.  Victims, by `policy`:
     * "gdsf" (default; greedy-dual-size-frequency): the idle problem with
       the lowest H = L + hits * cost / bytes, where cost is the problem's
       measured admission time and L (the clock) rises to each victim's H, so
       that a large problem that is expensive to rebuild and asked for often
       stays, and problems not asked for lately age out;
-    * "lru": the least recently used idle problem."""
+    * "lru": the least recently used idle problem.
+
+    Device memory is recycled, not returned: an admission that needs room
+    assembles into the context of the first victim (in policy order) whose
+    G buffer holds its G.  Only when no victim's buffer does is a new one
+    allocated, after evicting victims and freeing their buffers until the
+    budget holds it.  A
+    hipMalloc / hipFree of a G of gigabytes stalls the kernels of every
+    stream on the device for up to a few hundred milliseconds (measured on
+    MI355X: a 12 GB allocate-free cycle held a concurrent 5 ms forward
+    request for 200 ms).  The budget bounds the G memory held, resident and
+    spare."""
 
     def __init__(self, pool, budget_bytes: float, device: int = 0, policy: str = "gdsf"):
         if policy not in ("gdsf", "lru"):
             raise ValueError(f"unknown eviction policy {policy!r}")
         self.pool, self.budget, self.device, self.policy = pool, budget_bytes, device, policy
         self.entries = collections.OrderedDict()
-        self.used = 0.0
+        self.spare = []                 # [capacity, ctx] of contexts holding no problem
+        self.used = 0.0                 # G capacity held, resident and spare
         self.admissions = 0
         self.evictions = 0
+        self.reuses = 0                 # admissions into a spare buffer
+        self.frees = 0                  # spare buffers returned to the device
         self.admit_s = 0.0
         self.clock = 0.0
         self.cost = {}                  # pid -> last measured admission time (s)
@@ -192,27 +224,98 @@ class ResidentCache:
 
     @staticmethod
     def g_bytes(prob: CgProblem) -> float:
+        """Bytes of the problem's G as the engine lays it out (rows of Nm
+        entries padded to a multiple of 64)."""
         b = {"f64": 8.0, "f32": 4.0, "bf16": 2.0}[prob.store]
         if prob.instance >= 0:
             inst = _instances()[prob.instance]
-            return b * inst.prisms.shape[0] * inst.points.shape[0] * bin(inst.flag).count("1")
-        nx, ny, nz = prob.shape
-        return b * nx * ny * nz * prob.n_pts ** 2 * bin(prob.flag).count("1")
+            nm, nd = inst.prisms.shape[0], inst.points.shape[0] * bin(inst.flag).count("1")
+        else:
+            nx, ny, nz = prob.shape
+            nm, nd = nx * ny * nz, prob.n_pts ** 2 * bin(prob.flag).count("1")
+        return b * nd * ((nm + 63) // 64 * 64)
 
     def prepare(self):
         """Builds every problem's host-side inputs (before a replay)."""
         for p in self.pool:
             problem_inputs(p)
 
+    def warm(self, stream_of=None):
+        """A server's start-up: admits the pinned problems, then the others,
+        most requested first, while they fit in the budget without evictions;
+        the admission counters then restart from zero.  Returns the number
+        admitted."""
+        n = 0
+        for pid in sorted(range(len(self.pool)),
+                          key=lambda k: (not self.pool[k].pinned, -self.pool[k].weight)):
+            if self.used + self.g_bytes(self.pool[pid]) <= self.budget:
+                self.release(self.get(pid, stream_of=stream_of))
+                n += 1
+        self.admissions, self.evictions, self.reuses, self.frees = 0, 0, 0, 0
+        self.admit_s = 0.0
+        return n
+
     def _value(self, pid, e):
         cost = self.cost.get(pid, e["bytes"] * 1e-11)     # ~ assembly time before measured
         return self.clock + e["hits"] * cost / e["bytes"]
 
     def _victims(self):
-        idle = [(pid, e) for pid, e in self.entries.items() if e.get("ready") and e["busy"] == 0]
+        idle = [(pid, e) for pid, e in self.entries.items()
+                if e.get("ready") and e["busy"] == 0 and not self.pool[pid].pinned]
         if self.policy == "gdsf":
             idle.sort(key=lambda t: t[1]["H"])
         return idle
+
+    def _evict(self, pid):
+        """Moves an idle resident problem's context to the spares (lock held)."""
+        old = self.entries.pop(pid)
+        self.spare.append([old["bytes"], old["ctx"]])
+        self.evictions += 1
+        if self.policy == "gdsf":
+            self.clock = max(self.clock, old["H"])
+
+    def _take_spare(self, need):
+        """The smallest spare holding `need` bytes, removed from the list."""
+        fit = [s for s in self.spare if s[0] >= need]
+        if not fit:
+            return None
+        best = min(fit, key=lambda s: s[0])
+        self.spare.remove(best)
+        return best
+
+    def _free_spare(self):
+        cap, ctx = self.spare.pop(0)
+        ctx.close()
+        self.used -= cap
+        self.frees += 1
+
+    def _place(self, need):
+        """[capacity, ctx or None] for an admission of `need` bytes (lock
+        held): the smallest spare that holds it, else new memory if the
+        budget has room, else the buffer of a victim that holds it, else new
+        memory after evicting and freeing; then spares are freed and victims
+        evicted until the budget holds what is kept (in-use problems may
+        have pushed it over)."""
+        got = self._take_spare(need)
+        if got is None and self.used + need <= self.budget:
+            got = [need, None]
+        if got is None:
+            for pid, e in self._victims():
+                if e["bytes"] >= need:
+                    self._evict(pid)
+                    got = self._take_spare(need)
+                    break
+        if got is None:
+            got = [need, None]
+        extra = need if got[1] is None else 0.0
+        while self.spare and self.used + extra > self.budget:
+            self._free_spare()
+        for pid, e in self._victims():
+            if self.used + extra <= self.budget:
+                break
+            self._evict(pid)
+            self._free_spare()
+        return got
 
     def get(self, pid: int, info: dict | None = None, stream_of=None):
         """The resident entry of problem pid, admitted if needed; the caller
@@ -236,39 +339,38 @@ class ResidentCache:
                 self.cv.wait()
             prob = self.pool[pid]
             need = self.g_bytes(prob)
-            if need > self.budget:
-                raise MemoryError(f"problem {prob.name} ({need / 1e9:.1f} GB) exceeds the budget")
-            # evict idle problems; in-use ones stay (the budget may be
-            # exceeded until they are released)
-            for old_pid, old in self._victims():
-                if self.used + need <= self.budget:
-                    break
-                del self.entries[old_pid]
-                old["ctx"].close()
-                self.used -= old["bytes"]
-                self.evictions += 1
-                if self.policy == "gdsf":
-                    self.clock = max(self.clock, old["H"])
-            placeholder = dict(ready=False, busy=1, bytes=need, hits=1, H=0.0,
+            pinned = sum(e["bytes"] for k, e in self.entries.items() if self.pool[k].pinned)
+            if need + (0.0 if prob.pinned else pinned) > self.budget:
+                raise MemoryError(f"problem {prob.name} ({need / 1e9:.1f} GB) exceeds the budget "
+                                  f"beside the pinned problems ({pinned / 1e9:.1f} GB)")
+            cap, ctx = self._place(need)
+            if ctx is None:
+                self.used += cap
+            else:
+                self.reuses += 1
+            placeholder = dict(ready=False, busy=1, bytes=cap, hits=1, H=0.0,
                                lock=threading.Lock())
             if info is not None:
                 info["admitted"] = True
             self.entries[pid] = placeholder
-            self.used += need
             self.admissions += 1
         t0 = time.perf_counter()
         try:
-            e = self._admit(prob, need, stream_of)
+            e = self._admit(prob, need, stream_of, ctx)
         except BaseException:
             with self.lock:
                 del self.entries[pid]
-                self.used -= need
+                if ctx is not None:
+                    self.spare.append([cap, ctx])
+                else:
+                    self.used -= cap
                 self.cv.notify_all()
             raise
         dt = time.perf_counter() - t0
         with self.lock:
             self.admit_s += dt
             self.cost[pid] = dt
+            e["bytes"] = cap
             placeholder.update(e, ready=True)
             placeholder["H"] = self._value(pid, placeholder)
             self.cv.notify_all()
@@ -278,14 +380,18 @@ class ResidentCache:
         with self.lock:
             e["busy"] -= 1
 
-    def _admit(self, prob: CgProblem, need: float, stream_of=None):
+    def _admit(self, prob: CgProblem, need: float, stream_of=None, ctx=None):
+        """Assembles the problem into `ctx` (a spare context whose G buffer
+        holds it) or a new context; returns the entry's fields."""
         h = problem_inputs(prob)
-        ctx = Context(self.device)
+        fresh = ctx is None
+        ctx = Context(self.device) if fresh else ctx
         try:
+            ctx.tune(fused_min_blocks=FUSED_MIN_BLOCKS)
             if stream_of is not None:
                 ctx.use_stream_of(stream_of)
             inv = GaussNewtonInversion(h["prisms"], h["pts"], h["flag"], ctx=ctx)
-            inv.compute_G(prob.store)
+            inv.compute_G(prob.store, prob.method)
             inv.set_regularization(*h["W"])
             nobs = h["pts"].shape[0]
             dobs, sigma = add_noise(inv.G_vec_mul(h["rho"]), nobs, bin(h["flag"]).count("1"), 7)
@@ -295,7 +401,8 @@ class ResidentCache:
             if stream_of is not None:
                 ctx.use_stream_of(None)
         except BaseException:
-            ctx.close()
+            if fresh:
+                ctx.close()
             raise
         Wd = np.repeat(1.0 / sigma, nobs)
         nm = h["prisms"].shape[0]
@@ -306,8 +413,37 @@ class ResidentCache:
         for e in self.entries.values():
             if e.get("ready"):
                 e["ctx"].close()
+        for _, ctx in self.spare:
+            ctx.close()
         self.entries.clear()
+        self.spare = []
         self.used = 0.0
+
+
+def store_error(prob: CgProblem, lams=(0.1, 1.0, 10.0, 100.0), device: int = 0):
+    """The served solve's error from its G assembly, storage and product: dx
+    of the 20-iteration solve on the problem as served (prob.method,
+    prob.store, prob.fused) against the FP64 two-pass solve on the face/edge
+    G from the same inputs (that G's data), max over `lams` of
+    ||dx - dx_f64|| / ||dx_f64||, and of the relative difference of the
+    returned CG error measure."""
+    import dataclasses
+    both = [prob, dataclasses.replace(prob, store="f64", fused=False, method="faces")]
+    cache = ResidentCache(both, 1e15, device)
+    try:
+        a, b = cache.get(0), cache.get(1)
+        zero = np.zeros(b["nm"])
+        e_dx = e_err = 0.0
+        for lam in lams:
+            ref = b["inv"].solve_cg(0.0, 20, lam, zero, zero, b["Wd"], b["dobs"], b["P"])
+            got = a["inv"].solve_cg(0.0, 20, lam, zero, zero, b["Wd"], b["dobs"], b["P"],
+                                    fused=a["fused"])
+            e_dx = max(e_dx, float(np.linalg.norm(got[0] - ref[0]) / np.linalg.norm(ref[0])))
+            e_err = max(e_err, abs(got[1] - ref[1]) / abs(ref[1]))
+        return dict(dx_rel_err=e_dx, cg_err_rel_diff=e_err, store=prob.store, fused=prob.fused,
+                    method=prob.method, lambdas=list(lams))
+    finally:
+        cache.close()
 
 
 def _percentiles(x):
@@ -321,7 +457,9 @@ def _percentiles(x):
 
 def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahead: int = 32,
           progress=None, forward_method: str = "nodes", records: bool = False,
-          workers: int = 3, outputs: bool = False, max_batch: int = 8, policy: str = "gdsf"):
+          workers: int = 3, outputs: bool = False, max_batch: int = 8, policy: str = "gdsf",
+          batch_evals: float = 2e8, cg_order: str = "sjf",
+          cg_workers: int = 1, prewarm: bool = True):
     """Replays `requests` in arrival order; returns a summary dict (times in ms;
     with `records`, also the per-request arrival / start / finish times; with
     `outputs`, each request's result by request id: the FP32 forward array,
@@ -330,11 +468,25 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
     `workers` server threads, each with its own engine context (stream),
     take the released requests in arrival order: while one request's kernel
     runs, the next one's inputs are uploaded, and a short request can run
-    beside a long one instead of waiting for it to finish.  A worker that
-    takes a forward request also takes the forward requests queued right
-    behind it whose inputs are ready (up to `max_batch` in all) and serves
-    them in one launch (gra_forward_rectilinear_batch; the node-sum form
-    only), each result bit-identical to its own launch."""
+    beside a long one instead of waiting for it to finish.  With
+    `cg_workers` > 0, those threads take the forward requests only and
+    `cg_workers` more take the CG requests from a queue of their own, so that
+    a forward request never waits in the queue behind solves (the two kinds
+    still share the device: the forward kernels are FP64-issue bound, the
+    solves HBM bound); with 0, every thread takes both kinds.  The CG
+    threads take the queued solve with the smallest G first (`cg_order`
+    "sjf"; "fifo": arrival order), so that the many solves on small G do not
+    wait behind one on a 47 GB G.  A worker that
+    takes a small forward request also takes the small forward requests
+    queued right behind it whose inputs are ready (up to `max_batch` in all,
+    each at most a quarter and all together at most `batch_evals`
+    (node, point) evaluations) and serves them in one launch
+    (gra_forward_rectilinear_batch; the node-sum form only), each result
+    bit-identical to its own launch.  Large requests go alone: a batch ends
+    when its largest request does, so batching a large request with small
+    ones would hold the small ones' results back.  With `prewarm`,
+    the cache is filled before the first arrival (ResidentCache.warm), as a
+    server loads its resident problems at start-up."""
     payloads, cv = {}, threading.Condition()
     stop = [False]
     prod_err = []
@@ -370,15 +522,23 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
             return q
 
     workers = max(1, int(workers))
+    cg_workers = max(0, int(cg_workers))
+    if cg_order not in ("sjf", "fifo"):
+        raise ValueError(f"unknown CG order {cg_order!r}")
+    g_size = [ResidentCache.g_bytes(p) for p in pool]
     batching = forward_method == "nodes" and max_batch > 1
-    fctxs = [Context(device) for _ in range(workers)]
+    fctxs = [Context(device) for _ in range(workers + cg_workers)]
     cache = ResidentCache(pool, budget_bytes, device, policy)
     cache.prepare()
     th = threading.Thread(target=producer, daemon=True)
     th.start()
     rec, rec_lock = [], threading.Lock()
     results = {}
-    pending, qcv = collections.deque(), threading.Condition()
+    # one queue for both kinds, or a forward queue and a CG queue
+    split = cg_workers > 0
+    queues = {"forward": collections.deque(), "cg": collections.deque()} if split else \
+        {"all": collections.deque()}
+    qcv = threading.Condition()
     closing = [False]
     errors = []
     t_start = [0.0]
@@ -390,19 +550,31 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
                                        ctx=fctx)
         return forward(q["prisms"], q["rho"], q["pts"], flag, ctx=fctx)
 
-    def next_work():
-        """The head request and, for a forward one, the forward requests
-        queued right behind it with ready inputs."""
+    def next_work(pending):
+        """The head request of `pending` and, for a forward one, the forward
+        requests queued right behind it with ready inputs."""
         with qcv:
             while not pending and not closing[0]:
                 qcv.wait()
             if not pending:
                 return None
+            if split and cg_order == "sjf" and pending is queues["cg"]:
+                k = min(range(len(pending)),
+                        key=lambda q: (g_size[pending[q].params["problem"]], q))
+                r = pending[k]
+                del pending[k]
+                return [r]
             group = [pending.popleft()]
             if group[0].kind == "forward" and batching:
+                small = 0.25 * batch_evals
                 with cv:
-                    while (pending and len(group) < max_batch and pending[0].kind == "forward"
-                           and pending[0].rid in payloads):
+                    head = payloads.get(group[0].rid)
+                    total = head["evals"] if head is not None else batch_evals + 1.0
+                    while (total <= small and pending and len(group) < max_batch
+                           and pending[0].kind == "forward" and pending[0].rid in payloads
+                           and payloads[pending[0].rid]["evals"] <= small
+                           and total + payloads[pending[0].rid]["evals"] <= batch_evals):
+                        total += payloads[pending[0].rid]["evals"]
                         group.append(pending.popleft())
             return group
 
@@ -416,9 +588,9 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         if progress and n % progress == 0:
             print(f"[serve] {n}/{len(requests)} requests, t = {t1:.1f} s", flush=True)
 
-    def worker(fctx):
+    def worker(fctx, pending):
         while True:
-            group = next_work()
+            group = next_work(pending)
             if group is None:
                 return
             try:
@@ -467,10 +639,12 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
                 errors.append(ex)
                 with qcv:                        # stop the replay: drain and close
                     closing[0] = True
-                    pending.clear()
+                    for q in queues.values():
+                        q.clear()
                     qcv.notify_all()
 
     threads = []
+    warmed, t_origin_ns = 0, 0
     try:
         # warm each forward context once (first-call module loading is not a request)
         warm = forward_payload(dict(cells=64, npts=16, flag=0x3FE, height=10.0, seed=1))
@@ -479,8 +653,13 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
             if batching:
                 forward_rectilinear_batch([(warm["xs"], warm["ys"], warm["zs"], warm["rho"],
                                             warm["pts"], 0x3FE)] * 2, ctx=fctx)
+        warmed = cache.warm(fctxs[-1]) if prewarm else 0
         t_start[0] = time.perf_counter()
-        threads = [threading.Thread(target=worker, args=(f,), daemon=True) for f in fctxs]
+        t_origin_ns = time.monotonic_ns()
+        owner = [queues["forward"]] * workers + [queues["cg"]] * cg_workers if split else \
+            [queues["all"]] * workers
+        threads = [threading.Thread(target=worker, args=(f, q), daemon=True)
+                   for f, q in zip(fctxs, owner)]
         for t in threads:
             t.start()
         for r in requests:
@@ -490,8 +669,8 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
             if now < r.t_arrival:
                 time.sleep(r.t_arrival - now)
             with qcv:
-                pending.append(r)
-                qcv.notify()
+                queues[r.kind if split else "all"].append(r)
+                qcv.notify_all()
     finally:
         with qcv:
             closing[0] = True
@@ -502,7 +681,8 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
             stop[0] = True
             cv.notify_all()
         th.join(timeout=10)
-        cache_stats = dict(admissions=cache.admissions, evictions=cache.evictions,
+        cache_stats = dict(prewarmed=warmed, admissions=cache.admissions,
+                           evictions=cache.evictions, reuses=cache.reuses, frees=cache.frees,
                            admission_s=cache.admit_s, resident_gb=cache.used / 1e9,
                            budget_gb=cache.budget / 1e9, policy=cache.policy,
                            pool_gb=sum(cache.g_bytes(p) for p in pool) / 1e9)
@@ -536,15 +716,16 @@ def serve(requests, pool, budget_bytes: float = 100e9, device: int = 0, lookahea
         requests=len(rec), makespan_s=makespan,
         offered_rate=len(rec) / requests[-1].t_arrival if rec else 0.0,
         achieved_rate=len(rec) / makespan if makespan else 0.0,
-        workers=workers,
+        workers=workers, cg_workers=cg_workers,
         mean_in_service=sum(svc) / 1e3 / makespan if makespan else 0.0,
-        utilization=sum(svc) / 1e3 / makespan / workers if makespan else 0.0,
-        sustainable_rate=(1e3 / np.mean(svc) if svc else 0.0) if workers == 1 else None,
+        utilization=sum(svc) / 1e3 / makespan / (workers + cg_workers) if makespan else 0.0,
+        sustainable_rate=((1e3 / np.mean(svc) if svc else 0.0)
+                          if workers + cg_workers == 1 else None),
         latency_ms=_percentiles(lat), service_ms=_percentiles(svc), by_kind=by,
         cg_by_problem=by_problem, cg_warm_service_ms=_percentiles(warm_cg), cache=cache_stats,
         max_in_service=_max_overlap(rec),
         forward_batches=dict(launches=n_batched[0], requests=n_batched[1]),
-        **(dict(records=rec) if records else {}),
+        **(dict(records=rec, t_origin_ns=t_origin_ns) if records else {}),
         **(dict(outputs=results) if outputs else {}))
 
 

@@ -190,3 +190,46 @@ def test_fused_gauss_newton_l1_irls_vs_oracle(ctx):
         assert its == itso == 15
         assert abs(mis - miso) < 1e-10 * miso, k
         assert rel_norm(inv.gn_model(), m) < 1e-10, k
+
+
+def test_concurrent_single_read_solves_from_several_streams():
+    """Single-read solves queued at once from three threads on three contexts
+    (three streams): cooperative grids of one device run one at a time, so
+    none of their bounded waits fails, and every result equals the same
+    solve run alone, bit for bit."""
+    import threading
+    from graada3d_amd.engine import Context
+    ctxs = [Context(0) for _ in range(3)]
+    try:
+        probs = []
+        for k, c in enumerate(ctxs):
+            inv, G, aux = small_inversion(c, nx=16 + 4 * k, ny=12, nz=8, npt=14, seed=k)
+            probs.append((inv, aux, _cg_inputs(inv, aux)[:4]))
+        alone = [inv.solve_cg(0.0, 40, 2.0, *a[:3], aux["dobs"], a[3], fused=True)
+                 for inv, aux, a in probs]
+        got = [[None] * 6 for _ in probs]
+        errs = []
+
+        def run(q):
+            inv, aux, a = probs[q]
+            try:
+                for rep in range(6):
+                    got[q][rep] = inv.solve_cg(0.0, 40, 2.0, *a[:3], aux["dobs"], a[3],
+                                               fused=True)
+            except BaseException as ex:
+                errs.append(ex)
+
+        ts = [threading.Thread(target=run, args=(q,)) for q in range(3)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert not errs, errs
+        for q, (inv, aux, a) in enumerate(probs):
+            assert inv.ctx.tuning()["fused_fallbacks"] == 0
+            for rep in range(6):
+                dx, err, its = got[q][rep]
+                assert np.array_equal(dx, alone[q][0]) and (err, its) == alone[q][1:], (q, rep)
+    finally:
+        for c in ctxs:
+            c.close()

@@ -65,12 +65,15 @@ class _Sized(S.ResidentCache):
     def g_bytes(self, prob):
         return float(self.sizes[int(prob.name[1:])])
 
-    def _admit(self, prob, need, stream_of=None):
+    def _admit(self, prob, need, stream_of=None, ctx=None):
         import time
         k = int(prob.name[1:])
         self.admitted.append(k)
         time.sleep(self.admit_s_of[k])
-        return dict(ctx=_FakeCtx(self.closed, prob.name), bytes=need)
+        if ctx is None:
+            ctx = _FakeCtx(self.closed, f"buf{len(self.admitted)}")
+        ctx.holds = prob.name
+        return dict(ctx=ctx, bytes=need)
 
 
 def test_default_pool_covers_the_benchmark_sizes():
@@ -78,10 +81,12 @@ def test_default_pool_covers_the_benchmark_sizes():
     names = [p.name for p in pool]
     assert names[0] == "inversion_small" and names[-1] == "tensor_large"
     assert len(pool) == 66 and abs(sum(p.weight for p in pool) - 1.0) < 1e-12
+    assert all(p.store == "f32" and not p.fused for p in pool)
+    assert [p.method for p in pool] == ["nodes"] + ["corners"] * 64 + ["nodes"]
+    assert [p.name for p in pool if p.pinned] == ["tensor_large"]
     gb = [S.ResidentCache.g_bytes(p) / 1e9 for p in pool]
-    assert abs(gb[0] - 1.28) < 1e-9 and abs(gb[-1] - 47.18592) < 1e-9
-    assert 113 < sum(gb[1:-1]) < 115                  # the 64 octree instances
-    assert pool[-1].store == "f32" and pool[-1].fused
+    assert abs(gb[0] - 0.64) < 1e-9 and abs(gb[-1] - 47.18592) < 1e-9
+    assert 56 < sum(gb[1:-1]) < 58                    # the 64 octree instances
 
 
 def test_resident_cache_lru_admission_and_eviction():
@@ -93,15 +98,46 @@ def test_resident_cache_lru_admission_and_eviction():
     c.release(c.get(0, info))                    # hit; 1 is now least recent
     assert not info["admitted"]
     assert c.admissions == 2 and c.evictions == 0
-    c.release(c.get(2))                          # evicts 1 (LRU): 3 + 5 fits
-    assert c.closed == ["p1"]
-    assert c.used <= c.budget and list(c.entries) == [0, 2]
-    c.release(c.get(1))                          # evicts 0, then 2
-    assert c.closed == ["p1", "p0", "p2"] and list(c.entries) == [1]
+    # 2 (5) fits no buffer: 1 is evicted and its buffer freed (3 + 5 fits)
+    c.release(c.get(2))
+    assert c.closed == ["buf2"] and list(c.entries) == [0, 2] and c.used == 8.0
+    # 1 (4) goes into the buffer of a victim that holds it: 2's (5), LRU
+    # order 0 (3, too small), then 2; nothing is freed
+    c.release(c.get(1))
+    assert c.closed == ["buf2"] and list(c.entries) == [0, 1]
+    assert c.entries[1]["ctx"].name == "buf3" and c.entries[1]["bytes"] == 5.0
+    assert c.reuses == 1 and c.frees == 1 and c.used == 8.0 <= c.budget
     with pytest.raises(MemoryError):
         _Sized([3.0], 1.0, "lru").get(0)
     with pytest.raises(ValueError):
         S.ResidentCache([], 1.0, policy="fifo")
+
+
+def test_warm_fills_the_budget_most_requested_first():
+    c = _Sized([3.0, 4.0, 5.0, 1.0], 8.5, "gdsf")
+    for p, w in zip(c.pool, (0.4, 0.1, 0.3, 0.2)):
+        p.weight = w
+    assert c.warm() == 2                         # 0 (3), then 2 (5); 3 no longer fits
+    assert sorted(c.entries) == [0, 2] and c.admitted == [0, 2]
+    assert c.admissions == 0 and c.evictions == 0 and c.used == 8.0
+
+
+def test_pinned_problems_are_warmed_first_and_never_evicted():
+    c = _Sized([6.0, 2.0, 2.0, 3.0], 10.0, "lru")
+    c.pool[0].pinned = True
+    c.pool[0].weight = 0.0
+    assert c.warm() == 3 and c.admitted == [0, 1, 2]       # pinned first, then by weight
+    c.release(c.get(3))                                    # frees 1's and 2's buffers, not 0's
+    assert c.closed == ["buf2", "buf3"] and sorted(c.entries) == [0, 3]
+    c.release(c.get(2))                                    # into 3's buffer
+    assert sorted(c.entries) == [0, 2] and c.entries[2]["ctx"].name == "buf4"
+    assert c.closed == ["buf2", "buf3"] and c.used == 9.0
+    c2 = _Sized([6.0, 5.0], 10.0, "gdsf")
+    c2.pool[0].pinned = True
+    c2.release(c2.get(0))
+    with pytest.raises(MemoryError):                       # 6 pinned + 5 > 10
+        c2.get(1)
+    assert list(c2.entries) == [0] and c2.used == 6.0
 
 
 def test_gdsf_keeps_a_frequent_problem_that_is_expensive_to_rebuild():
@@ -118,10 +154,10 @@ def test_gdsf_keeps_a_frequent_problem_that_is_expensive_to_rebuild():
         for pid in seq:
             c.release(c.get(pid))
             assert c.used <= c.budget
-        runs[policy] = c.admitted
-    assert runs["lru"].count(0) == 3
-    assert runs["gdsf"].count(0) == 1
-    assert len(runs["gdsf"]) < len(runs["lru"])
+        runs[policy] = (c.admitted, sum(admit_s[k] for k in c.admitted))
+    assert runs["lru"][0].count(0) == 3
+    assert runs["gdsf"][0].count(0) == 1
+    assert runs["gdsf"][1] < 0.5 * runs["lru"][1]            # admission time spent
 
 
 def test_resident_cache_threads_share_one_admission_and_keep_busy_entries():
@@ -137,11 +173,13 @@ def test_resident_cache_threads_share_one_admission_and_keep_busy_entries():
         t.join()
     assert c.admitted == [0] and got[0] is got[1] and got[0]["busy"] == 2
     e1 = c.get(1)                                # 0 is busy: kept, budget exceeded
-    assert c.closed == [] and 0 in c.entries and 1 in c.entries
+    assert c.closed == [] and 0 in c.entries and 1 in c.entries and c.used == 4.0
     for e in got + [e1]:
         c.release(e)
-    c.release(c.get(2))                          # now the idle 0 and 1 go
-    assert set(c.closed) == {"p0", "p1"} and c.used <= c.budget
+    # 2 goes into an idle victim's buffer, and the other idle one is freed
+    # to bring the held memory back under the budget
+    c.release(c.get(2))
+    assert list(c.entries) == [2] and len(c.closed) == 1 and c.used == 2.0 <= c.budget
 
 
 @pytest.mark.gpu
@@ -176,8 +214,8 @@ def test_short_replay_on_gpu():
 
 @pytest.mark.gpu
 def test_results_alone_equal_in_the_mix():
-    """Each request's result in the concurrent mix (3 server streams with
-    requests overlapping on the device, forward requests batched into shared
+    """Each request's result in the concurrent mix (3 forward and 1 CG server
+    streams with requests overlapping on the device, forward requests batched into shared
     launches, problems admitted and evicted under a budget smaller than the
     pool) equals its result served alone (one stream, one request on the GPU
     at a time, no batching), bit for bit: the forward arrays and the CG
@@ -200,7 +238,8 @@ def test_results_alone_equal_in_the_mix():
     probs = {r.params["problem"] for r in reqs if r.kind == "cg"}
     assert {"forward", "cg"} <= set(kinds.values()) and {2, 3} <= probs
     mix = S.serve(reqs, pool, budget_bytes=budget, workers=3, outputs=True)
-    alone = S.serve(reqs, pool, budget_bytes=budget, workers=1, max_batch=1, outputs=True)
+    alone = S.serve(reqs, pool, budget_bytes=budget, workers=1, cg_workers=0, max_batch=1,
+                    outputs=True)
     assert mix["max_in_service"] >= 2 and alone["max_in_service"] == 1
     assert alone["forward_batches"]["launches"] == 0
     print("mix:", mix["forward_batches"], mix["cache"])
