@@ -566,8 +566,8 @@ def run_serving(args):
                                f"(seed {args.seed}), 80% matrix-free forward (5k-130k cells, "
                                f"1k-20k points, 1-9 comps, FP32 out, {args.forward_method} "
                                f"form, up to {args.max_batch} per launch), 20% 20-iteration "
-                               f"CGLS on resident FP32 G, single-read products (pool: "
-                               f"inversion_small 45%, the 64 octree instances 54%, "
+                               f"CGLS on resident FP32 corner-sum G, two-pass products "
+                               f"(pool: inversion_small 45%, the 64 octree instances 54%, "
                                f"tensor_large 1%; "
                                f"{res['cache']['pool_gb']:.0f} GB of G under a "
                                f"{args.budget_gb:g} GB {args.policy} budget, tensor_large "

@@ -124,6 +124,13 @@ bool prisms_valid(const double *pr, int64_t n)
     return true;
 }
 
+double max_point_z(const double *pts, int64_t n)
+{
+    double z = pts[2];
+    for (int64_t i = 1; i < n; ++i) z = std::max(z, pts[3 * i + 2]);
+    return z;
+}
+
 bool points_valid(const double *pt, int64_t n)
 {
     for (int64_t i = 0; i < 3 * n; ++i)
@@ -372,6 +379,7 @@ gra_status gra_assemble(gra_ctx *ctx, const double *prisms, int64_t Nm, const do
     HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream), "status");
     HIPCHK(hipStreamSynchronize(c.stream), "assemble");
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    c.g_corners = false;
     ++c.g_version;
     if (hbad)
         return set_error(c, GRA_DEGENERATE_GEOMETRY,
@@ -432,6 +440,8 @@ gra_status gra_assemble_rectilinear(gra_ctx *ctx, const double *xs, int64_t nx, 
     }
     HIPCHK(hipStreamSynchronize(c.stream), "assemble");
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    c.g_corners = true;
+    c.pts_zmax = max_point_z(pts, N_obs);
     ++c.g_version;
     return GRA_OK;
 }
@@ -478,6 +488,8 @@ gra_status gra_assemble_corners(gra_ctx *ctx, const double *prisms, int64_t Nm, 
     }
     HIPCHK(hipStreamSynchronize(c.stream), "assemble");
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = ld; c.flag = field_flag; c.store = store;
+    c.g_corners = true;
+    c.pts_zmax = max_point_z(pts, N_obs);
     ++c.g_version;
     return GRA_OK;
 }
@@ -543,6 +555,7 @@ gra_status gra_assemble_wavelet(gra_ctx *ctx, const double *prisms, int64_t Nm, 
     c.Nd = Nd; c.Nm = Nm; c.N_obs = N_obs; c.ld = (Nm + 63) / 64 * 64; c.flag = field_flag;
     c.store = GRA_F64;
     c.wav = true;
+    c.g_corners = false;
     ++c.g_version;
     if (hbad)
         return set_error(c, GRA_DEGENERATE_GEOMETRY,
@@ -761,6 +774,8 @@ gra_status assemble_batch(gra_ctx *const *ctxs, int64_t n, const gra_batch_item 
         cb.ld = tab[(size_t)b].ld;
         cb.flag = items[b].field_flag;
         cb.store = store;
+        cb.g_corners = corners;
+        cb.pts_zmax = max_point_z(items[b].pts, items[b].N_obs);
         ++cb.g_version;
     }
     for (int64_t b = 0; b < n; ++b)
@@ -799,6 +814,14 @@ gra_status gra_expand_cols(gra_ctx *ctx, const int64_t *split_cols, int64_t n_sp
     const int64_t nchild = 8 * n_split;
     if (!prisms_valid(child_prisms, nchild))
         return set_error(c, GRA_BAD_ARG, "child prism bounds must be finite with x0<x1, y0<y1, z0<z1");
+    // the children of a corner-sum G are corner sums too (the two forms
+    // differ where the face/edge one loses digits, so mixing them would
+    // break the children-sum-to-parent identity there)
+    if (c.g_corners)
+        for (int64_t j = 0; j < nchild; ++j)
+            if (!(c.pts_zmax < child_prisms[6 * j + 4]))
+                return set_error(c, GRA_BAD_ARG,
+                                 "points must lie strictly above every child prism (corner-sum G)");
     const int64_t Nm2 = c.Nm + 7 * n_split;
     if (Nm2 > INT32_MAX) return set_error(c, GRA_BAD_ARG, "Nm exceeds int32 column indices");
     const int64_t ld2 = (Nm2 + 63) / 64 * 64, ldc = (nchild + 63) / 64 * 64;
@@ -825,9 +848,14 @@ gra_status gra_expand_cols(gra_ctx *ctx, const int64_t *split_cols, int64_t n_sp
     HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), c.stream), "clear status");
     {
         TimedRegion tr(c, kTimeAssemble);
-        HIPCHK(launch_assemble(dch.as<double>(), nchild, c.pts_dev.as<double>(), c.N_obs, c.flag,
-                               c.store, Gc.p, ldc, bad, c.stream),
-               "assemble child columns");
+        if (c.g_corners)
+            HIPCHK(launch_cell_corner(dch.as<double>(), nchild, c.pts_dev.as<double>(), c.N_obs,
+                                      c.flag, c.store, Gc.p, ldc, c.stream),
+                   "child columns by corner sums");
+        else
+            HIPCHK(launch_assemble(dch.as<double>(), nchild, c.pts_dev.as<double>(), c.N_obs,
+                                   c.flag, c.store, Gc.p, ldc, bad, c.stream),
+                   "assemble child columns");
     }
     HIPCHK(launch_remap_cols(c.G.p, c.ld, Gc.p, ldc, dsrc.as<int32_t>(), c.Nd, Nm2, G2.p, ld2,
                              (int)esz, c.stream),

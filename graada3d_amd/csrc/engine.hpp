@@ -109,6 +109,11 @@ struct Ctx {
     uint32_t flag = 0;
     gra_dtype store = GRA_F64;
     uint64_t g_version = 0;       // bumped whenever G changes
+    // G's values come from corner sums (gra_assemble_corners / _rectilinear /
+    // _batch_corners), not the face/edge form; pts_zmax: the deepest point
+    // (all points lie strictly above every prism of such a G)
+    bool g_corners = false;
+    double pts_zmax = 0.0;
 
     // mixed precision (gra_mixed_precision): a reduced-precision copy of the
     // FP64 G for the CG iteration's products, FP64 residual refreshes every

@@ -8,8 +8,12 @@ with the oracle's restatement of the reference's face/edge formulation to
 1e-12 of the component block's largest value, except where the reference's
 formulation is off (points over a cell's face plane far along it): there the
 entry must be within 1e-12 of the block scale of the prism integral
-evaluated at 60 digits and nearer to it than the oracle.  FP32 / BF16
-storage rounds the FP64 sums."""
+evaluated at 60 digits and nearer to it than the oracle, and -- independent
+of the corner formula and its limit convention -- within the range of the
+oracle's face/edge values at the point moved off the planes diagonally by
++-delta in x and y (delta = 1e-2 and 1e-3 m): the field is continuous above
+the body, so those values bracket its value at the point to O(delta^2).
+FP32 / BF16 storage rounds the FP64 sums."""
 import numpy as np
 import pytest
 
@@ -34,6 +38,18 @@ def _octree(seed, rounds=2):
     return mesh.prisms()
 
 
+def _within_moved_point_values(prism, p, bit, value, scale):
+    """value lies within the spread of the face/edge oracle's values at p
+    moved by (+-delta, +-delta) in x and y, off every face plane through p."""
+    for d in (1e-2, 1e-3):
+        q = np.array([[p[0] + sx * d, p[1] + sy * d, p[2]] for sx in (1, -1) for sy in (1, -1)])
+        v, st = oracle.assemble(prism[None], q, 1 << bit)
+        assert st == 0
+        v = v[:, 0]
+        mid = v.mean()
+        assert abs(value - mid) <= np.abs(v - mid).max() + 1e-11 * scale, (bit, d, value, v)
+
+
 def _check(prisms, pts, flag, ctx):
     f = Fwd(prisms, pts, flag, ctx=ctx)
     f.compute_G(method="corners")
@@ -54,6 +70,7 @@ def _check(prisms, pts, flag, ctx):
             ex = corner_exact(prisms[j], pts[ip], bit)
             assert abs(a[ip, j] - ex) <= EXACT_TOL * scale, (bit, ip, j, a[ip, j], ex)
             assert abs(a[ip, j] - ex) < abs(b[ip, j] - ex), (bit, ip, j)
+            _within_moved_point_values(prisms[j], pts[ip], bit, a[ip, j], scale)
         assert rel_norm(a, b) < 1e-7, bit
     return f, G
 

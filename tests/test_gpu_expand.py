@@ -1,6 +1,7 @@
 """expand_G on the GPU (AdaptiveInversion::expand_G, AdaptiveInversion.cpp:
-23-106): the expanded G equals a fresh assembly over the refined leaf list
-bit for bit (same kernels, columns moved as stored bits), and each parent's
+23-106): the expanded G equals a fresh assembly over the refined leaf list by
+the same formulation bit for bit (same kernels, columns moved as stored
+bits; the children of a corner-sum G are corner sums), and each parent's
 column equals the sum of its children's (superposition; the face/edge form's
 rounding, <= 1e-9 of the column's largest entry)."""
 import numpy as np
@@ -13,14 +14,15 @@ from graada3d_amd.octree import OctreeMesh
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("store,flag", [("f64", 0x12), ("f64", 0x3FE), ("f32", 0x2 | 0x80),
-                                        ("bf16", 0x10)])
-def test_expand_equals_fresh_assembly_and_superposition(ctx, store, flag):
+@pytest.mark.parametrize("store,flag,method", [("f64", 0x12, "faces"), ("f64", 0x3FE, "faces"),
+                                               ("f32", 0x2 | 0x80, "faces"), ("bf16", 0x10, "faces"),
+                                               ("f64", 0x3FE, "corners"), ("f32", 0x12, "corners")])
+def test_expand_equals_fresh_assembly_and_superposition(ctx, store, flag, method):
     rng = np.random.default_rng(9)
     o = OctreeMesh((0, 800), 8, (0, 800), 8, (0, 400), 4)
     pts = M.grid_points(9, -50.0, 850.0, 8, -50.0, 850.0, -7.5)
     f = Fwd(o.prisms(), pts, flag, ctx=ctx)
-    f.compute_G(store)
+    f.compute_G(store, method)
     for rnd in range(3):
         G_old = f.get_G()
         targets = rng.choice(o.n_cells, size=o.n_cells // 8, replace=False)
@@ -30,7 +32,7 @@ def test_expand_equals_fresh_assembly_and_superposition(ctx, store, flag):
         G_new = f.get_G()
         fresh = Fwd(o.prisms(), pts, flag, ctx=None)
         try:
-            fresh.compute_G(store)
+            fresh.compute_G(store, method)
             assert np.array_equal(G_new, fresh.get_G()), rnd
         finally:
             fresh.ctx.close()
@@ -47,6 +49,17 @@ def test_expand_equals_fresh_assembly_and_superposition(ctx, store, flag):
     # the grown G still works in the products
     x = rng.standard_normal(f.Nm)
     assert np.allclose(f.G_vec_mul(x), f.get_G() @ x, rtol=1e-12, atol=0)
+
+
+def test_expand_of_a_corner_sum_G_needs_points_above_the_children(ctx):
+    from graada3d_amd import GraError
+    o = OctreeMesh((0, 400), 4, (0, 400), 4, (0, 200), 2)
+    f = Fwd(o.prisms(), M.grid_points(4, 0.0, 400.0, 4, 0.0, 400.0, -5.0), 0x2, ctx=ctx)
+    f.compute_G("f64", "corners")
+    kids = np.tile(o.prisms()[:1], (8, 1))
+    kids[:, 4] = -10.0                      # a child reaching above the points
+    with pytest.raises(GraError):
+        f.expand_G([0], kids)
 
 
 def test_expand_rejects_bad_split_lists(ctx):
