@@ -637,20 +637,18 @@ static gra_status matmul_host(gra_ctx *ctx, int trans, const double *in, int64_t
     if (c.wav) return set_error(c, GRA_BAD_ARG, "not available for the wavelet-compressed G");
     if (!c.G.p || c.Nd == 0) return set_error(c, GRA_BAD_ARG, "no G assembled");
     const int64_t nin = trans ? c.Nd : c.Nm, nout = trans ? c.Nm : c.Nd;
-    const int64_t ldv = trans ? c.Nd : c.ld;
     const MatMulPlan p = plan_mat_mul(c.store, trans != 0, c.Nd, c.Nm, c.ld, k);
-    const size_t in_b = c.store == GRA_F64 ? 8 : 4;
     HIPCHK(put(c.mm_in, in, (size_t)(nin * k), c.stream), "upload right-hand sides");
-    HIPCHK(c.mm_pack.reserve(in_b * (size_t)p.ncol * (size_t)ldv), "alloc packed");
+    HIPCHK(c.mm_pack.reserve(mat_mul_pack_bytes(c.store, p)), "alloc packed");
     HIPCHK(c.mm_part.reserve(sizeof(double) * (size_t)p.splits * p.ncol * (size_t)p.ldp),
            "alloc partials");
     HIPCHK(c.mm_out.reserve(sizeof(double) * (size_t)(nout * k)), "alloc out");
-    HIPCHK(launch_pack_rhs(c.store, c.mm_in.as<double>(), nin, k, ldv, p.ncol, c.mm_pack.p,
+    HIPCHK(launch_pack_rhs(c.store, trans != 0, c.mm_in.as<double>(), nin, k, p, c.mm_pack.p,
                            c.stream),
            "pack");
     {
         TimedRegion tr(c, kTimeMatMul);
-        HIPCHK(launch_mat_mul(c.G.p, c.store, trans != 0, c.Nd, c.Nm, c.ld, c.mm_pack.p, ldv, p,
+        HIPCHK(launch_mat_mul(c.G.p, c.store, trans != 0, c.Nd, c.Nm, c.ld, c.mm_pack.p, p,
                               c.mm_part.as<double>(), c.stream),
                "matrix-core product");
     }

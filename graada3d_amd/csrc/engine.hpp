@@ -251,19 +251,23 @@ hipError_t launch_node_forward_batch(const FwdItem *h_items, const FwdItem *d_it
 // ncol: right-hand sides padded to tiles of 16; span: summation columns
 // (G x) or rows (G^T r) per split; ldp: leading dimension of the partials.
 struct MatMulPlan {
-    int ncol = 16;
-    int splits = 1;
-    int64_t span = 0;
-    int64_t ldp = 0;
+    int ncol = 16;          // right-hand sides rounded up to whole 16-wide tiles
+    int splits = 1;         // splits of the summation range
+    int64_t span = 0;       // its length per split
+    int64_t ldp = 0;        // partial-sum rows per right-hand side
+    int64_t pack_rows = 0;  // rows of the packed right-hand sides
 };
 MatMulPlan plan_mat_mul(gra_dtype dt, bool trans, int64_t Nd, int64_t Nm, int64_t ld, int64_t k);
-// Right-hand sides (host layout, n rows x k columns) into the MFMA input
-// type (FP64 for FP64 G, FP32 otherwise), ncol columns of ldv rows.
-hipError_t launch_pack_rhs(gra_dtype dt, const double *src, int64_t n, int64_t k, int64_t ldv,
-                           int ncol, void *dst, hipStream_t st);
+// Bytes of the packed right-hand sides of a plan.
+size_t mat_mul_pack_bytes(gra_dtype dt, const MatMulPlan &p);
+// Right-hand sides (host layout, n rows x k columns) in the layout and
+// element type the product kernels read: FP64 for FP64 G, FP32 for FP32 G,
+// three BF16 parts for BF16 G.
+hipError_t launch_pack_rhs(gra_dtype dt, bool trans, const double *src, int64_t n, int64_t k,
+                           const MatMulPlan &p, void *dst, hipStream_t st);
 hipError_t launch_mat_mul(const void *G, gra_dtype dt, bool trans, int64_t Nd, int64_t Nm,
-                          int64_t ld, const void *V, int64_t ldv, const MatMulPlan &p,
-                          double *part, hipStream_t st);
+                          int64_t ld, const void *V, const MatMulPlan &p, double *part,
+                          hipStream_t st);
 hipError_t launch_sum_splits(const double *part, const MatMulPlan &p, int64_t n, int64_t k,
                              double *out, hipStream_t st);
 

@@ -3,24 +3,43 @@
 // (Fwd.cpp:114-147, 80-113) for k vectors at once, so that one pass over G
 // serves k requests on the same resident G.
 //
-// MFMA flavours: FP64 G with v_mfma_f64_16x16x4 (FP64 in, FP64 accumulate);
-// FP32 and BF16 G with v_mfma_f32_16x16x4 (the BF16 elements widened to FP32
-// exactly, the vectors rounded to FP32), whose FP32 accumulators are added
-// into FP64 registers after every 16-32 products, so that FP32 rounding acts
-// on short sums only.  Lane layouts (checked on the device): operand A lane l
-// = row l % 16, k l / 16; operand B lane l = k l / 16, column l % 16; result
-// element i of lane l = row 4 (l / 16) + i (f32) or l / 16 + 4 i (f64),
-// column l % 16.
+// MFMA flavours per storage type:
+// * FP64 G: v_mfma_f64_16x16x4 (FP64 in, FP64 accumulate);
+// * FP32 G: v_mfma_f32_16x16x4 with the vectors rounded to FP32;
+// * BF16 G: the BF16 matrix cores with the vectors split exactly into three
+//   BF16 parts (x32 = hi + mid + lo, 8 significand bits each), so that the
+//   products are those of the FP32-rounded vectors: v_mfma_f32_16x16x32_bf16
+//   for G X (a lane's 16 bytes of a row are its 8 k values as they lie in
+//   memory), v_mfma_f32_16x16x16bf16_1k for G^T R (four G rows per lane,
+//   regrouped from row segments by byte permutes).
+// FP32 accumulators are added into FP64 registers after at most 32 products
+// of the hi part (plus the mid / lo parts' products, 2^-8 and 2^-16 smaller),
+// so that FP32 rounding acts on short sums only.
 //
-// Work split: G x -- a workgroup owns 16 rows of G and up to four 16-wide
-// tiles of right-hand sides over one split of the summation range, staging G
-// through LDS; G^T r -- a wave owns 16 E columns of G and one tile of
-// right-hand sides over one split of the rows, every lane loading 16 bytes
-// of one row (coalesced 256-byte row segments).  The splits' FP64 partial
-// tiles are summed in split order by a second kernel, so the products are
-// bit-for-bit repeatable.
+// Operand lane layouts (16x16 tiles; l = lane): A lane l = row l % 16,
+// k group l / 16; B lane l = k group l / 16, column l % 16; a lane holds 1
+// (x4), 4 (x16) or 8 (x32) consecutive k of its group.  Result element i of
+// lane l = row 4 (l / 16) + i (f32) or l / 16 + 4 i (f64), column l % 16.
+//
+// Work split:
+// * G X (gmx_kernel): a workgroup of four waves owns 64 rows (16 per wave)
+//   and one 16-wide tile of right-hand sides over one split of the columns.
+//   G goes straight from HBM into registers in the A-operand layout (no LDS
+//   round trip), one chunk ahead of use; the chunk's X values (shared by the
+//   64 rows) are staged in LDS, double-buffered, one barrier per chunk.
+// * G^T R (gtx_kernel, gtb_kernel): a wave owns a strip of G columns (16 B
+//   per lane and row, 256-byte row segments) and one tile of right-hand
+//   sides over one split of the rows; the next 16-row step's loads are issued
+//   before the current step's MFMAs.  Every lane runs the same number of
+//   steps (rows past the split are zero-filled): an MFMA reads the operands
+//   of all 64 lanes, so a lane that left a divergent loop would contribute
+//   stale registers.
+// The splits' FP64 partial tiles are summed in split order by a second
+// kernel, so the products are bit-for-bit repeatable.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "engine.hpp"
 
@@ -34,290 +53,460 @@ constexpr int kWavesPerBlk = kBlk / kWave;
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
-typedef double v2d __attribute__((ext_vector_type(2)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v4s __attribute__((ext_vector_type(4)));
 
-// 16 bytes of G as E elements widened to the MFMA input type.
-template <typename T> struct Gload;
-template <> struct Gload<double> {
-    static constexpr int E = 2;
+// Element q of a 16-byte vector as FP64 (two per vector) or FP32 (four).
+__device__ inline double el64(const v4u &v, int q)
+{
+    return q == 0 ? __hiloint2double((int)v.y, (int)v.x) : __hiloint2double((int)v.w, (int)v.z);
+}
+__device__ inline float el32(const v4u &v, int q)
+{
+    return __uint_as_float(q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w)));
+}
+
+// Per storage type: G elements per 16 bytes (E), the vectors' element type
+// and number of parts, G X chunk width (columns; 16 bytes per lane and piece,
+// 8 pieces per chunk).
+template <typename T> struct MM;
+template <> struct MM<double> {
     typedef double In;
-    __device__ static inline void run(const double *p, double *o)
-    {
-        v2d v = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p));
-        o[0] = v.x; o[1] = v.y;
-    }
-    __device__ static inline void from_lds(const double *p, double *o)
-    {
-        v2d v = *reinterpret_cast<const v2d *>(p);
-        o[0] = v.x; o[1] = v.y;
-    }
+    static constexpr int E = 2, NPART = 1, KC = 64;
 };
-template <> struct Gload<float> {
-    static constexpr int E = 4;
+template <> struct MM<float> {
     typedef float In;
-    __device__ static inline void run(const float *p, float *o)
-    {
-        v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
-        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-    }
-    __device__ static inline void from_lds(const float *p, float *o)
-    {
-        v4f v = *reinterpret_cast<const v4f *>(p);
-        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-    }
+    static constexpr int E = 4, NPART = 1, KC = 128;
 };
-template <> struct Gload<__hip_bfloat16> {
-    static constexpr int E = 8;
-    typedef float In;
-    __device__ static inline void widen(v4u v, float *o)
-    {
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            o[2 * q] = __uint_as_float(w[q] << 16);
-            o[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
-        }
-    }
-    __device__ static inline void run(const __hip_bfloat16 *p, float *o)
-    {
-        widen(__builtin_nontemporal_load(reinterpret_cast<const v4u *>(p)), o);
-    }
-    __device__ static inline void from_lds(const __hip_bfloat16 *p, float *o)
-    {
-        widen(*reinterpret_cast<const v4u *>(p), o);
-    }
+template <> struct MM<__hip_bfloat16> {
+    typedef __bf16 In;
+    static constexpr int E = 8, NPART = 3, KC = 256;
 };
 
-// One MFMA step and the result layout of the two flavours.
-__device__ inline void mfma(double a, double b, v4d &c)
-{
-    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-__device__ inline void mfma(float a, float b, v4f &c)
-{
-    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-template <typename In> __device__ inline int out_row(int lane, int i);
-template <> __device__ inline int out_row<double>(int lane, int i) { return lane / 16 + 4 * i; }
-template <> __device__ inline int out_row<float>(int lane, int i) { return 4 * (lane / 16) + i; }
-
-template <typename In> struct Acc;
-template <> struct Acc<double> { typedef v4d V; };
-template <> struct Acc<float> { typedef v4f V; };
-
-// Y tile through LDS: the workgroup's four waves share 16 rows of G; each
-// chunk of 256 columns is read with coalesced 16-byte loads (a wave covers
-// 1 KB of one row per instruction), held in registers while the previous
-// chunk's MFMAs run from LDS, and wave w multiplies columns 64w..64w+63 of
-// the chunk.  The four wave sums of each output element meet in LDS and
-// are added in wave order.  Rows are padded by 16 bytes in LDS so that the
-// 16 rows of an MFMA operand fall on different banks.  NCT tiles of 16
-// right-hand sides (tiles blockIdx.z * NCT ..) share each pass over G.
-template <typename T, int NCT>
-__global__ void __launch_bounds__(kBlk)
-gm_lds_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
-              const typename Gload<T>::In *__restrict__ X, int64_t ldx, int ncol,
-              double *__restrict__ part)
-{
-    typedef typename Gload<T>::In In;
-    constexpr int E = Gload<T>::E;              // elements per 16 bytes
-    constexpr int U = 16 / E;                   // 16-byte steps per wave quarter (64 columns)
-    constexpr int kChunk = 256;                 // columns per chunk
-    constexpr int CH = kChunk / E;              // 16-byte pieces per row and chunk
-    constexpr int PER = 16 * CH / kBlk;         // pieces per thread and chunk
-    constexpr int ROWB = kChunk * sizeof(T) + 16;   // padded LDS row, bytes
-    __shared__ __attribute__((aligned(16))) unsigned char tile[16 * ROWB];
-    __shared__ double red[kWavesPerBlk][16][16];
-    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int64_t rt = blockIdx.x;
-    const int split = blockIdx.y, ct0 = blockIdx.z * NCT;
-    const int r = lane & 15, kk = lane >> 4;
-    const int64_t k0 = (int64_t)split * kspl;
-    const int64_t k1 = k0 + kspl < ld ? k0 + kspl : ld;
-    const In *x[NCT];
-#pragma unroll
-    for (int t = 0; t < NCT; ++t) x[t] = X + (int64_t)((ct0 + t) * 16 + r) * ldx;
-    // this thread's pieces: q = threadIdx.x + kBlk * j -> row q / CH, piece q % CH
-    // (k1 - k0 is a multiple of 64, not of the chunk: the last chunk of a
-    // range may be partial; pieces past k1 are not loaded and wave quarters
-    // past k1 skip their MFMAs)
-    const T *src[PER];
-    int dst[PER], off[PER];
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        const int q = threadIdx.x + kBlk * j, row = q / CH, pc = q % CH;
-        const int64_t grow = rt * 16 + row < Nd ? rt * 16 + row : Nd - 1;   // clamp: dropped
-        src[j] = G + grow * ld + (int64_t)pc * E;
-        dst[j] = row * ROWB + pc * 16;
-        off[j] = pc * E;
-    }
-    v4u buf[PER];
-    typename Acc<In>::V acc[NCT];
-    double a64[NCT][4];
-#pragma unroll
-    for (int t = 0; t < NCT; ++t)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            acc[t][i] = 0;
-            a64[t][i] = 0.0;
-        }
-    int64_t kc = k0;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        buf[j] = v4u{0u, 0u, 0u, 0u};
-        if (kc + off[j] < k1)
-            buf[j] = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(src[j] + kc));
-    }
-    for (; kc < k1; kc += kChunk) {
-        __syncthreads();                        // previous chunk's LDS reads done
-#pragma unroll
-        for (int j = 0; j < PER; ++j) *reinterpret_cast<v4u *>(tile + dst[j]) = buf[j];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            if (kc + kChunk + off[j] < k1)
-                buf[j] = __builtin_nontemporal_load(
-                    reinterpret_cast<const v4u *>(src[j] + kc + kChunk));
-        }
-        const int cbase = 64 * wv;              // this wave's 64 columns of the chunk
-        if (kc + cbase >= k1) continue;         // uniform per wave
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int col = cbase + u * 4 * E + E * kk;
-            In gv[E];
-            Gload<T>::from_lds(reinterpret_cast<const T *>(tile + r * ROWB) + col, gv);
-#pragma unroll
-            for (int t = 0; t < NCT; ++t) {
-                In xv[E];
-#pragma unroll
-                for (int e = 0; e < E; ++e) xv[e] = x[t][kc + col + e];
-#pragma unroll
-                for (int e = 0; e < E; ++e) mfma(gv[e], xv[e], acc[t]);
-                if (sizeof(In) == 4) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        a64[t][i] += (double)acc[t][i];
-                        acc[t][i] = 0;
-                    }
-                }
-            }
-        }
-    }
-    if (sizeof(In) == 8) {
-#pragma unroll
-        for (int t = 0; t < NCT; ++t)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a64[t][i] = (double)acc[t][i];
-    }
-    const int orow = threadIdx.x / 16, ocol = threadIdx.x % 16;   // 256 threads = 16 x 16
-    const int64_t rr = rt * 16 + orow;
-#pragma unroll
-    for (int t = 0; t < NCT; ++t) {
-        __syncthreads();                        // red free again
-#pragma unroll
-        for (int i = 0; i < 4; ++i) red[wv][out_row<In>(lane, i)][lane & 15] = a64[t][i];
-        __syncthreads();
-        if (rr < Nd) {
-            double s = red[0][orow][ocol];
-#pragma unroll
-            for (int q = 1; q < kWavesPerBlk; ++q) s += red[q][orow][ocol];
-            part[((int64_t)split * ncol + (ct0 + t) * 16 + ocol) * Nd + rr] = s;
-        }
-    }
-}
-
-// Z tile: G columns m0 = mt*16E .. +16E (lane group r takes columns m0+E r..
-// +E-1, one MFMA per element phase e), right-hand sides ct*16.., summation
-// over G rows [split*rspl, min(+rspl, Nd)).  R: the right-hand sides in MFMA
-// input type, column c at R + c*ldr (zero padded to whole tiles of 16).
-// part[(split * ncol + c) * ld + column].
+// Y = G X.  Lane (r, kk) of wave w owns row 64 b + 16 w + r; piece j of a
+// chunk is its 16 bytes at chunk column cofs(j, kk) (two neighbouring pieces
+// of a lane are adjacent, so that each pair of load instructions covers 128
+// contiguous bytes of every row).  The B operand of the same k values is
+// read from the LDS copy of X: rows = right-hand sides, columns = the chunk.
+// Xp: NPART parts of ncol rows of ldx elements (part p, right-hand side c at
+// Xp + (p * ncol + c) * ldx).  part[(split * ncol + c) * Nd + row].
 template <typename T>
 __global__ void __launch_bounds__(kBlk)
-gt_kernel(const T *__restrict__ G, int64_t Nd, int64_t Nm, int64_t ld, int64_t rspl,
-          const typename Gload<T>::In *__restrict__ R, int64_t ldr, int64_t col_tiles, int ncol,
-          double *__restrict__ part)
+gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
+           const typename MM<T>::In *__restrict__ Xp, int64_t ldx, int ncol,
+           double *__restrict__ part)
 {
-    typedef typename Gload<T>::In In;
-    constexpr int E = Gload<T>::E;
-    constexpr int U = 4;                      // row steps per iteration (4 rows each)
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t mt = (int64_t)blockIdx.x * kWavesPerBlk + (threadIdx.x / kWave);
-    if (mt >= col_tiles) return;
-    const int split = blockIdx.y, ct = blockIdx.z;
+    typedef typename MM<T>::In In;
+    constexpr int E = MM<T>::E, KC = MM<T>::KC, NPART = MM<T>::NPART;
+    constexpr int PIECES = KC / (4 * E);                 // 8
+    constexpr int XROW = KC * (int)sizeof(In) + 16;      // LDS bytes per right-hand side (padded)
+    constexpr int XT = 16 * XROW;                        // per part
+    constexpr int XPC = KC * (int)sizeof(In) / 16;       // 16-byte X pieces per right-hand side
+    constexpr int XPER = 16 * XPC / kBlk;                // per thread and part
+    constexpr int XV = 16 / (int)sizeof(In);             // X elements per 16 bytes
+    constexpr bool F64 = sizeof(In) == 8;
+    constexpr bool BF = NPART == 3;
+    __shared__ __attribute__((aligned(16))) unsigned char xl[2][NPART][XT];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int r = lane & 15, kk = lane >> 4;
-    const int64_t m0 = mt * 16 * E;
-    const int64_t mcol = m0 + (int64_t)E * r;         // this lane's first G column
-    const bool colok = mcol < ld;                      // ld is a multiple of 64 >= E
-    const In *rv = R + (int64_t)(ct * 16 + r) * ldr;
-    const int64_t r0 = (int64_t)split * rspl;
-    const int64_t r1 = r0 + rspl < Nd ? r0 + rspl : Nd;
-    typename Acc<In>::V acc[E];
-    double a64[E][4];
+    const int64_t row = (int64_t)blockIdx.x * 64 + wv * 16 + r;
+    const int64_t growc = row < Nd ? row : Nd - 1;       // clamped rows are not stored
+    const int split = blockIdx.y, ct = blockIdx.z;
+    const int64_t k0 = (int64_t)split * kspl;
+    const int64_t k1 = k0 + kspl < ld ? k0 + kspl : ld;
+    const T *grow = G + growc * ld;
+
+    auto cofs = [&](int j) { return (j >> 1) * 8 * E + kk * 2 * E + (j & 1) * E; };
+    v4u g[2][PIECES];
+    v4u xs[NPART][XPER];
+    // k1 and every piece start are multiples of E, so a piece below k1 lies
+    // wholly below it (likewise for X pieces)
+    auto load_g = [&](int64_t kc, v4u (&dst)[PIECES]) {
 #pragma unroll
-    for (int e = 0; e < E; ++e)
+        for (int j = 0; j < PIECES; ++j) {
+            const int64_t c = kc + cofs(j);
+            dst[j] = c < k1 ? *reinterpret_cast<const v4u *>(grow + c) : v4u{0u, 0u, 0u, 0u};
+        }
+    };
+    auto load_x = [&](int64_t kc) {
+#pragma unroll
+        for (int p = 0; p < NPART; ++p)
+#pragma unroll
+            for (int u = 0; u < XPER; ++u) {
+                const int q = threadIdx.x + kBlk * u, n = q / XPC, pc = q % XPC;
+                const int64_t c = kc + (int64_t)pc * XV;
+                const In *src = Xp + ((int64_t)p * ncol + ct * 16 + n) * ldx + c;
+                xs[p][u] = c < k1 ? *reinterpret_cast<const v4u *>(src) : v4u{0u, 0u, 0u, 0u};
+            }
+    };
+    auto store_x = [&](int b) {
+#pragma unroll
+        for (int p = 0; p < NPART; ++p)
+#pragma unroll
+            for (int u = 0; u < XPER; ++u) {
+                const int q = threadIdx.x + kBlk * u, n = q / XPC, pc = q % XPC;
+                *reinterpret_cast<v4u *>(&xl[b][p][n * XROW + pc * 16]) = xs[p][u];
+            }
+    };
+
+    v4d acc64 = {0.0, 0.0, 0.0, 0.0};
+    v4f acc32 = {0.0f, 0.0f, 0.0f, 0.0f}, accml = {0.0f, 0.0f, 0.0f, 0.0f};
+    double a64[4] = {0.0, 0.0, 0.0, 0.0};
+    auto flush = [&](v4f &acc) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            acc[e][i] = 0;
-            a64[e][i] = 0.0;
+            a64[i] += (double)acc[i];
+            acc[i] = 0.0f;
         }
-    for (int64_t rb = r0 + kk; rb < r1; rb += 4 * U) {
-        In gv[U][E], b[U];
+    };
+    auto compute = [&](int b, const v4u (&gv)[PIECES]) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t row = rb + 4 * u;
-            // the lane's A rows (G columns mcol..) and its B column (right-hand
-            // side r) are independent: a lane past the last G column still
-            // supplies its right-hand side value
-            b[u] = row < r1 ? rv[row] : 0;
-            if (colok && row < r1) {
-                Gload<T>::run(G + row * ld + mcol, gv[u]);
+        for (int j = 0; j < PIECES; ++j) {
+            const int off = r * XROW + cofs(j) * (int)sizeof(In);
+            if constexpr (BF) {
+                const v8bf a = __builtin_bit_cast(v8bf, gv[j]);
+                const v8bf xh = *reinterpret_cast<const v8bf *>(&xl[b][0][off]);
+                const v8bf xm = *reinterpret_cast<const v8bf *>(&xl[b][1][off]);
+                const v8bf xo = *reinterpret_cast<const v8bf *>(&xl[b][2][off]);
+                acc32 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xh, acc32, 0, 0, 0);
+                accml = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xm, accml, 0, 0, 0);
+                accml = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xo, accml, 0, 0, 0);
+                flush(acc32);                          // 32 products of the hi part
             } else {
+                const v4u xv = *reinterpret_cast<const v4u *>(&xl[b][0][off]);
+                if constexpr (F64) {
 #pragma unroll
-                for (int e = 0; e < E; ++e) gv[u][e] = 0;
+                    for (int e = 0; e < E; ++e)
+                        acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(el64(gv[j], e), el64(xv, e),
+                                                                      acc64, 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < E; ++e)
+                        acc32 = __builtin_amdgcn_mfma_f32_16x16x4f32(el32(gv[j], e), el32(xv, e),
+                                                                      acc32, 0, 0, 0);
+                    if (j & 1) flush(acc32);           // 32 products
+                }
             }
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int e = 0; e < E; ++e) mfma(gv[u][e], b[u], acc[e]);
-        if (sizeof(In) == 4) {                // FP32 partial -> FP64 every 16 products
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    a64[e][i] += (double)acc[e][i];
-                    acc[e][i] = 0;
-                }
+        if constexpr (BF) flush(accml);                // 512 products of the mid / lo parts
+    };
+
+    // chunk kc: G in g[slot], X in xl[b]; the next chunk's loads are issued
+    // (X first, so that storing it waits only for its own loads) before this
+    // chunk's MFMAs
+    int64_t kc = k0;
+    load_x(kc);
+    load_g(kc, g[0]);
+    store_x(0);
+    __syncthreads();
+    int b = 0;
+    for (; kc < k1; kc += 2 * KC) {
+        const bool more1 = kc + KC < k1;
+        if (more1) {
+            load_x(kc + KC);
+            load_g(kc + KC, g[1]);
         }
+        compute(b, g[0]);
+        if (!more1) break;
+        store_x(b ^ 1);                                // xl[b ^ 1] was last read before the previous barrier
+        __syncthreads();
+        b ^= 1;
+        const bool more2 = kc + 2 * KC < k1;
+        if (more2) {
+            load_x(kc + 2 * KC);
+            load_g(kc + 2 * KC, g[0]);
+        }
+        compute(b, g[1]);
+        if (!more2) break;
+        store_x(b ^ 1);
+        __syncthreads();
+        b ^= 1;
     }
-    if (sizeof(In) == 8) {
+    if constexpr (F64) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a64[i] = acc64[i];
+    } else {
+        flush(acc32);
+    }
+    const int64_t rbase = (int64_t)blockIdx.x * 64 + wv * 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int orow = F64 ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
+        const int64_t rr = rbase + orow;
+        if (rr < Nd) part[((int64_t)split * ncol + ct * 16 + r) * Nd + rr] = a64[i];
+    }
+}
+
+// Z = G^T R for FP64 / FP32 G.  Wave wt owns columns m0 = wt * MC * 16 E ..;
+// for column tile mc, lane (r, kk) loads 16 B (E columns) of rows kk, kk + 4,
+// kk + 8, kk + 12 of a 16-row step and runs one MFMA per element (A = those
+// G values, B = R at the lane's row).  Rp: right-hand-side tile t, row i,
+// column n at Rp[(t * ldr + i) * 16 + n].  part[(split * ncol + c) * ld + m].
+template <typename T, int MC>
+__global__ void __launch_bounds__(kBlk)
+gtx_kernel(const T *__restrict__ G, int64_t Nd, int64_t Nm, int64_t ld, int64_t rspl,
+           const typename MM<T>::In *__restrict__ Rp, int64_t ldr, int64_t wave_tiles, int ncol,
+           double *__restrict__ part)
+{
+    typedef typename MM<T>::In In;
+    typedef typename std::conditional<sizeof(In) == 8, v4d, v4f>::type Acc;
+    constexpr int E = MM<T>::E;
+    constexpr int U = 4;                                 // 4-row groups per step
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wt = (int64_t)blockIdx.x * kWavesPerBlk + threadIdx.x / kWave;
+    if (wt >= wave_tiles) return;                       // whole wave
+    const int split = blockIdx.y, ct = blockIdx.z;
+    const int r = lane & 15, kk = lane >> 4;
+    const int64_t m0 = wt * MC * 16 * E;
+    const In *rv = Rp + (int64_t)ct * ldr * 16 + r;
+    const int64_t r0 = (int64_t)split * rspl;
+    const int64_t r1 = r0 + rspl < Nd ? r0 + rspl : Nd;
+    Acc acc[MC][E];
+    double a64[MC][E][4];
+#pragma unroll
+    for (int mc = 0; mc < MC; ++mc)
 #pragma unroll
         for (int e = 0; e < E; ++e)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a64[e][i] = (double)acc[e][i];
+            for (int i = 0; i < 4; ++i) {
+                acc[mc][e][i] = 0;
+                a64[mc][e][i] = 0.0;
+            }
+    In gv[2][U][MC][E], bv[2][U];
+    auto load = [&](int64_t rb, int sl) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t row = rb + kk + 4 * u;
+            bv[sl][u] = row < r1 ? rv[row * 16] : (In)0;
+#pragma unroll
+            for (int mc = 0; mc < MC; ++mc) {
+                const int64_t col = m0 + mc * 16 * E + (int64_t)E * r;
+                if (col < ld && row < r1) {
+                    const v4u v = *reinterpret_cast<const v4u *>(G + row * ld + col);
+#pragma unroll
+                    for (int e = 0; e < E; ++e)
+                        gv[sl][u][mc][e] = sizeof(In) == 8 ? (In)el64(v, e) : (In)el32(v, e);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < E; ++e) gv[sl][u][mc][e] = (In)0;
+                }
+            }
+        }
+    };
+    auto compute = [&](int sl) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int mc = 0; mc < MC; ++mc)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if constexpr (sizeof(In) == 8)
+                        acc[mc][e] = __builtin_amdgcn_mfma_f64_16x16x4f64(gv[sl][u][mc][e], bv[sl][u],
+                                                                          acc[mc][e], 0, 0, 0);
+                    else
+                        acc[mc][e] = __builtin_amdgcn_mfma_f32_16x16x4f32(gv[sl][u][mc][e], bv[sl][u],
+                                                                          acc[mc][e], 0, 0, 0);
+                }
+        if constexpr (sizeof(In) == 4) {                 // FP32 partial -> FP64 every 16 products
+#pragma unroll
+            for (int mc = 0; mc < MC; ++mc)
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        a64[mc][e][i] += (double)acc[mc][e][i];
+                        acc[mc][e][i] = 0;
+                    }
+        }
+    };
+    load(r0, 0);
+    for (int64_t rb = r0; rb < r1; rb += 8 * U) {       // the same trip count in every lane
+        load(rb + 4 * U, 1);
+        compute(0);
+        if (rb + 4 * U >= r1) break;
+        load(rb + 8 * U, 0);
+        compute(1);
     }
-    const int c = ct * 16 + (lane & 15);
+    if constexpr (sizeof(In) == 8) {
+#pragma unroll
+        for (int mc = 0; mc < MC; ++mc)
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a64[mc][e][i] = acc[mc][e][i];
+    }
+    const int c = ct * 16 + r;
     double *out = part + ((int64_t)split * ncol + c) * ld;
 #pragma unroll
-    for (int e = 0; e < E; ++e)
+    for (int mc = 0; mc < MC; ++mc)
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int orow = sizeof(In) == 8 ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
+                const int64_t m = m0 + mc * 16 * E + (int64_t)E * orow + e;
+                if (m < Nm) out[m] = a64[mc][e][i];
+            }
+}
+
+// Z = G^T R for BF16 G.  Wave wt owns the 128 columns m0 = 128 wt ..; lane
+// (j, kq) loads 16 B (columns m0 + 8j .. + 7) of rows 4kq .. 4kq + 3 of a
+// 16-row step, regroups them into eight 4-row column operands (the low /
+// high halves of each 32-bit word) and multiplies each with the three parts
+// of R.  MFMA row m' of column operand c is G column m0 + 8 m' + c.
+// Rb: part p, tile t, rows 4q .. 4q + 3 of right-hand side n at
+// Rb[p * pstride + (t * ldr4 + q) * 64 + 4 n]; ldr4 * 4 >= Nd rounded up to 16.
+__global__ void __launch_bounds__(kBlk)
+gtb_kernel(const __hip_bfloat16 *__restrict__ G, int64_t Nd, int64_t Nm, int64_t ld, int64_t rspl,
+           const unsigned short *__restrict__ Rb, int64_t ldr4, int64_t pstride,
+           int64_t wave_tiles, int ncol, double *__restrict__ part)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wt = (int64_t)blockIdx.x * kWavesPerBlk + threadIdx.x / kWave;
+    if (wt >= wave_tiles) return;
+    const int split = blockIdx.y, ct = blockIdx.z;
+    const int j = lane & 15, kq = lane >> 4;
+    const int64_t m0 = wt * 128;
+    const int64_t r0 = (int64_t)split * rspl;           // rspl is a multiple of 16
+    const int64_t r1 = r0 + rspl < Nd ? r0 + rspl : Nd;
+    const unsigned short *rb0 = Rb + (int64_t)ct * ldr4 * 64 + j * 4;
+    v4f acc[8];
+    double a64[8][4];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int64_t m = m0 + (int64_t)E * out_row<In>(lane, i) + e;
-            if (m < Nm) out[m] = a64[e][i];
+            acc[c][i] = 0.0f;
+            a64[c][i] = 0.0;
+        }
+    v4u g[2][4];
+    v4s bp[2][3];
+    const int64_t col = m0 + 8 * j;
+    auto load = [&](int64_t rb, int sl) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row = rb + 4 * kq + i;
+            g[sl][i] = (row < r1 && col < ld) ? *reinterpret_cast<const v4u *>(G + row * ld + col)
+                                              : v4u{0u, 0u, 0u, 0u};
+        }
+        // a step past the split (the prefetch of the loop's last turn) reads
+        // the last group of rows; its G values are zero
+        int64_t q4 = rb / 4 + kq;
+        q4 = q4 < ldr4 - 1 ? q4 : ldr4 - 1;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+            bp[sl][p] = *reinterpret_cast<const v4s *>(rb0 + p * pstride + q4 * 64);
+    };
+    int step = 0;
+    auto compute = [&](int sl) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t w0 = g[sl][0][q], w1 = g[sl][1][q], w2 = g[sl][2][q], w3 = g[sl][3][q];
+            const uint32_t lo01 = (w0 & 0xffffu) | (w1 << 16), lo23 = (w2 & 0xffffu) | (w3 << 16);
+            const uint32_t hi01 = (w0 >> 16) | (w1 & 0xffff0000u), hi23 = (w2 >> 16) | (w3 & 0xffff0000u);
+            const v4s ae = __builtin_bit_cast(v4s, (unsigned long long)lo01 | ((unsigned long long)lo23 << 32));
+            const v4s ao = __builtin_bit_cast(v4s, (unsigned long long)hi01 | ((unsigned long long)hi23 << 32));
+            // lo and mid parts first, hi last: the small terms are added
+            // before the accumulator grows
+#pragma unroll
+            for (int p = 2; p >= 0; --p) {
+                acc[2 * q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ae, bp[sl][p], acc[2 * q], 0, 0, 0);
+                acc[2 * q + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ao, bp[sl][p], acc[2 * q + 1], 0, 0, 0);
+            }
+        }
+        if (step & 1) {                                  // 32 products of the hi part
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a64[c][i] += (double)acc[c][i];
+                    acc[c][i] = 0.0f;
+                }
+        }
+        ++step;
+    };
+    load(r0, 0);
+    for (int64_t rb = r0; rb < r1; rb += 32) {          // the same trip count in every lane
+        load(rb + 16, 1);
+        compute(0);
+        if (rb + 16 >= r1) break;
+        load(rb + 32, 0);
+        compute(1);
+    }
+    const int n = ct * 16 + j;
+    double *out = part + ((int64_t)split * ncol + n) * ld;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t m = m0 + 8 * (4 * kq + i) + c;
+            if (m < Nm) out[m] = a64[c][i] + (double)acc[c][i];
         }
 }
 
-// The right-hand sides (host layout: column c at src + c*n, n rows, k
-// columns) in MFMA input type, padded with zeros to ldv rows and ncol columns.
-template <typename In>
-__global__ void pack_rhs_kernel(const double *__restrict__ src, int64_t n, int64_t k,
-                                int64_t ldv, int ncol, In *__restrict__ dst)
+// x (FP64) as FP32, and as three BF16 parts whose sum is that FP32 value
+// exactly (8 significand bits each; each remainder is exact in FP32).
+__device__ inline void split3(double v, __bf16 &h, __bf16 &m, __bf16 &o)
+{
+    const float f = (float)v;
+    h = (__bf16)f;
+    const float r1 = f - (float)h;
+    m = (__bf16)r1;
+    o = (__bf16)(r1 - (float)m);
+}
+
+// G X right-hand sides (host layout: column c at src + c*n, n rows, k
+// columns) padded with zeros to ncol columns of ldv: FP64 / FP32 rows
+// (NPART 1) or three BF16 parts of ncol * ldv each.
+template <typename In, int NPART>
+__global__ void pack_x_kernel(const double *__restrict__ src, int64_t n, int64_t k, int64_t ldv,
+                              int ncol, In *__restrict__ dst)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)ncol * ldv) return;
+    const int64_t per = (int64_t)ncol * ldv;
+    if (i >= per) return;
     const int64_t c = i / ldv, row = i - c * ldv;
+    const double v = (c < k && row < n) ? src[c * n + row] : 0.0;
+    if constexpr (NPART == 1) {
+        dst[i] = (In)v;
+    } else {
+        __bf16 h, m, o;
+        split3(v, h, m, o);
+        dst[i] = h;
+        dst[per + i] = m;
+        dst[2 * per + i] = o;
+    }
+}
+
+// G^T R right-hand sides for FP64 / FP32 G, row-major per 16-wide tile:
+// dst[(t * ldr + i) * 16 + n] = column 16 t + n, row i.
+template <typename In>
+__global__ void pack_r_kernel(const double *__restrict__ src, int64_t n, int64_t k, int64_t ldr,
+                              int ncol, In *__restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)ncol * ldr) return;
+    const int64_t t = i / (ldr * 16), rem = i - t * ldr * 16, row = rem / 16, nn = rem % 16;
+    const int64_t c = t * 16 + nn;
     dst[i] = (c < k && row < n) ? (In)src[c * n + row] : (In)0;
+}
+
+// G^T R right-hand sides for BF16 G: three BF16 parts, each [t][row / 4][n][row % 4].
+__global__ void pack_rb_kernel(const double *__restrict__ src, int64_t n, int64_t k, int64_t ldr4,
+                               int ncol, unsigned short *__restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)ncol * ldr4 * 4;
+    if (i >= per) return;
+    const int64_t t = i / (ldr4 * 64), rem = i - t * ldr4 * 64;
+    const int64_t q4 = rem / 64, w = rem % 64, nn = w / 4, row = q4 * 4 + w % 4;
+    const int64_t c = t * 16 + nn;
+    __bf16 h, m, o;
+    split3((c < k && row < n) ? src[c * n + row] : 0.0, h, m, o);
+    dst[i] = __builtin_bit_cast(unsigned short, h);
+    dst[per + i] = __builtin_bit_cast(unsigned short, m);
+    dst[2 * per + i] = __builtin_bit_cast(unsigned short, o);
 }
 
 // out[c*n + i] = sum over splits, in split order, of part[(s*ncol + c)*ldp + i].
@@ -334,100 +523,115 @@ __global__ void sum_splits_kernel(const double *__restrict__ part, int splits, i
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-template <typename T>
-hipError_t mm_n(const void *G, int64_t Nd, int64_t ld, const void *X, int64_t ldx, int ncol,
-                int splits, int64_t kspl, double *part, hipStream_t st)
-{
-    const int64_t row_tiles = (Nd + 15) / 16;
-    const int ctiles = ncol / 16;
-    // right-hand-side tiles per pass over G: 4 when they divide evenly, else 2 or 1
-    const int nct = ctiles % 4 == 0 ? 4 : (ctiles % 2 == 0 ? 2 : 1);
-    dim3 grid((unsigned)row_tiles, (unsigned)splits, (unsigned)(ctiles / nct));
-    typedef typename Gload<T>::In In;
-#define GRA_GM(N)                                                                              \
-    hipLaunchKernelGGL((gm_lds_kernel<T, N>), grid, dim3(kBlk), 0, st, (const T *)G, Nd, ld,   \
-                       kspl, (const In *)X, ldx, ncol, part)
-    if (nct == 4) GRA_GM(4);
-    else if (nct == 2) GRA_GM(2);
-    else GRA_GM(1);
-#undef GRA_GM
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t mm_t(const void *G, int64_t Nd, int64_t Nm, int64_t ld, const void *R, int64_t ldr,
-                int ncol, int splits, int64_t rspl, double *part, hipStream_t st)
-{
-    const int64_t col_tiles = (ld + 16 * Gload<T>::E - 1) / (16 * Gload<T>::E);
-    dim3 grid(nblk(col_tiles, kWavesPerBlk), (unsigned)splits, (unsigned)(ncol / 16));
-    hipLaunchKernelGGL(gt_kernel<T>, grid, dim3(kBlk), 0, st, (const T *)G, Nd, Nm, ld, rspl,
-                       (const typename Gload<T>::In *)R, ldr, col_tiles, ncol, part);
-    return hipGetLastError();
-}
+int elems_per_16b(gra_dtype dt) { return dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8); }
+int gx_chunk(gra_dtype dt) { return dt == GRA_F64 ? 64 : (dt == GRA_F32 ? 128 : 256); }
+// columns per G^T R wave: two 16-lane tiles of 16-byte segments (FP64 /
+// FP32), 128 for BF16
+int64_t gt_wave_cols(gra_dtype dt) { return dt == GRA_BF16 ? 128 : 2 * 16 * elems_per_16b(dt); }
 
 }  // namespace
 
-int gemm_elems_per_load(gra_dtype dt) { return dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8); }
-
-// Splits of the summation range: about 3,000 waves in all.
+// G X: splits of the columns so that about 1,536 workgroups of 64 rows run,
+// at most 6 (chunk-aligned spans); G^T R: splits of the rows (multiples of 16) so that
+// about 2,048 waves run.  Both from sweeps on MI355X (16 right-hand sides,
+// 5,000 x 32,000 and 16,384 x 131,072 G).
 MatMulPlan plan_mat_mul(gra_dtype dt, bool trans, int64_t Nd, int64_t Nm, int64_t ld, int64_t k)
 {
     MatMulPlan p;
+    (void)Nm;
     p.ncol = (int)((k + 15) / 16 * 16);
-    const int64_t ctiles = p.ncol / 16;
     if (!trans) {
-        const int64_t tiles = (Nd + 15) / 16 * ctiles;
-        int64_t s = (3000 + tiles - 1) / tiles;
-        const int64_t smax = ld / 256 > 0 ? ld / 256 : 1;     // at least 256 columns each
+        const int64_t wgs = (Nd + 63) / 64, kc = gx_chunk(dt);
+        int64_t s = (1536 + wgs / 2) / wgs;
+        s = s < 6 ? s : 6;                               // more were slower on the 5,000-row G
+        const int64_t smax = (ld + kc - 1) / kc;          // at least one chunk each
         s = s < 1 ? 1 : (s > smax ? smax : s);
-        int64_t kspl = (ld + s - 1) / s;
-        p.span = (kspl + 63) / 64 * 64;
+        int64_t span = (ld + s - 1) / s;
+        p.span = (span + kc - 1) / kc * kc;
         p.splits = (int)((ld + p.span - 1) / p.span);
         p.ldp = Nd;
+        p.pack_rows = ld;
     } else {
-        const int E = gemm_elems_per_load(dt);
-        const int64_t tiles = (ld + 16 * E - 1) / (16 * E) * ctiles;
-        int64_t s = (3000 + tiles - 1) / tiles;
-        const int64_t smax = Nd / 64 > 0 ? Nd / 64 : 1;       // at least 64 rows each
+        const int64_t waves = (ld + gt_wave_cols(dt) - 1) / gt_wave_cols(dt);
+        int64_t s = (2048 + waves / 2) / waves;
+        const int64_t smax = (Nd + 15) / 16;             // at least 16 rows each
         s = s < 1 ? 1 : (s > smax ? smax : s);
-        p.span = (Nd + s - 1) / s;
+        const int64_t span = (Nd + s - 1) / s;
+        p.span = (span + 15) / 16 * 16;
         p.splits = (int)((Nd + p.span - 1) / p.span);
         p.ldp = ld;
+        p.pack_rows = (Nd + 15) / 16 * 16;
     }
     return p;
 }
 
-hipError_t launch_pack_rhs(gra_dtype dt, const double *src, int64_t n, int64_t k, int64_t ldv,
-                           int ncol, void *dst, hipStream_t st)
+size_t mat_mul_pack_bytes(gra_dtype dt, const MatMulPlan &p)
 {
-    const int64_t tot = (int64_t)ncol * ldv;
-    if (dt == GRA_F64)
-        hipLaunchKernelGGL(pack_rhs_kernel<double>, dim3(nblk(tot, kBlk)), dim3(kBlk), 0, st, src,
-                           n, k, ldv, ncol, (double *)dst);
-    else
-        hipLaunchKernelGGL(pack_rhs_kernel<float>, dim3(nblk(tot, kBlk)), dim3(kBlk), 0, st, src,
-                           n, k, ldv, ncol, (float *)dst);
+    const size_t per = (size_t)p.ncol * (size_t)p.pack_rows;
+    return dt == GRA_F64 ? 8 * per : (dt == GRA_F32 ? 4 * per : 6 * per);
+}
+
+hipError_t launch_pack_rhs(gra_dtype dt, bool trans, const double *src, int64_t n, int64_t k,
+                           const MatMulPlan &p, void *dst, hipStream_t st)
+{
+    const int64_t tot = (int64_t)p.ncol * p.pack_rows;
+    const dim3 g(nblk(tot, kBlk)), b(kBlk);
+    if (!trans) {
+        if (dt == GRA_F64)
+            hipLaunchKernelGGL((pack_x_kernel<double, 1>), g, b, 0, st, src, n, k, p.pack_rows,
+                               p.ncol, (double *)dst);
+        else if (dt == GRA_F32)
+            hipLaunchKernelGGL((pack_x_kernel<float, 1>), g, b, 0, st, src, n, k, p.pack_rows,
+                               p.ncol, (float *)dst);
+        else
+            hipLaunchKernelGGL((pack_x_kernel<__bf16, 3>), g, b, 0, st, src, n, k, p.pack_rows,
+                               p.ncol, (__bf16 *)dst);
+    } else {
+        if (dt == GRA_F64)
+            hipLaunchKernelGGL(pack_r_kernel<double>, g, b, 0, st, src, n, k, p.pack_rows, p.ncol,
+                               (double *)dst);
+        else if (dt == GRA_F32)
+            hipLaunchKernelGGL(pack_r_kernel<float>, g, b, 0, st, src, n, k, p.pack_rows, p.ncol,
+                               (float *)dst);
+        else
+            hipLaunchKernelGGL(pack_rb_kernel, g, b, 0, st, src, n, k, p.pack_rows / 4, p.ncol,
+                               (unsigned short *)dst);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_mat_mul(const void *G, gra_dtype dt, bool trans, int64_t Nd, int64_t Nm,
-                          int64_t ld, const void *V, int64_t ldv, const MatMulPlan &p,
-                          double *part, hipStream_t st)
+                          int64_t ld, const void *V, const MatMulPlan &p, double *part,
+                          hipStream_t st)
 {
+    const int ctiles = p.ncol / 16;
     if (!trans) {
-        switch (dt) {
-        case GRA_F64: return mm_n<double>(G, Nd, ld, V, ldv, p.ncol, p.splits, p.span, part, st);
-        case GRA_F32: return mm_n<float>(G, Nd, ld, V, ldv, p.ncol, p.splits, p.span, part, st);
-        default:
-            return mm_n<__hip_bfloat16>(G, Nd, ld, V, ldv, p.ncol, p.splits, p.span, part, st);
-        }
+        const dim3 grid((unsigned)((Nd + 63) / 64), (unsigned)p.splits, (unsigned)ctiles);
+        if (dt == GRA_F64)
+            hipLaunchKernelGGL(gmx_kernel<double>, grid, dim3(kBlk), 0, st, (const double *)G, Nd,
+                               ld, p.span, (const double *)V, p.pack_rows, p.ncol, part);
+        else if (dt == GRA_F32)
+            hipLaunchKernelGGL(gmx_kernel<float>, grid, dim3(kBlk), 0, st, (const float *)G, Nd, ld,
+                               p.span, (const float *)V, p.pack_rows, p.ncol, part);
+        else
+            hipLaunchKernelGGL(gmx_kernel<__hip_bfloat16>, grid, dim3(kBlk), 0, st,
+                               (const __hip_bfloat16 *)G, Nd, ld, p.span, (const __bf16 *)V,
+                               p.pack_rows, p.ncol, part);
+        return hipGetLastError();
     }
-    switch (dt) {
-    case GRA_F64: return mm_t<double>(G, Nd, Nm, ld, V, ldv, p.ncol, p.splits, p.span, part, st);
-    case GRA_F32: return mm_t<float>(G, Nd, Nm, ld, V, ldv, p.ncol, p.splits, p.span, part, st);
-    default:
-        return mm_t<__hip_bfloat16>(G, Nd, Nm, ld, V, ldv, p.ncol, p.splits, p.span, part, st);
-    }
+    const int64_t waves = (ld + gt_wave_cols(dt) - 1) / gt_wave_cols(dt);
+    const dim3 grid(nblk(waves, kWavesPerBlk), (unsigned)p.splits, (unsigned)ctiles);
+    if (dt == GRA_F64)
+        hipLaunchKernelGGL((gtx_kernel<double, 2>), grid, dim3(kBlk), 0, st, (const double *)G, Nd,
+                           Nm, ld, p.span, (const double *)V, p.pack_rows, waves, p.ncol, part);
+    else if (dt == GRA_F32)
+        hipLaunchKernelGGL((gtx_kernel<float, 2>), grid, dim3(kBlk), 0, st, (const float *)G, Nd,
+                           Nm, ld, p.span, (const float *)V, p.pack_rows, waves, p.ncol, part);
+    else
+        hipLaunchKernelGGL(gtb_kernel, grid, dim3(kBlk), 0, st, (const __hip_bfloat16 *)G, Nd, Nm,
+                           ld, p.span, (const unsigned short *)V, p.pack_rows / 4,
+                           (int64_t)p.ncol * p.pack_rows, waves, p.ncol, part);
+    return hipGetLastError();
 }
 
 hipError_t launch_sum_splits(const double *part, const MatMulPlan &p, int64_t n, int64_t k,

@@ -67,6 +67,27 @@ def test_split_summation_vs_numpy(ctx, store):
     _check(f.GT_mat_mul(R), G.T, R, store)
 
 
+@pytest.mark.parametrize("store", ["f64", "f32", "bf16"])
+def test_small_survey_g_vs_numpy(ctx, store):
+    """The 5,000 x 32,000 G of the small inversion (gz + Tzz on the 40 x 40 x
+    20 mesh) with 16 right-hand sides.  Its 5,000 rows once split into spans
+    of 1,667 and 1,666 rows, which broke the earlier G^T R kernel: lanes that
+    left its row loop one turn early still fed their stale registers to the
+    wave's last MFMAs (4e-2 relative error)."""
+    from graada3d_amd.workloads import forward_problem_small
+    print(f"mat_mul small survey store {store}", flush=True)
+    fp = forward_problem_small()
+    f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
+    f.compute_G(store)
+    G = f.get_G()
+    assert G.shape == (5000, 32000)
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((f.Nm, 16))
+    R = rng.standard_normal((f.Nd, 16))
+    _check(f.G_mat_mul(X), G, X, store)
+    _check(f.GT_mat_mul(R), G.T, R, store)
+
+
 def test_columns_equal_single_vector_products(ctx):
     f, G = _fwd(ctx, (12, 10, 6), 6, 0x3FE, "f64")
     rng = np.random.default_rng(9)

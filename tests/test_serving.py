@@ -255,6 +255,56 @@ def test_results_alone_equal_in_the_mix():
 
 
 @pytest.mark.gpu
+def test_results_alone_equal_in_the_mix_at_the_benchmark_sizes():
+    """The same bit-identity on the serving mix at its specified sizes: forward
+    requests of 5k-130k cells on 1k-20k points with 1-9 components (none
+    capped), CG requests on the 5,000 x 32,000 survey, on the largest (11.5 GB)
+    and a median octree instance and on the pinned 90,000 x 131,072
+    nine-component survey (47 GB of FP32 G), under a budget that holds all but
+    half of the smallest problem, so that admissions evict."""
+    import dataclasses
+    S._instances()
+    base = S.default_cg_pool()
+    gb = [S.ResidentCache.g_bytes(p) for p in base]
+    order = np.argsort(gb[1:-1])
+    k_big, k_mid = int(order[-1]), int(order[len(order) // 2])
+    pool = [dataclasses.replace(base[0], weight=0.3),
+            dataclasses.replace(base[1 + k_big], weight=0.25),
+            dataclasses.replace(base[1 + k_mid], weight=0.25),
+            dataclasses.replace(base[-1], weight=0.2)]
+    assert pool[-1].name == "tensor_large" and pool[-1].pinned
+    sizes = [S.ResidentCache.g_bytes(p) for p in pool]
+    budget = sum(sizes) - 0.5 * min(sizes)
+    reqs = S.request_stream(200.0, 0.5, 22, pool)
+    kinds = {r.rid: r.kind for r in reqs}
+    fwd = [r.params for r in reqs if r.kind == "forward"]
+    per_problem = np.bincount([r.params["problem"] for r in reqs if r.kind == "cg"],
+                              minlength=len(pool))
+    # the stream reaches the ends of the specified ranges and every problem
+    assert len(fwd) > 60 and np.all(per_problem >= 1)
+    assert max(p["cells"] for p in fwd) > 100000 and max(p["npts"] for p in fwd) > 15000
+    assert min(p["cells"] for p in fwd) < 10000
+    print("CG requests per problem:", dict(zip([p.name for p in pool], per_problem.tolist())),
+          f"budget {budget / 1e9:.2f} GB of {sum(sizes) / 1e9:.2f}", flush=True)
+    mix = S.serve(reqs, pool, budget_bytes=budget, workers=4, cg_workers=1, outputs=True)
+    print("mix:", mix["forward_batches"], mix["cache"], flush=True)
+    alone = S.serve(reqs, pool, budget_bytes=budget, workers=1, cg_workers=0, max_batch=1,
+                    outputs=True)
+    print("alone:", alone["cache"], flush=True)
+    assert mix["max_in_service"] >= 2 and alone["max_in_service"] == 1
+    assert alone["forward_batches"]["launches"] == 0
+    assert mix["cache"]["evictions"] >= 1 and alone["cache"]["evictions"] >= 1
+    assert set(mix["outputs"]) == set(alone["outputs"]) == set(kinds)
+    for rid, a in alone["outputs"].items():
+        b = mix["outputs"][rid]
+        if kinds[rid] == "forward":
+            assert a.dtype == np.float32 and np.array_equal(a, b), rid
+        else:
+            assert np.array_equal(a["dx"], b["dx"]), rid
+            assert a["err"] == b["err"] and a["iterations"] == b["iterations"] == 20, rid
+
+
+@pytest.mark.gpu
 def test_batched_forward_equals_single_launches():
     """gra_forward_rectilinear_batch: uneven meshes, point sets and component
     sets in one launch, each result bit-identical to its own launch."""
