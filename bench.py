@@ -64,13 +64,15 @@ no JSON line):
   gz+Tzz at 50x50 points and all nine at 30x30, three times each.
 * mat-mul: the matrix-core products with 16 right-hand sides on the
   5,000 x 32,000 G in FP64, FP32 and BF16 (MFMA counters).
+* mat-mul-large: the same on the 90,000 x 131,072 G of tensor_large
+  (assembled by corner sums).
 
     python bench.py [--workload tensor_large|inversion_small|batch|serving|karst] [--steps K]
                     [--warmup W]
                     [--stores f64,f32,bf16] [--no-fused] [--no-cpu] [--gpus 1]
                     [--rate R] [--duration S] [--budget-gb B] [--policy gdsf|lru]
                     [--max-batch K] [--workers W]
-    python bench.py --probe gemv|gemv-large|assemble|octree|mat-mul [--all-dtypes]
+    python bench.py --probe gemv|gemv-large|assemble|octree|mat-mul|mat-mul-large [--all-dtypes]
 """
 from __future__ import annotations
 
@@ -271,7 +273,13 @@ def run(args):
     return pb, nd, nm, results, cpu
 
 
-MFMA_PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 157.3}   # TFLOP/s of the MFMA flavour used
+# TFLOP/s of the MFMA flavour used (dense peaks); BF16 G runs on the BF16
+# matrix cores with each vector split into three BF16 parts, so it issues
+# three MFMA products per useful one
+MFMA_PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+MFMA_NAME = {"f64": "f64 16x16x4", "f32": "f32 16x16x4",
+             "bf16": "bf16 16x16x32 (G X) / 16x16x16 (G^T R), vectors as 3 BF16 parts"}
+MFMA_ISSUE = {"f64": 1.0, "f32": 1.0, "bf16": 3.0}
 MM_RHS = 16
 
 
@@ -284,8 +292,7 @@ def mat_mul_rates(ctx, inv, store, gbytes, gn_ms, gt_ms):
     rng = np.random.default_rng(17)
     X = rng.standard_normal((inv.Nm, MM_RHS))
     R = rng.standard_normal((inv.Nd, MM_RHS))
-    out = {"rhs": MM_RHS, "mfma": "f64 16x16x4" if store == "f64" else "f32 16x16x4",
-           "peak_TFLOPs": MFMA_PEAK[store]}
+    out = {"rhs": MM_RHS, "mfma": MFMA_NAME[store], "peak_TFLOPs": MFMA_PEAK[store]}
     for name, fn, arg, vec_ms in (("G_X", inv.G_mat_mul, X, gn_ms), ("GT_R", inv.GT_mat_mul, R, gt_ms)):
         fn(arg)
         ctx.timing(True)
@@ -296,7 +303,8 @@ def mat_mul_rates(ctx, inv, store, gbytes, gn_ms, gt_ms):
         ctx.timing(False)
         tf = 2.0 * inv.Nd * inv.Nm * MM_RHS / (ms * 1e-3) / 1e12
         out[name] = {"avg_ms": ms, "G_GBps": gbytes / (ms * 1e-3) / 1e9, "TFLOPs": tf,
-                     "mfma_frac": tf / MFMA_PEAK[store],
+                     "mfma_frac": tf * MFMA_ISSUE[store] / MFMA_PEAK[store],
+                     "hbm_frac": gbytes / (ms * 1e-3) / 8e12,
                      "per_vector_speedup_vs_gemv": vec_ms * MM_RHS / ms if vec_ms else None}
     return out
 
@@ -756,14 +764,18 @@ def run_probe(args) -> None:
     from graada3d_amd.workloads import forward_problem_small
     from graada3d_amd import mesh as M
 
-    if args.probe == "mat-mul":
+    if args.probe in ("mat-mul", "mat-mul-large"):
         ctx = Context(0)
-        fp = forward_problem_small()
+        if args.probe == "mat-mul":
+            fp, method = forward_problem_small(), "faces"
+        else:
+            from graada3d_amd.workloads import tensor_problem_large
+            fp, method = tensor_problem_large(), "nodes"
         f = Fwd(fp.prisms, fp.points, fp.flag, ctx=ctx)
         rng = np.random.default_rng(0)
         X, R = rng.standard_normal((f.Nm, 16)), rng.standard_normal((f.Nd, 16))
         for store in ("f64", "f32", "bf16"):
-            f.compute_G(store)
+            f.compute_G(store, method=method)
             for _ in range(2):
                 f.G_mat_mul(X)
                 f.GT_mat_mul(R)
@@ -897,7 +909,8 @@ def main():
     ap.add_argument("--karst-threshold", type=float, default=5e-4,
                     help="karst: relative threshold of the wavelet compression (method 2)")
     ap.add_argument("--probe", default=None,
-                    choices=["gemv", "gemv-large", "assemble", "octree", "mat-mul"],
+                    choices=["gemv", "gemv-large", "assemble", "octree", "mat-mul",
+                             "mat-mul-large"],
                     help="run a fixed launch sequence for rocprofv3 counter passes instead")
     ap.add_argument("--all-dtypes", action="store_true",
                     help="probe gemv: also with G stored in FP32 and BF16")

@@ -256,6 +256,7 @@ struct MatMulPlan {
     int64_t span = 0;       // its length per split
     int64_t ldp = 0;        // partial-sum rows per right-hand side
     int64_t pack_rows = 0;  // rows of the packed right-hand sides
+    int mc = 1;             // G^T R: column tiles per wave
 };
 MatMulPlan plan_mat_mul(gra_dtype dt, bool trans, int64_t Nd, int64_t Nm, int64_t ld, int64_t k);
 // Bytes of the packed right-hand sides of a plan.
