@@ -118,18 +118,21 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
     const T *grow = G + growc * ld;
 
     auto cofs = [&](int j) { return (j >> 1) * 8 * E + kk * 2 * E + (j & 1) * E; };
+    // Whole chunks are loaded without masks (a masked load makes the
+    // compiler wait for every load in flight, the prefetched chunk's too);
+    // a partial last chunk (the end of G) is loaded with masks after them.
+    const int64_t nfull = (k1 - k0) / KC;
     v4u g[2][PIECES];
-    v4u xs[NPART][XPER];
-    // k1 and every piece start are multiples of E, so a piece below k1 lies
-    // wholly below it (likewise for X pieces)
-    auto load_g = [&](int64_t kc, v4u (&dst)[PIECES]) {
+    v4u xs[2][NPART][XPER];
+    auto load_g = [&](int64_t kc, v4u (&dst)[PIECES], bool masked) {
 #pragma unroll
         for (int j = 0; j < PIECES; ++j) {
             const int64_t c = kc + cofs(j);
-            dst[j] = c < k1 ? *reinterpret_cast<const v4u *>(grow + c) : v4u{0u, 0u, 0u, 0u};
+            if (!masked) dst[j] = *reinterpret_cast<const v4u *>(grow + c);
+            else dst[j] = c < k1 ? *reinterpret_cast<const v4u *>(grow + c) : v4u{0u, 0u, 0u, 0u};
         }
     };
-    auto load_x = [&](int64_t kc) {
+    auto load_x = [&](int64_t kc, v4u (&dst)[NPART][XPER], bool masked) {
 #pragma unroll
         for (int p = 0; p < NPART; ++p)
 #pragma unroll
@@ -137,16 +140,17 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
                 const int q = threadIdx.x + kBlk * u, n = q / XPC, pc = q % XPC;
                 const int64_t c = kc + (int64_t)pc * XV;
                 const In *src = Xp + ((int64_t)p * ncol + ct * 16 + n) * ldx + c;
-                xs[p][u] = c < k1 ? *reinterpret_cast<const v4u *>(src) : v4u{0u, 0u, 0u, 0u};
+                if (!masked) dst[p][u] = *reinterpret_cast<const v4u *>(src);
+                else dst[p][u] = c < k1 ? *reinterpret_cast<const v4u *>(src) : v4u{0u, 0u, 0u, 0u};
             }
     };
-    auto store_x = [&](int b) {
+    auto store_x = [&](int b, const v4u (&src)[NPART][XPER]) {
 #pragma unroll
         for (int p = 0; p < NPART; ++p)
 #pragma unroll
             for (int u = 0; u < XPER; ++u) {
                 const int q = threadIdx.x + kBlk * u, n = q / XPC, pc = q % XPC;
-                *reinterpret_cast<v4u *>(&xl[b][p][n * XROW + pc * 16]) = xs[p][u];
+                *reinterpret_cast<v4u *>(&xl[b][p][n * XROW + pc * 16]) = src[p][u];
             }
     };
 
@@ -192,36 +196,43 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
         if constexpr (BF) flush(accml);                // 512 products of the mid / lo parts
     };
 
-    // chunk kc: G in g[slot], X in xl[b]; the next chunk's loads are issued
-    // (X first, so that storing it waits only for its own loads) before this
-    // chunk's MFMAs
-    int64_t kc = k0;
-    load_x(kc);
-    load_g(kc, g[0]);
-    store_x(0);
-    __syncthreads();
+    // Whole chunks: chunk ch's G in g[ch & 1] is loaded one chunk ahead, its
+    // X two chunks ahead (X loads are issued before the G loads of the same
+    // turn, so that storing X to LDS waits only for loads issued a turn
+    // earlier); X of chunk ch is in xl[ch & 1].
+    auto xchunk = [&](int64_t ch) { return k0 + (ch < nfull ? ch : nfull - 1) * KC; };
     int b = 0;
-    for (; kc < k1; kc += 2 * KC) {
-        const bool more1 = kc + KC < k1;
-        if (more1) {
-            load_x(kc + KC);
-            load_g(kc + KC, g[1]);
-        }
-        compute(b, g[0]);
-        if (!more1) break;
-        store_x(b ^ 1);                                // xl[b ^ 1] was last read before the previous barrier
+    if (nfull > 0) {
+        load_x(xchunk(0), xs[0], false);
+        store_x(0, xs[0]);
+        load_x(xchunk(1), xs[1], false);
+        load_g(k0, g[0], false);
         __syncthreads();
-        b ^= 1;
-        const bool more2 = kc + 2 * KC < k1;
-        if (more2) {
-            load_x(kc + 2 * KC);
-            load_g(kc + 2 * KC, g[0]);
+        for (int64_t ch = 0; ch < nfull; ch += 2) {
+            load_x(xchunk(ch + 2), xs[0], false);
+            if (ch + 1 < nfull) load_g(k0 + (ch + 1) * KC, g[1], false);
+            compute(b, g[0]);
+            if (ch + 1 >= nfull) break;
+            store_x(b ^ 1, xs[1]);                     // xl[b ^ 1] was last read before the previous barrier
+            __syncthreads();
+            b ^= 1;
+            load_x(xchunk(ch + 3), xs[1], false);
+            if (ch + 2 < nfull) load_g(k0 + (ch + 2) * KC, g[0], false);
+            compute(b, g[1]);
+            if (ch + 2 >= nfull) break;
+            store_x(b ^ 1, xs[0]);
+            __syncthreads();
+            b ^= 1;
         }
-        compute(b, g[1]);
-        if (!more2) break;
-        store_x(b ^ 1);
+    }
+    if (k0 + nfull * KC < k1) {                        // partial last chunk
+        const int64_t kc = k0 + nfull * KC;
+        load_x(kc, xs[0], true);
+        load_g(kc, g[0], true);
+        __syncthreads();                               // every wave is done with xl[b ^ 1]
+        store_x(b ^ 1, xs[0]);
         __syncthreads();
-        b ^= 1;
+        compute(b ^ 1, g[0]);
     }
     if constexpr (F64) {
 #pragma unroll
