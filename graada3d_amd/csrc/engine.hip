@@ -1728,8 +1728,9 @@ gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t)
     for (int blk : {0, 1}) {
         const int64_t span = blk ? 2 * V * 256 : 4 * V * 64;   // columns a split keeps at least
         const int64_t max_splits = c.Nm / span > 0 ? c.Nm / span : 1;
-        for (int rows : {2, 4, 8}) {
+        for (int rows : {2, 4, 8, 12, 16}) {
             if (c.store == GRA_F64 && rows == 8 && !blk) continue;
+            if (rows > 8 && (c.store != GRA_BF16 || blk)) continue;   // BF16, wave-owned only
             for (int un : {1, 2})
                 for (int sp : {1, 2, 4, 8, 16}) {
                     if (sp > max_splits) continue;

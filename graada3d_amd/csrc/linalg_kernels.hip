@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 #include "engine.hpp"
 
 namespace gra {
@@ -133,6 +135,91 @@ gemv_n_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t cols_per_
 #pragma unroll
             for (int q = 0; q < V; ++q) acc[r] = fma(g[q], x[c + q], acc[r]);
         }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double v = acc[r];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+        acc[r] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (row0 + r < Nd) part[(int64_t)split * Nd + row0 + r] = acc[r];
+    }
+}
+
+// BF16 G x with up to 16 rows per wave: the 16-byte pieces are kept as
+// stored (8 BF16 values in 4 registers) and widened at the multiply-add, so
+// that a wave can hold 12-16 rows' pieces and load each 64 bytes of x once
+// for all of them (with the pieces widened at the load, 8 rows already took
+// the registers of occupancy 2).  Per lane and row the products are added in
+// the same order as gemv_n_kernel's, so the results are identical for the
+// same column splits.
+template <int R, int U, bool NT>
+__global__ void __launch_bounds__(kBlk)
+gemv_nr_kernel(const __hip_bfloat16 *__restrict__ G, int64_t Nd, int64_t ld, int64_t cols_per_split,
+               const double *__restrict__ x, double *__restrict__ part, const int *done)
+{
+    if (done && *done) return;
+    constexpr int V = 8;
+    constexpr int STEP = V * kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = (int64_t)blockIdx.x * (kBlk / kWave) + (threadIdx.x >> 6);
+    const int64_t row0 = wave * R;
+    const int split = blockIdx.y;
+    if (row0 >= Nd) return;
+    int64_t c0 = (int64_t)split * cols_per_split;
+    int64_t c1 = c0 + cols_per_split;
+    if (c1 > ld) c1 = ld;
+    const __hip_bfloat16 *rows[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int64_t row = row0 + r < Nd ? row0 + r : Nd - 1;   // clamp: duplicate reads, result dropped
+        rows[r] = G + row * ld;
+    }
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    auto load = [&](const __hip_bfloat16 *p) {
+        return NT ? __builtin_nontemporal_load(reinterpret_cast<const v4u32 *>(p))
+                  : *reinterpret_cast<const v4u32 *>(p);
+    };
+    auto madd = [&](const v4u32 &g, const double *xv, double &a) {
+        const uint32_t w[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a = fma((double)__uint_as_float(w[q] << 16), xv[2 * q], a);
+            a = fma((double)__uint_as_float(w[q] & 0xffff0000u), xv[2 * q + 1], a);
+        }
+    };
+    int64_t c = c0 + (int64_t)V * lane;
+    for (; c + (U - 1) * STEP < c1; c += U * STEP) {
+        v4u32 g[U][R];
+        double xv[U][V];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) g[u][r] = load(rows[r] + c + u * STEP);
+#pragma unroll
+            for (int q = 0; q < V; q += 2) {
+                double2 xx = *reinterpret_cast<const double2 *>(x + c + u * STEP + q);
+                xv[u][q] = xx.x;
+                xv[u][q + 1] = xx.y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r) madd(g[u][r], xv[u], acc[r]);
+    }
+    for (; c < c1; c += STEP) {
+        double xv[V];
+#pragma unroll
+        for (int q = 0; q < V; ++q) xv[q] = x[c + q];
+#pragma unroll
+        for (int r = 0; r < R; ++r) madd(load(rows[r] + c), xv, acc[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -340,12 +427,35 @@ void launch_n_nt(const void *G, int64_t Nd, int64_t ld, int splits, bool nt, boo
     }
 }
 
+template <int R, int U>
+void launch_nr(const void *G, int64_t Nd, int64_t ld, int splits, bool nt, const double *x,
+               double *part, const int *done, hipStream_t st)
+{
+    constexpr int STEP = 8 * kWave;
+    int64_t waves = (Nd + R - 1) / R;
+    int64_t blocks = (waves + (kBlk / kWave) - 1) / (kBlk / kWave);
+    int64_t cps = (ld + splits - 1) / splits;
+    cps = (cps + STEP - 1) / STEP * STEP;   // keep 16-B lane alignment
+    dim3 grid((unsigned)blocks, (unsigned)splits), block(kBlk);
+    const __hip_bfloat16 *g = (const __hip_bfloat16 *)G;
+    if (nt) hipLaunchKernelGGL((gemv_nr_kernel<R, U, true>), grid, block, 0, st, g, Nd, ld, cps, x, part, done);
+    else hipLaunchKernelGGL((gemv_nr_kernel<R, U, false>), grid, block, 0, st, g, Nd, ld, cps, x, part, done);
+}
+
 template <typename T>
 void launch_n_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, const double *x,
                     double *part, const int *done, hipStream_t st)
 {
     const bool nt = (s.nontemporal & 1) != 0, blk = s.n_block != 0;
     const int sp = s.n_splits;
+    if constexpr (std::is_same<T, __hip_bfloat16>::value) {
+        // 12 or 16 rows per wave: BF16 pieces kept as stored
+        if (!blk && s.n_rows >= 12) {
+            if (s.n_rows >= 16) launch_nr<16, 1>(G, Nd, ld, sp, nt, x, part, done, st);
+            else launch_nr<12, 2>(G, Nd, ld, sp, nt, x, part, done, st);
+            return;
+        }
+    }
     if (s.n_rows == 2) {
         if (s.n_unroll >= 2) launch_n_nt<T, 2, 2>(G, Nd, ld, sp, nt, blk, x, part, done, st);
         else launch_n_nt<T, 2, 1>(G, Nd, ld, sp, nt, blk, x, part, done, st);
@@ -381,16 +491,18 @@ void launch_t_shape(const void *G, int64_t Nd, int64_t ld, const GemvShape &s, c
 }  // namespace
 
 // Default launch shapes, from a sweep on MI355X (5,000 x 32,000 G in each
-// storage type): G x is fastest with 4 / 8 rows per wave for FP32 / BF16 and
-// ~8k waves in flight, and for FP64 with 4 rows shared by a workgroup;
+// storage type): G x is fastest with 4 rows per wave for FP32 and ~8k waves
+// in flight, for FP64 with 4 rows shared by a workgroup, for BF16 with 16
+// rows per wave kept as stored (90,000 x 131,072 G: 3.54 ms against 4.01 ms
+// with 8 rows widened at the load);
 // G^T w with long row chunks, ~256 blocks (one per CU) in total;
 // non-temporal loads help both.
 GemvShape default_gemv_shape(gra_dtype dt, int64_t Nd, int64_t Nm)
 {
     GemvShape s;
     const int V = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
-    s.n_rows = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 8);
-    s.n_unroll = 2;
+    s.n_rows = dt == GRA_F64 ? 2 : (dt == GRA_F32 ? 4 : 16);
+    s.n_unroll = dt == GRA_BF16 ? 1 : 2;
     s.t_unroll = 8;
     s.nontemporal = 3;
     int64_t waves = (Nd + s.n_rows - 1) / s.n_rows;
