@@ -642,6 +642,11 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
 {
     hipStream_t st = c.stream;
     const int64_t nd = c.Nd, nm = c.Nm, ld = c.ld;
+    // the last gn_step's affine single-read pass (see Ctx::gn_q_ready)
+    const int gm_splits = have_Gmk ? c.gn_gm_splits : 0;
+    bool have_q = have_Gmk && c.gn_q_ready && c.gn_q && c.gn_q_groups > 0;
+    const void *scratch_in = c.fused_scratch.p;
+    c.gn_q_ready = false;
     const bool crg = a.use_cross_gradient != 0;
     if (crg && !c.have_crg_ops)
         return set_error(c, GRA_BAD_ARG, "cross-gradient requested but no T operators uploaded");
@@ -678,6 +683,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     if (fused) {
         if ((e = c.fused_scratch.reserve(fused_scratch_bytes(plan, ld))) != hipSuccess)
             return hip_fail(c, e, "alloc fused scratch");
+        if (c.fused_scratch.p != scratch_in) have_q = false;   // reallocated: gn_q is gone
         hipLaunchKernelGGL(k_init_count, dim3(1), dim3(1), 0, st, fused_err);
     }
     const bool need_rloop = (fused || mixed) && crg && a.quirk_mode;
@@ -700,6 +706,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
             return hip_fail(c, e, "alloc refresh vectors");
     }
     if (have_Gmk && c.part_n.bytes < sizeof(double) * (size_t)splits * nd) have_Gmk = false;
+    if (!have_Gmk) have_q = false;
     if ((e = c.part_n.reserve(sizeof(double) * (size_t)splits * nd)) != hipSuccess)
         return hip_fail(c, e, "alloc part_n");
     c.partn_holds_gn_model = false;
@@ -788,19 +795,25 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     hipLaunchKernelGGL(k_fill, dim3(nblocks(nm)), dim3(kB), 0, st, d.dx, nm, 0.0);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(kB), 0, st, d.sc);
 
-    // b, s, first residual and A^T b
+    // b, s, first residual and A^T b (G m_k and, after an affine single-read
+    // pass, G^T w as well come from the last gn_step)
+    CgDev d0 = d;
+    if (have_Gmk && gm_splits > 0) d0.splits = gm_splits;
     if (!have_Gmk) {
         TimedRegion tr(c, kTimeGemvN);
         if ((e = launch_G_n(c, shape, d_mk, c.part_n.as<double>(), nullptr, st)))
             return hip_fail(c, e, "gemv_n");
     }
-    hipLaunchKernelGGL(k_b_init, dim3(nblocks(nrow)), dim3(kB), 0, st, d);
-    {
+    hipLaunchKernelGGL(k_b_init, dim3(nblocks(nrow)), dim3(kB), 0, st, d0);
+    if (have_q) {
+        d0.part_t = const_cast<double *>(c.gn_q);
+        d0.chunks = c.gn_q_groups;
+    } else {
         TimedRegion tr(c, kTimeGemvT);
         if ((e = launch_G_t(c, shape, d.w, c.part_t.as<double>(), nullptr, st)))
             return hip_fail(c, e, "gemv_t");
     }
-    hipLaunchKernelGGL(k_r_update<1>, dim3(nblocks(nm)), dim3(kB), 0, st, d);
+    hipLaunchKernelGGL(k_r_update<1>, dim3(nblocks(nm)), dim3(kB), 0, st, d0);
     hipLaunchKernelGGL(k_scalar_init, dim3(1), dim3(kB), 0, st, d, nblk_nm);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(c, e, "cg init");
 

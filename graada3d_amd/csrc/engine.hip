@@ -1420,10 +1420,41 @@ gra_status gn_enqueue(Ctx &c, double lambda, double cg_tol, int cg_maxit)
            "barrier_update");
     const GemvShape shape = gemv_shape(c);
     HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * nd), "alloc");
-    HIPCHK(launch_G_n(c, shape, mt, c.part_n.as<double>(), nullptr, c.stream), "G x");
-    HIPCHK(launch_misfit(c.part_n.as<double>(), shape.n_splits, nd, c.gn_Wd.as<double>(),
-                         c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
-           "misfit");
+    // With the single-read iteration the misfit product is an affine
+    // single-read pass: G m_trial (into part_n, one split) and, from the same
+    // read of G, G^T (Wd^2 (G m_trial - dobs)) for the next solve's initial
+    // residual.  Its bounded waits report through the solve's status flag.
+    FusedPlan plan;
+    const bool mixed = c.mixed_refresh > 0 && c.Glo.p && c.lo_version == c.g_version;
+    const bool affine = c.gn_fused && c.pend_fused && !c.fused_failed && !c.wav && !mixed &&
+                        fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth,
+                                   c.fused_math, c.fused_min_blocks, &plan) &&
+                        c.fused_scratch.bytes >= fused_scratch_bytes(plan, ld);
+    if (affine) {
+        int *fused_err = c.status.as<int>() + 1;
+        double *zp = nullptr, *t2p = nullptr;
+        HIPCHK(launch_fused(plan, c.G.p, nd, ld, mt, c.gn_Wd.as<double>(), nullptr,
+                            c.fused_scratch.p, fused_err, nullptr, c.fused_fail_block, &zp, &t2p,
+                            c.stream, c.gn_dobs.as<double>(), c.part_n.as<double>()),
+               "affine single-read pass");
+        HIPCHK(launch_misfit(c.part_n.as<double>(), 1, nd, c.gn_Wd.as<double>(),
+                             c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
+               "misfit");
+        HIPCHK(hipMemcpyAsync(&c.host_stage->ferr, fused_err, sizeof(int), hipMemcpyDeviceToHost,
+                              c.stream),
+               "copy status");
+        c.gn_gm_splits = 1;
+        c.gn_q = zp;
+        c.gn_q_groups = plan.S;
+        c.gn_q_ready = true;
+    } else {
+        HIPCHK(launch_G_n(c, shape, mt, c.part_n.as<double>(), nullptr, c.stream), "G x");
+        HIPCHK(launch_misfit(c.part_n.as<double>(), shape.n_splits, nd, c.gn_Wd.as<double>(),
+                             c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
+               "misfit");
+        c.gn_gm_splits = 0;
+        c.gn_q_ready = false;
+    }
     // part_n now holds G * m_trial, which becomes the next step's G * m_k
     c.partn_holds_gn_model = true;
     // accept the trial model
@@ -1467,6 +1498,7 @@ gra_status gn_collect(Ctx &c, double *misfit, double *cg_err, int *cg_iterations
         c.gn_have_wl = c.gn_backup_wl;
         // part_n was overwritten by the misfit product of the failed step
         c.partn_holds_gn_model = false;
+        c.gn_q_ready = false;
         (void)c.gn_backup_partn;
         if ((s = gn_enqueue(c, c.gn_args.lambda, c.gn_args.tol, c.gn_args.maxit)) != GRA_OK)
             return s;

@@ -177,6 +177,15 @@ struct Ctx {
     // part_n holds the column-split partials of G * gn_m (left by the last
     // gn_step's misfit product); the next step reuses them for G * m_k.
     bool partn_holds_gn_model = false;
+    // With the single-read iteration, the gn_step's misfit product is an
+    // affine single-read pass: part_n then holds G * gn_m in one split
+    // (gn_gm_splits) and gn_q the row-group partials (gn_q_groups of ld) of
+    // G^T (Wd^2 (G gn_m - dobs)), which the next solve uses for its initial
+    // residual instead of a G^T product; valid with partn_holds_gn_model.
+    int gn_gm_splits = 0;
+    bool gn_q_ready = false;
+    const double *gn_q = nullptr;
+    int gn_q_groups = 0;
     bool last_solve_fused = false;   // the last CG solve used the single-read product
     int gn_fused = 0;
     int gn_lp = 2;
@@ -335,10 +344,12 @@ bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs
                 int math, int min_blocks, FusedPlan *out);
 size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld);
 // fail_block >= 0 makes the wait on that row block fail (test hook).
+// dobs / yout: the affine pass (q = G^T (Wd^2 (G x - dobs)), y = G x to yout).
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
                         const double *x, const double *Wd, double *tdat, void *scratch, int *err,
                         const int *done, int64_t fail_block, double **zpart_out,
-                        double **t2part_out, hipStream_t st);
+                        double **t2part_out, hipStream_t st, const double *dobs = nullptr,
+                        double *yout = nullptr);
 
 // CGLS (cgls.hip)
 struct CgDiag {

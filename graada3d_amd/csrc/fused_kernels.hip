@@ -44,6 +44,11 @@
 // *err, abandons its remaining waits, runs to the end, and the host discards
 // the result and reruns the solve with the two-pass iteration.
 //
+// Affine pass (dobs given): c = Wd^2 (y - dobs) instead of Wd^2 y, and y
+// itself is written (to yout) instead of t_d: one read of G gives both the
+// data misfit of x and G^T (Wd^2 (G x - dobs)), the data part of the next
+// CG solve's initial residual A^T b.
+//
 // At the end every thread writes its columns of q for the group (a group's
 // columns are owned by exactly one tile, so there is no cross-workgroup sum);
 // the consumer adds the S group partials in order, so results are bit-for-bit
@@ -132,6 +137,8 @@ struct FusedArgs {
     int64_t nrb;          // row blocks per group
     const double *x;      // [ld] (zero beyond Nm)
     const double *Wd;     // [Nd] (null = 1)
+    const double *dobs;   // [Nd]: affine pass (null: the CG product)
+    double *yout;         // [Nd]: y = G x of the affine pass
     double *partial;      // [S*nrb][NT][RB], sentinel on entry
     double *tdat;         // [Nd]   Wd * y
     double *t2part;       // [S*nrb]
@@ -308,7 +315,7 @@ fused_normal_kernel(FusedArgs a)
     // The poll of block j and its rows' Wd are loaded one step early (vector
     // loads complete in order: a load issued and waited for within one step
     // would make the wave wait for that step's whole prefetched block).
-    auto issue_poll = [&](int64_t j, double (&v)[kMaxPoll], double &wd) {
+    auto issue_poll = [&](int64_t j, double (&v)[kMaxPoll], double &wd, double &dv) {
         const double *pb = a.partial + (rb0 + j) * NT * RB;
 #pragma unroll
         for (int u = 0; u < kMaxPoll; ++u) {
@@ -316,7 +323,9 @@ fused_normal_kernel(FusedArgs a)
             v[u] = idx < np ? ld_agent(pb + idx) : 0.0;
         }
         const int64_t row = g_row0 + j * RB + lane;
-        wd = (lane < RB && row < g_row1 && a.Wd) ? a.Wd[row] : 1.0;
+        const bool inr = lane < RB && row < g_row1;
+        wd = (inr && a.Wd) ? a.Wd[row] : 1.0;
+        dv = (inr && a.dobs) ? a.dobs[row] : 0.0;
     };
     auto pending = [&](const double (&v)[kMaxPoll]) {
         bool pend = false;
@@ -324,7 +333,7 @@ fused_normal_kernel(FusedArgs a)
         for (int u = 0; u < kMaxPoll; ++u) pend = pend || (lane + 64 * u < np && is_sentinel(v[u]));
         return __any(pend);
     };
-    auto finish_poll = [&](int64_t j, double (&v)[kMaxPoll], double wd, int par) {
+    auto finish_poll = [&](int64_t j, double (&v)[kMaxPoll], double wd, double dv, int par) {
         const double *pb = a.partial + (rb0 + j) * NT * RB;
         if (j == a.fail_block && !failed) {
             failed = true;
@@ -369,10 +378,18 @@ fused_normal_kernel(FusedArgs a)
         if (lane < RB) {
             double c = 0.0;
             if (valid) {
-                const double td = wd * y;
-                c = wd * td;
-                t2 = td * td;
-                if (writer) a.tdat[row] = td;     // read by later kernels only
+                if (a.dobs) {
+                    // affine pass: q = G^T (Wd^2 (y - dobs)), y kept
+                    const double td = wd * (y - dv);
+                    c = wd * td;
+                    t2 = td * td;
+                    if (writer) a.yout[row] = y;
+                } else {
+                    const double td = wd * y;
+                    c = wd * td;
+                    t2 = td * td;
+                    if (writer) a.tdat[row] = td;     // read by later kernels only
+                }
             }
             cl[par][lane] = c;
         }
@@ -381,7 +398,7 @@ fused_normal_kernel(FusedArgs a)
         if (writer && lane == 0) a.t2part[rb0 + j] = t2;
     };
 
-    double pv[kMaxPoll], pwd = 1.0;
+    double pv[kMaxPoll], pwd = 1.0, pdv = 0.0;
 #pragma unroll
     for (int u = 0; u < kMaxPoll; ++u) pv[u] = 0.0;
 
@@ -410,7 +427,7 @@ fused_normal_kernel(FusedArgs a)
             if (k >= nrb + L) break;
             const int64_t j = k - L;
             const int pj = (s - L + 2 * U) & 1, pk = s & 1;
-            if (wave == 0 && j >= 0 && j < nrb) finish_poll(j, pv, pwd, pj);
+            if (wave == 0 && j >= 0 && j < nrb) finish_poll(j, pv, pwd, pdv, pj);
             // slot (s+PF)%NS held block k+PF-NS = k-L-1, consumed at step k-1
             load_block(k + PF < last ? k + PF : last, ring[(s + PF) % NS]);
             if (k < nrb) phase_a(ring[s % NS], pk);
@@ -418,7 +435,7 @@ fused_normal_kernel(FusedArgs a)
             if (wave == 0) {
                 if (k < nrb) post(k, pk);
                 const int64_t jp = k - L + 1;
-                if (jp >= 0 && jp < nrb) issue_poll(jp, pv, pwd);
+                if (jp >= 0 && jp < nrb) issue_poll(jp, pv, pwd, pdv);
             }
             if (j >= 0 && j < nrb) phase_b(ring[(s - L + U) % NS], pj);
         }
@@ -546,7 +563,7 @@ CoopOrder coop_order[kMaxDevices];
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
                         const double *x, const double *Wd, double *tdat, void *scratch, int *err,
                         const int *done, int64_t fail_block, double **zpart_out,
-                        double **t2part_out, hipStream_t st)
+                        double **t2part_out, hipStream_t st, const double *dobs, double *yout)
 {
     const int64_t nrb = (int64_t)p.S * p.nrb_group;
     char *b = static_cast<char *>(scratch);
@@ -562,6 +579,8 @@ hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t l
     a.nrb = p.nrb_group;
     a.x = x;
     a.Wd = Wd;
+    a.dobs = dobs;
+    a.yout = yout;
     a.partial = reinterpret_cast<double *>(b);
     b += sizeof(double) * (size_t)(nrb * p.NT * p.RB);
     a.t2part = reinterpret_cast<double *>(b);
