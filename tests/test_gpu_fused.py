@@ -166,6 +166,10 @@ def test_fused_tolerance_stop(ctx):
 
 
 def test_fused_gauss_newton_l1_irls_vs_oracle(ctx):
+    """Three L1 (IRLS) Gauss-Newton steps with the single-read iteration
+    against the oracle's recurrence form.  From the second step on, the
+    solve's initial residual takes G^T (Wd^2 (G m - dobs)) from the previous
+    step's affine single-read misfit pass instead of a G^T product."""
     inv, G, aux = small_inversion(ctx)
     nm = inv.Nm
     lo, hi = np.full(nm, -1000.0), np.full(nm, 1000.0)
