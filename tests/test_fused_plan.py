@@ -11,8 +11,9 @@ import pytest
 
 LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostcheck", "libhostcheck.so")
 
-# (TW, RB) of the kernel per storage type: 4,096-column tiles, 64 KB blocks
-KERNELS = {"f64": (4096, 2), "f32": (4096, 4), "bf16": (4096, 8)}
+# (TW, RB) of the kernel per storage type: 64 KB blocks; 4,096-column tiles
+# (FP64 / FP32), 8,192 (BF16)
+KERNELS = {"f64": (4096, 2), "f32": (4096, 4), "bf16": (8192, 4)}
 MAX_POLL = 4
 
 
@@ -73,19 +74,22 @@ def test_plan_covers_and_fits(hc, store, nd, ld):
         check_cover(p, nd, ld, cus, n_xcd, per_cu)
 
 
-def test_plan_config3_bf16_is_one_group_per_xcd(hc):
+def test_plan_config3_bf16_groups_stay_in_an_xcd(hc):
+    # 16 tiles of 8,192 columns: two row groups per XCD, 32 workgroups each
     p = plan(hc, 90000, 131072, "bf16")
-    assert (p["NT"], p["S"], p["per_xcd"], p["grid"]) == (32, 8, 32, 256)
-    assert p["rows"] == 11256 and p["nrb"] == 1407
+    assert (p["NT"], p["S"], p["per_xcd"], p["grid"]) == (16, 16, 32, 256)
+    assert p["per_xcd"] % p["NT"] == 0
+    assert p["rows"] == 5628 and p["nrb"] == 1407 and p["rows"] * p["S"] >= 90000
 
 
 def test_plan_refuses_what_cannot_be_resident(hc):
     # no occupancy: the kernel does not fit a CU
     assert plan(hc, 5000, 32000, "bf16", per_cu=0) is None
     # more tiles than co-resident workgroups
-    assert plan(hc, 5000, 4096 * 40, "bf16", cus=32, n_xcd=1) is None
+    assert plan(hc, 5000, 8192 * 40, "bf16", cus=32, n_xcd=1) is None
     # a block's partials exceed the polling wave (NT * RB > 256)
-    assert plan(hc, 90000, 4096 * 33, "bf16") is None
+    assert plan(hc, 90000, 8192 * 65, "bf16") is None
+    assert plan(hc, 90000, 4096 * 65, "f32") is None
     # a requested grid is capped at the co-resident limit
     p = plan(hc, 90000, 131072, "f64", target=100000)
     assert p["grid"] <= 256
