@@ -776,7 +776,7 @@ def run_probe(args) -> None:
         X, R = rng.standard_normal((f.Nm, 16)), rng.standard_normal((f.Nd, 16))
         for store in ("f64", "f32", "bf16"):
             f.compute_G(store, method=method)
-            for _ in range(2):
+            for _ in range(4):                         # one warm-up call and three more
                 f.G_mat_mul(X)
                 f.GT_mat_mul(R)
             print(f"[probe] {store}: G X and G^T R with 16 right-hand sides", flush=True)

@@ -24,9 +24,10 @@
 // Work split:
 // * G X (gmx_kernel): a workgroup of four waves owns 64 rows (16 per wave)
 //   and one 16-wide tile of right-hand sides over one split of the columns.
-//   G goes straight from HBM into registers in the A-operand layout (no LDS
-//   round trip), one chunk ahead of use; the chunk's X values (shared by the
-//   64 rows) are staged in LDS, double-buffered, one barrier per chunk.
+//   G is read one chunk ahead of use in whole 256-byte row segments and
+//   turned into the A-operand layout through a small wave-private LDS tile;
+//   the chunk's X values (shared by the 64 rows) are staged in LDS,
+//   double-buffered, one barrier per chunk.
 // * G^T R (gtx_kernel, gtb_kernel): a wave owns a strip of G columns (16 B
 //   per lane and row, 256-byte row segments) and one tile of right-hand
 //   sides over one split of the rows; the next 16-row step's loads are issued
@@ -84,11 +85,17 @@ template <> struct MM<__hip_bfloat16> {
     static constexpr int E = 8, NPART = 3, KC = 256;
 };
 
-// Y = G X.  Lane (r, kk) of wave w owns row 64 b + 16 w + r; piece j of a
-// chunk is its 16 bytes at chunk column cofs(j, kk) (two neighbouring pieces
-// of a lane are adjacent, so that each pair of load instructions covers 128
-// contiguous bytes of every row).  The B operand of the same k values is
-// read from the LDS copy of X: rows = right-hand sides, columns = the chunk.
+// Y = G X.  Lane (r, kk) of wave w owns row 64 b + 16 w + r in the MFMA
+// layout; its piece j of a chunk is the 16 bytes at chunk column cofs(j, kk).
+// G is loaded with non-temporal loads of whole 256-byte row segments (load
+// instruction j: rows 4 (j & 3) + kk, 16-byte piece 16 (j >> 2) + r, i.e. four
+// rows per instruction) and turned into the MFMA layout through a wave-private
+// LDS tile, half a chunk at a time (piece p of row r at slot p ^ r: conflict-
+// free stores and loads).  Loading the MFMA layout directly touches 16 rows
+// per instruction with 64-byte segments, which held the kernel at 6.1 TB/s
+// with or without non-temporal loads; the staged form reads 6.5-6.7 TB/s on
+// the 90,000 x 131,072 G.  The B operand of the same k values is read from
+// the LDS copy of X: rows = right-hand sides, columns = the chunk.
 // Xp: NPART parts of ncol rows of ldx elements (part p, right-hand side c at
 // Xp + (p * ncol + c) * ldx).  part[(split * ncol + c) * Nd + row].
 template <typename T>
@@ -99,7 +106,7 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
 {
     typedef typename MM<T>::In In;
     constexpr int E = MM<T>::E, KC = MM<T>::KC, NPART = MM<T>::NPART;
-    constexpr int PIECES = KC / (4 * E);                 // 8
+    constexpr int PIECES = KC / (4 * E);                 // 8 (16 bytes each, 512 B of a row per chunk)
     constexpr int XROW = KC * (int)sizeof(In) + 16;      // LDS bytes per right-hand side (padded)
     constexpr int XT = 16 * XROW;                        // per part
     constexpr int XPC = KC * (int)sizeof(In) / 16;       // 16-byte X pieces per right-hand side
@@ -107,17 +114,25 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
     constexpr int XV = 16 / (int)sizeof(In);             // X elements per 16 bytes
     constexpr bool F64 = sizeof(In) == 8;
     constexpr bool BF = NPART == 3;
+    static_assert(PIECES == 8, "512-byte row chunks");
     __shared__ __attribute__((aligned(16))) unsigned char xl[2][NPART][XT];
+    __shared__ __attribute__((aligned(16))) v4u gl[kWavesPerBlk][16 * 16];   // 16 rows x half a chunk
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int r = lane & 15, kk = lane >> 4;
-    const int64_t row = (int64_t)blockIdx.x * 64 + wv * 16 + r;
-    const int64_t growc = row < Nd ? row : Nd - 1;       // clamped rows are not stored
+    const int64_t rbase = (int64_t)blockIdx.x * 64 + wv * 16;
     const int split = blockIdx.y, ct = blockIdx.z;
     const int64_t k0 = (int64_t)split * kspl;
     const int64_t k1 = k0 + kspl < ld ? k0 + kspl : ld;
-    const T *grow = G + growc * ld;
+    // rows read by load instruction j: 4 (j & 3) + kk (clamped rows are not stored)
+    const T *lrow[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t row = rbase + 4 * q + kk;
+        lrow[q] = G + (row < Nd ? row : Nd - 1) * ld;
+    }
+    v4u *tile = gl[wv];
 
-    auto cofs = [&](int j) { return (j >> 1) * 8 * E + kk * 2 * E + (j & 1) * E; };
+    auto cpiece = [&](int j) { return (j >> 1) * 8 + kk * 2 + (j & 1); };
     // Whole chunks are loaded without masks (a masked load makes the
     // compiler wait for every load in flight, the prefetched chunk's too);
     // a partial last chunk (the end of G) is loaded with masks after them.
@@ -127,10 +142,33 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
     auto load_g = [&](int64_t kc, v4u (&dst)[PIECES], bool masked) {
 #pragma unroll
         for (int j = 0; j < PIECES; ++j) {
-            const int64_t c = kc + cofs(j);
-            if (!masked) dst[j] = *reinterpret_cast<const v4u *>(grow + c);
-            else dst[j] = c < k1 ? *reinterpret_cast<const v4u *>(grow + c) : v4u{0u, 0u, 0u, 0u};
+            const int64_t c = kc + (int64_t)(16 * (j >> 2) + r) * E;
+            const v4u *src = reinterpret_cast<const v4u *>(lrow[j & 3] + c);
+            if (!masked) dst[j] = __builtin_nontemporal_load(src);
+            else dst[j] = c < k1 ? __builtin_nontemporal_load(src) : v4u{0u, 0u, 0u, 0u};
         }
+    };
+    // half h of a chunk (pieces 16 h .. 16 h + 15 of every row) from the
+    // loaded registers into the MFMA layout: dst[i] = piece cpiece(4 h + i)
+    // of row r.  The tile is private to the wave; its stores and loads are
+    // ordered at wavefront scope.
+    auto stage = [&](const v4u (&src)[PIECES], int h, v4u (&dst)[PIECES / 2]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * i + kk;
+            tile[row * 16 + (r ^ row)] = src[4 * h + i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int pc = cpiece(4 * h + i) - 16 * h;
+            dst[i] = tile[r * 16 + (pc ^ r)];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
     auto load_x = [&](int64_t kc, v4u (&dst)[NPART][XPER], bool masked) {
 #pragma unroll
@@ -164,12 +202,15 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
             acc[i] = 0.0f;
         }
     };
-    auto compute = [&](int b, const v4u (&gv)[PIECES]) {
+    auto compute = [&](int b, const v4u (&graw)[PIECES]) {
+        v4u gh[PIECES / 2];
 #pragma unroll
         for (int j = 0; j < PIECES; ++j) {
-            const int off = r * XROW + cofs(j) * (int)sizeof(In);
+            if ((j & 3) == 0) stage(graw, j >> 2, gh);
+            const v4u gv = gh[j & 3];
+            const int off = r * XROW + cpiece(j) * 16;
             if constexpr (BF) {
-                const v8bf a = __builtin_bit_cast(v8bf, gv[j]);
+                const v8bf a = __builtin_bit_cast(v8bf, gv);
                 const v8bf xh = *reinterpret_cast<const v8bf *>(&xl[b][0][off]);
                 const v8bf xm = *reinterpret_cast<const v8bf *>(&xl[b][1][off]);
                 const v8bf xo = *reinterpret_cast<const v8bf *>(&xl[b][2][off]);
@@ -182,12 +223,12 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
                 if constexpr (F64) {
 #pragma unroll
                     for (int e = 0; e < E; ++e)
-                        acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(el64(gv[j], e), el64(xv, e),
+                        acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(el64(gv, e), el64(xv, e),
                                                                       acc64, 0, 0, 0);
                 } else {
 #pragma unroll
                     for (int e = 0; e < E; ++e)
-                        acc32 = __builtin_amdgcn_mfma_f32_16x16x4f32(el32(gv[j], e), el32(xv, e),
+                        acc32 = __builtin_amdgcn_mfma_f32_16x16x4f32(el32(gv, e), el32(xv, e),
                                                                       acc32, 0, 0, 0);
                     if (j & 1) flush(acc32);           // 32 products
                 }
@@ -196,27 +237,25 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
         if constexpr (BF) flush(accml);                // 512 products of the mid / lo parts
     };
 
-    // Whole chunks: chunk ch's G in g[ch & 1] is loaded one chunk ahead, its
-    // X two chunks ahead (X loads are issued before the G loads of the same
-    // turn, so that storing X to LDS waits only for loads issued a turn
-    // earlier); X of chunk ch is in xl[ch & 1].
+    // Whole chunks: chunk ch's G in g[ch & 1] and its X in xs[ch & 1] are
+    // loaded one chunk ahead (X first, so that storing X to LDS waits only for
+    // loads issued before that chunk's G loads); X of chunk ch is in xl[ch & 1].
     auto xchunk = [&](int64_t ch) { return k0 + (ch < nfull ? ch : nfull - 1) * KC; };
     int b = 0;
     if (nfull > 0) {
         load_x(xchunk(0), xs[0], false);
         store_x(0, xs[0]);
-        load_x(xchunk(1), xs[1], false);
         load_g(k0, g[0], false);
         __syncthreads();
         for (int64_t ch = 0; ch < nfull; ch += 2) {
-            load_x(xchunk(ch + 2), xs[0], false);
+            load_x(xchunk(ch + 1), xs[1], false);
             if (ch + 1 < nfull) load_g(k0 + (ch + 1) * KC, g[1], false);
             compute(b, g[0]);
             if (ch + 1 >= nfull) break;
             store_x(b ^ 1, xs[1]);                     // xl[b ^ 1] was last read before the previous barrier
             __syncthreads();
             b ^= 1;
-            load_x(xchunk(ch + 3), xs[1], false);
+            load_x(xchunk(ch + 2), xs[0], false);
             if (ch + 2 < nfull) load_g(k0 + (ch + 2) * KC, g[0], false);
             compute(b, g[1]);
             if (ch + 2 >= nfull) break;
@@ -240,7 +279,6 @@ gmx_kernel(const T *__restrict__ G, int64_t Nd, int64_t ld, int64_t kspl,
     } else {
         flush(acc32);
     }
-    const int64_t rbase = (int64_t)blockIdx.x * 64 + wv * 16;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int orow = F64 ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
