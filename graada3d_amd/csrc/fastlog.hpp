@@ -24,6 +24,17 @@ namespace gra {
 
 constexpr int kLogBits = 7;
 
+// Series coefficients 1/8, 1/7, -1/6, 1/5, -1/4, 1/3 and ln2 as hi + lo.  In
+// device code they sit in constant memory, so that each use reads them with
+// scalar loads into SGPRs instead of materialising every one in VGPRs with
+// two moves per use (a quarter of the log's instructions otherwise).
+constexpr double kLogCoef[8] = {1.0 / 8.0, 1.0 / 7.0, -1.0 / 6.0, 1.0 / 5.0, -0.25, 1.0 / 3.0,
+                                6.93147180369123816490e-01, 1.90821492927058770002e-10};
+#if defined(__HIP_DEVICE_COMPILE__)
+__constant__ double kLogC[8] = {kLogCoef[0], kLogCoef[1], kLogCoef[2], kLogCoef[3],
+                                kLogCoef[4], kLogCoef[5], kLogCoef[6], kLogCoef[7]};
+#endif
+
 GRA_HD inline double table_log(double v, const double *G, const double *T)
 {
     v = fmin(fmax(v, 9.33263618503218879e-302), 1.07150860718626732e301);
@@ -32,16 +43,21 @@ GRA_HD inline double table_log(double v, const double *G, const double *T)
     const int j = (int)((b >> (52 - kLogBits)) & ((1u << kLogBits) - 1u));
     const double m = __builtin_bit_cast(double, (b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
     const double r = fma(m, G[j], -1.0);
-    double p = 1.0 / 8.0;
-    p = fma(p, -r, 1.0 / 7.0);
-    p = fma(p, r, -1.0 / 6.0);
-    p = fma(p, r, 1.0 / 5.0);
-    p = fma(p, r, -0.25);
-    p = fma(p, r, 1.0 / 3.0);
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double *C = kLogC;
+#else
+    const double *C = kLogCoef;
+#endif
+    double p = C[0];
+    p = fma(p, -r, C[1]);
+    p = fma(p, r, C[2]);
+    p = fma(p, r, C[3]);
+    p = fma(p, r, C[4]);
+    p = fma(p, r, C[5]);
     p = fma(p, r, -0.5);
     p = fma(p * r, r, r);
     const double ed = (double)e;
-    return fma(ed, 6.93147180369123816490e-01, T[j]) + fma(ed, 1.90821492927058770002e-10, p);
+    return fma(ed, C[6], T[j]) + fma(ed, C[7], p);
 }
 
 // table_log with the constant-memory table.

@@ -222,8 +222,12 @@ GRA_HD inline double edge_log(const Edge &e)
 #if GRA_TABLE_LOG
     const double v = (e.s1 + e.R1) / (e.s0 + e.R0);
     // the table log for positive finite ratios; 0, inf and NaN keep the
-    // library's value (the reference's special cases rely on them)
-    return (v > 0.0 && v < INFINITY) ? fast_log(v) : log(v);
+    // library's value (the reference's special cases rely on them), on a
+    // branch of its own so that the library log is not evaluated beside
+    // the table log for every edge
+    double lv = fast_log(v);
+    if (__builtin_expect(!(v > 0.0 && v < INFINITY), 0)) lv = log(v);
+    return lv;
 #else
     return log((e.s1 + e.R1) / (e.s0 + e.R0));
 #endif
