@@ -22,7 +22,7 @@ namespace gra {
 // with scalar loads into SGPRs, where the FMAs take them directly; as
 // literals the compiler materialised them in VGPRs with two moves per
 // coefficient and use (about 13 % of the assembly kernels' instructions).
-__constant__ double kAtanPc[kAtanDeg + 1] = {
+static __constant__ double kAtanPc[kAtanDeg + 1] = {
     kAtanP[0],  kAtanP[1],  kAtanP[2],  kAtanP[3],  kAtanP[4],  kAtanP[5],  kAtanP[6],
     kAtanP[7],  kAtanP[8],  kAtanP[9],  kAtanP[10], kAtanP[11], kAtanP[12], kAtanP[13],
     kAtanP[14], kAtanP[15], kAtanP[16], kAtanP[17], kAtanP[18], kAtanP[19], kAtanP[20]};

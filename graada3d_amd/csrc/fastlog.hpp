@@ -31,7 +31,7 @@ constexpr int kLogBits = 7;
 constexpr double kLogCoef[8] = {1.0 / 8.0, 1.0 / 7.0, -1.0 / 6.0, 1.0 / 5.0, -0.25, 1.0 / 3.0,
                                 6.93147180369123816490e-01, 1.90821492927058770002e-10};
 #if defined(__HIP_DEVICE_COMPILE__)
-__constant__ double kLogC[8] = {kLogCoef[0], kLogCoef[1], kLogCoef[2], kLogCoef[3],
+static __constant__ double kLogC[8] = {kLogCoef[0], kLogCoef[1], kLogCoef[2], kLogCoef[3],
                                 kLogCoef[4], kLogCoef[5], kLogCoef[6], kLogCoef[7]};
 #endif
 
