@@ -88,6 +88,21 @@ def test_small_survey_g_vs_numpy(ctx, store):
     _check(f.GT_mat_mul(R), G.T, R, store)
 
 
+@pytest.mark.parametrize("store", ["f64", "f32", "bf16"])
+def test_tiny_g_one_partial_chunk(ctx, store):
+    """32 cells (one 64-column row pitch, less than one G X chunk for FP32 /
+    BF16, so only the masked tail chunk runs) and 24 rows (one partial
+    64-row workgroup)."""
+    print(f"mat_mul tiny store {store}", flush=True)
+    f, G = _fwd(ctx, (4, 4, 2), 3, 0x12, store)
+    assert G.shape == (24, 32)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((f.Nm, 3))
+    R = rng.standard_normal((f.Nd, 3))
+    _check(f.G_mat_mul(X), G, X, store)
+    _check(f.GT_mat_mul(R), G.T, R, store)
+
+
 def test_columns_equal_single_vector_products(ctx):
     f, G = _fwd(ctx, (12, 10, 6), 6, 0x3FE, "f64")
     rng = np.random.default_rng(9)
