@@ -364,12 +364,12 @@ __device__ inline double wave_sparse_dot(const int32_t *__restrict__ idx,
 #pragma unroll
         for (int u = 0; u < kSpU; ++u) {
             const int64_t k = k0 + (int64_t)kWave * u;
-            c[u] = idx[k < e ? k : last];
+            c[u] = __builtin_nontemporal_load(idx + (k < e ? k : last));
         }
 #pragma unroll
         for (int u = 0; u < kSpU; ++u) {
             const int64_t k = k0 + (int64_t)kWave * u;
-            vv[u] = val[k < e ? k : last];
+            vv[u] = __builtin_nontemporal_load(val + (k < e ? k : last));
             g[u] = x(c[u]);
         }
 #pragma unroll
