@@ -352,20 +352,22 @@ def cpu_baseline(args, inv, pb, dobs, Wd):
     Gs = rng.standard_normal((rows, fp.prisms.shape[0]))
     x = rng.standard_normal(fp.prisms.shape[0])
     y = rng.standard_normal(rows)
+    reps = 30                       # about 10 s on one core
+
     def products():
-        for _ in range(3):
+        for _ in range(reps):
             oracle.gemv(Gs, x)
             oracle.gemv(Gs, y, trans=True)
 
     c0 = time.perf_counter()
     products()
     c1 = time.perf_counter()
-    per_iter = (c1 - c0) / 3 * (inv.Nd / rows)
-    ac = all_cores(products, 3 * rows / inv.Nd, "CG iter/s")
+    per_iter = (c1 - c0) / reps * (inv.Nd / rows)
+    ac = all_cores(products, reps * rows / inv.Nd, "CG iter/s")
     return {"value": 1.0 / per_iter, "unit": "CG iter/s", "cores": 1, "kind": "port",
             "sample": f"single-threaded C oracle G x and G^T w on {rows} rows x {inv.Nm} columns "
-                      f"(3 repetitions, {c1 - c0:.1f} s), extrapolated to {inv.Nd} rows; host has "
-                      f"{os.cpu_count()} logical CPUs",
+                      f"({reps} repetitions, {c1 - c0:.1f} s), extrapolated to {inv.Nd} rows; host "
+                      f"has {os.cpu_count()} logical CPUs",
             "all_cores": ac}
 
 
