@@ -303,6 +303,22 @@ template <> __device__ inline __hip_bfloat16 store_as<__hip_bfloat16>(double v)
     return __float2bfloat16((float)v);
 }
 
+// Two neighbouring G entries (g even-aligned) in one store.
+template <typename T> __device__ inline void store_pair(T *g, T a, T b)
+{
+    if constexpr (sizeof(T) == 8) {
+        typedef double P __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<P *>(g) = P{(double)a, (double)b};
+    } else if constexpr (sizeof(T) == 4) {
+        typedef float P __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<P *>(g) = P{(float)a, (float)b};
+    } else {
+        const uint32_t lo = __builtin_bit_cast(unsigned short, a);
+        const uint32_t hi = __builtin_bit_cast(unsigned short, b);
+        *reinterpret_cast<uint32_t *>(g) = lo | (hi << 16);
+    }
+}
+
 // G by corner sums.  One workgroup per (point, tile of tile x tile cell
 // columns with all nz levels): the corner values of the tile's
 // (tile+1)^2 (nz+1) nodes go to LDS once (component-major, nmax per
@@ -367,10 +383,18 @@ node_assemble_kernel(const double *__restrict__ xs, const double *__restrict__ y
             const double e0 = (v[plane + NZ] - v[NZ]) - (v[plane] - v[0]);
             const double e1 = (v[plane + NZ + 1] - v[NZ + 1]) - (v[plane + 1] - v[1]);
             T *g = G + ((int64_t)q * N_obs + ip) * ld + j;
-            g[0] = store_as<T>(comp_scale(bit) * (e1 - e0));
+            const T y0 = store_as<T>(comp_scale(bit) * (e1 - e0));
             if (second) {
                 const double e2 = (v[plane + NZ + 2] - v[NZ + 2]) - (v[plane + 2] - v[2]);
-                g[1] = store_as<T>(comp_scale(bit) * (e2 - e1));
+                const T y1 = store_as<T>(comp_scale(bit) * (e2 - e1));
+                if ((j & 1) == 0) {
+                    store_pair(g, y0, y1);        // G rows start at even columns
+                } else {
+                    g[0] = y0;
+                    g[1] = y1;
+                }
+            } else {
+                g[0] = y0;
             }
             ++q;
         }
