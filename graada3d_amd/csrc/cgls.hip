@@ -680,8 +680,18 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
                      : launch_G_t(c, shape, w, c.part_t.as<double>(), dn, st);
     };
     int *fused_err = c.status.as<int>() + 1;
+    // Mixed precision with the single-read iteration: each FP64 residual
+    // refresh is one affine single-read pass over the FP64 G as well (G x
+    // and G^T (Wd^2 (G x - dobs)) from one read), on its own plan; the two
+    // plans share the scratch, used one after the other on the stream.
+    FusedPlan plan64;
+    const bool refresh_single = fused && mixed &&
+                                fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth,
+                                           c.fused_math, c.fused_min_blocks, &plan64);
     if (fused) {
-        if ((e = c.fused_scratch.reserve(fused_scratch_bytes(plan, ld))) != hipSuccess)
+        size_t need = fused_scratch_bytes(plan, ld);
+        if (refresh_single) need = std::max(need, fused_scratch_bytes(plan64, ld));
+        if ((e = c.fused_scratch.reserve(need)) != hipSuccess)
             return hip_fail(c, e, "alloc fused scratch");
         if (c.fused_scratch.p != scratch_in) have_q = false;   // reallocated: gn_q is gone
         hipLaunchKernelGGL(k_init_count, dim3(1), dim3(1), 0, st, fused_err);
@@ -838,6 +848,28 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     // next p update ends the iteration from this r.
     auto refresh_residual = [&]() -> hipError_t {
         hipLaunchKernelGGL(k_refresh_vec, dim3(nblocks(ld)), dim3(kB), 0, st, d);
+        if (refresh_single) {
+            // y = G (mk + P dx) into part_n (one split) and G^T (Wd^2 (y -
+            // dobs)) = G^T w into the group partials, from one read of G
+            CgDev dr = d;
+            double *zp = nullptr, *t2p = nullptr;
+            {
+                TimedRegion tr(c, kTimeRefresh);
+                hipError_t er = launch_fused(plan64, c.G.p, nd, ld, d.vmk, dg.Wd, nullptr,
+                                             c.fused_scratch.p, fused_err, done,
+                                             c.fused_fail_block, &zp, &t2p, st, d_dobs,
+                                             c.part_n.as<double>());
+                if (er != hipSuccess) return er;
+            }
+            dr.splits = 1;
+            hipLaunchKernelGGL(k_refresh_s, dim3(nblocks(nrow)), dim3(kB), 0, st, dr);
+            dr.part_t = zp;
+            dr.chunks = plan64.S;
+            hipLaunchKernelGGL((k_r_loop<false, false>), dim3((unsigned)nblk_r), dim3(kB), 0, st,
+                               dr);
+            ++n_refresh;
+            return hipGetLastError();
+        }
         {
             TimedRegion tr(c, kTimeRefresh);
             hipError_t er = launch_G_n(c, shape, d.vmk, c.part_n.as<double>(), done, st);

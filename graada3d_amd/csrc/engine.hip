@@ -1424,9 +1424,10 @@ gra_status gn_enqueue(Ctx &c, double lambda, double cg_tol, int cg_maxit)
     // single-read pass: G m_trial (into part_n, one split) and, from the same
     // read of G, G^T (Wd^2 (G m_trial - dobs)) for the next solve's initial
     // residual.  Its bounded waits report through the solve's status flag.
+    // (With the mixed-precision copy the pass reads the FP64 G, as the
+    // misfit and the initial residual of that mode do.)
     FusedPlan plan;
-    const bool mixed = c.mixed_refresh > 0 && c.Glo.p && c.lo_version == c.g_version;
-    const bool affine = c.gn_fused && c.pend_fused && !c.fused_failed && !c.wav && !mixed &&
+    const bool affine = c.gn_fused && c.pend_fused && !c.fused_failed && !c.wav &&
                         fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth,
                                    c.fused_math, c.fused_min_blocks, &plan) &&
                         c.fused_scratch.bytes >= fused_scratch_bytes(plan, ld);

@@ -21,7 +21,7 @@ that error.
 import numpy as np
 import pytest
 
-from graada3d_amd.engine import GraError
+from graada3d_amd.engine import TIME_REFRESH, GraError
 from helpers import rel_norm
 from test_cgls_oracle import stacked
 from test_gpu_cg import _cg_inputs, small_inversion
@@ -69,13 +69,23 @@ def test_mixed_converges_to_the_fp64_solution(ctx, inner, fused, quirks, wl):
     R = 10
     inv.mixed_precision(inner, R)
     try:
+        inv.ctx.timing(True)
+        inv.ctx.timing_reset()
         dx, err, its = _solve(inv, p, quirks, fused)
+        _, refresh_passes = inv.ctx.timing_get(TIME_REFRESH)
+        inv.ctx.timing(False)
         refreshes = inv.ctx.tuning()["last_refreshes"]
         dx2, err2, its2 = _solve(inv, p, quirks, fused)
     finally:
+        inv.ctx.timing(False)
         inv.mixed_precision("f64")
     assert its < 2000 and err < TOL
     assert its % R == 0 and refreshes == its // R      # it stopped on a refreshed residual
+    # with the single-read iteration each refresh is one affine single-read
+    # pass over the FP64 G (G x and G^T w from one read), otherwise two; the
+    # host queues every refresh point up to maxit (2,000) -- those after the
+    # device-side stop return at once -- and times each pass
+    assert refresh_passes == (1 if fused else 2) * (2000 // R)
     # with the quirk pair A_r^T A_t is not symmetric and CG's attainable
     # accuracy is lower: the FP64 solve itself lands ~2e-9 from the direct one
     bar = 1e-8 if quirks else 1e-9
