@@ -647,6 +647,7 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
     bool have_q = have_Gmk && c.gn_q_ready && c.gn_q && c.gn_q_groups > 0;
     const void *scratch_in = c.fused_scratch.p;
     c.gn_q_ready = false;
+    c.pend_affine = false;
     const bool crg = a.use_cross_gradient != 0;
     if (crg && !c.have_crg_ops)
         return set_error(c, GRA_BAD_ARG, "cross-gradient requested but no T operators uploaded");
@@ -1030,7 +1031,7 @@ gra_status cgls_collect(Ctx &c, double *err, int *iterations, bool *rerun)
     const CgScalars &hs = hst->sc;
     c.last_solve_fused = c.pend_fused;
     *rerun = false;
-    if (c.pend_fused && hst->ferr) {
+    if ((c.pend_fused || c.pend_affine) && hst->ferr) {
         c.fused_failed = true;
         c.fused_fail_count += 1;
         *rerun = true;

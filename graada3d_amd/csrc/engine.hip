@@ -1357,17 +1357,27 @@ gra_status gra_gn_setup_problem(gra_ctx *ctx, const gra_gn_setup *s)
 
 namespace gra {
 namespace {
+// After a two-pass solve the misfit product is an affine single-read pass only
+// on a G of at least this many bytes: there one pass less per step saves
+// milliseconds (2 % of the steps on a 94 GB G), while on a 1.28 GB G the
+// cooperative pass (200 us, with launch gaps of 10-15 us either side) and the
+// two ordinary products (185 + 178 us) came out even.
+constexpr double kAffineMinBytes = 4e9;
+
 // One Gauss-Newton step queued on the context stream (no host
 // synchronisation): its results land in c.host_stage.  When the step may use
-// the fused product, the model and the IRLS weights are first copied aside so
-// that a failed fused wait can be undone (gn_collect).
+// a single-read (cooperative) pass -- the fused iteration or the affine
+// misfit pass -- the model and the IRLS weights are first copied aside so
+// that a failed wait can be undone (gn_collect).
 gra_status gn_enqueue(Ctx &c, double lambda, double cg_tol, int cg_maxit)
 {
     if (!c.gn_ready) return set_error(c, GRA_BAD_ARG, "gra_gn_setup_problem not called");
     if (!c.Ws.present || c.Ws.n != c.Nm)
         return set_error(c, GRA_BAD_ARG, "regularisation operators not uploaded for this Nm");
     if (cg_maxit < 0 || !(lambda >= 0)) return set_error(c, GRA_BAD_ARG, "maxit/lambda");
-    c.gn_backup = c.gn_fused && !c.fused_failed;
+    // the step's misfit product may be an affine single-read pass (below)
+    const bool big_G = (double)c.Nd * (double)c.ld * (double)dtype_size(c.store) >= kAffineMinBytes;
+    c.gn_backup = !c.fused_failed && (c.gn_fused || (big_G && !c.wav));
     if (c.gn_backup) {
         const size_t bm = sizeof(double) * (size_t)c.ld;
         HIPCHK(c.gn_m_prev.reserve(bm), "alloc model backup");
@@ -1420,24 +1430,39 @@ gra_status gn_enqueue(Ctx &c, double lambda, double cg_tol, int cg_maxit)
            "barrier_update");
     const GemvShape shape = gemv_shape(c);
     HIPCHK(c.part_n.reserve(sizeof(double) * (size_t)shape.n_splits * nd), "alloc");
-    // With the single-read iteration the misfit product is an affine
-    // single-read pass: G m_trial (into part_n, one split) and, from the same
-    // read of G, G^T (Wd^2 (G m_trial - dobs)) for the next solve's initial
-    // residual.  Its bounded waits report through the solve's status flag.
-    // (With the mixed-precision copy the pass reads the FP64 G, as the
-    // misfit and the initial residual of that mode do.)
+    // The misfit product is an affine single-read pass where a grid fits:
+    // G m_trial (into part_n, one split) and, from the same read of G,
+    // G^T (Wd^2 (G m_trial - dobs)) for the next solve's initial residual --
+    // one pass over G instead of two per step, after single-read and
+    // two-pass solves alike (launch_fused orders the cooperative grids of a
+    // device, also across the streams of gra_gn_step_batch).  Its bounded
+    // waits report through the solve's status flag, and a failed wait redoes
+    // the step (gn_collect).  (With
+    // the mixed-precision copy the pass reads the FP64 G, as the misfit and
+    // the initial residual of that mode do.)
     FusedPlan plan;
-    const bool affine = c.gn_fused && c.pend_fused && !c.fused_failed && !c.wav &&
-                        fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth,
-                                   c.fused_math, c.fused_min_blocks, &plan) &&
-                        c.fused_scratch.bytes >= fused_scratch_bytes(plan, ld);
+    const bool two_pass_step = !(c.gn_fused && c.pend_fused);
+    bool affine = c.gn_backup && !c.wav && (!two_pass_step || big_G) &&
+                  fused_plan(c.store, nd, ld, c.device, c.fused_wgs, c.fused_depth, c.fused_math,
+                             c.fused_min_blocks, &plan);
+    if (affine && c.fused_scratch.bytes < fused_scratch_bytes(plan, ld)) {
+        // a two-pass solve reserved none: the next solve reads gn_q from here
+        if (!two_pass_step || c.fused_scratch.reserve(fused_scratch_bytes(plan, ld)) != hipSuccess)
+            affine = false;
+    }
+    c.pend_affine = affine;
     if (affine) {
         int *fused_err = c.status.as<int>() + 1;
+        if (two_pass_step)     // the solve did not clear the flag
+            HIPCHK(hipMemsetAsync(fused_err, 0, sizeof(int), c.stream), "clear status");
         double *zp = nullptr, *t2p = nullptr;
-        HIPCHK(launch_fused(plan, c.G.p, nd, ld, mt, c.gn_Wd.as<double>(), nullptr,
-                            c.fused_scratch.p, fused_err, nullptr, c.fused_fail_block, &zp, &t2p,
-                            c.stream, c.gn_dobs.as<double>(), c.part_n.as<double>()),
-               "affine single-read pass");
+        {
+            TimedRegion tr(c, kTimeAffine);
+            HIPCHK(launch_fused(plan, c.G.p, nd, ld, mt, c.gn_Wd.as<double>(), nullptr,
+                                c.fused_scratch.p, fused_err, nullptr, c.fused_fail_block, &zp,
+                                &t2p, c.stream, c.gn_dobs.as<double>(), c.part_n.as<double>()),
+                   "affine single-read pass");
+        }
         HIPCHK(launch_misfit(c.part_n.as<double>(), 1, nd, c.gn_Wd.as<double>(),
                              c.gn_dobs.as<double>(), c.red.as<double>(), dmis, c.stream),
                "misfit");

@@ -72,7 +72,7 @@ struct GemvShape {
 // Kernel timing classes (gra_timing_get).
 enum TimingClass {
     kTimeGemvN = 0, kTimeGemvT = 1, kTimeAssemble = 2, kTimeCgIter = 3, kTimeFused = 4,
-    kTimeMatMul = 5, kTimeRefresh = 6, kTimeCgGraph = 7, kTimeClasses = 8
+    kTimeMatMul = 5, kTimeRefresh = 6, kTimeCgGraph = 7, kTimeAffine = 8, kTimeClasses = 9
 };
 
 // Launch plan of the single-read fused product (fused_kernels.hip).
@@ -147,6 +147,7 @@ struct Ctx {
     HostStage *host_stage = nullptr;     // pinned (hipHostMalloc at context creation)
     // the solve / Gauss-Newton step whose results host_stage will hold
     bool pend_fused = false;
+    bool pend_affine = false;            // the queued step's misfit is an affine single-read pass
     int pend_maxit = 0, pend_refresh = 0;
     bool gn_backup = false;              // gn_m_prev / gn_WL_prev hold the step's inputs
     bool gn_backup_wl = false, gn_backup_partn = false;

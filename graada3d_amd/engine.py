@@ -165,7 +165,7 @@ class Context:
 
 
 TIME_GEMV_N, TIME_GEMV_T, TIME_ASSEMBLE, TIME_CG_ITER, TIME_FUSED, TIME_MAT_MUL = 0, 1, 2, 3, 4, 5
-TIME_REFRESH, TIME_CG_GRAPH = 6, 7
+TIME_REFRESH, TIME_CG_GRAPH, TIME_AFFINE = 6, 7, 8
 
 
 def csr_struct(m, keep: list) -> GraCsr:

@@ -496,7 +496,9 @@ gra_status gra_autotune(gra_ctx *ctx, int reps, double *ms_n, double *ms_t);
  * 4 = fused single-read product (G Pp and G^T Wd^2 G Pp), 5 = the
  * matrix-core multi-right-hand-side product (gra_G_mat_mul / gra_GT_mat_mul),
  * 6 = the FP64 G products of the mixed-precision residual refreshes,
- * 7 = one replay of the captured CG iterations (GRA_TUNE_CG_GRAPH).
+ * 7 = one replay of the captured CG iterations (GRA_TUNE_CG_GRAPH),
+ * 8 = the affine single-read pass of a Gauss-Newton step (G m_trial for the
+ * misfit and G^T Wd^2 (G m_trial - dobs) for the next solve, one read of G).
  * Timing events are recorded only while enabled: `on` = 0 disables; n >= 1
  * times every launch of these classes, except inside the CG loop, where only
  * iterations 0, n, 2n, ... are timed (each event is a queue packet of its

@@ -27,7 +27,7 @@ import pytest
 
 import oracle
 from graada3d_amd import GaussNewtonInversion
-from graada3d_amd.engine import assemble_batch
+from graada3d_amd.engine import TIME_AFFINE, assemble_batch
 from graada3d_amd.engine import Context
 from graada3d_amd.workloads import (batch_instances, batch_regularisation, forward_problem_small,
                                     inversion_small, tensor_problem_large)
@@ -137,6 +137,49 @@ def test_large_tensor_rows_forward_and_adjoint(large):
     w = rng.standard_normal(inv.Nd)
     Gx, GTw = inv.G_vec_mul(x), inv.GT_vec_mul(w)
     assert abs(Gx @ w - x @ GTw) <= 1e-12 * np.linalg.norm(Gx) * np.linalg.norm(w)
+
+
+def test_large_tensor_affine_misfit_pass_matches_two_products(large):
+    """On a G this size a two-pass Gauss-Newton step takes its misfit and the
+    next solve's initial residual from one affine single-read pass (G m and
+    G^T (Wd^2 (G m - dobs)) from one read of G) instead of G m and G^T w;
+    the steps must agree with the two-product form (switched off through
+    the single-read plan's minimum of row blocks) to rounding."""
+    fp, inv = large
+    nobs = fp.points.shape[0]
+    inv.compute_G("f64")
+    rng = np.random.default_rng(8)
+    dobs = inv.G_vec_mul(fp.rho) * (1.0 + 0.01 * rng.standard_normal(inv.Nd))
+    Wd = np.full(inv.Nd, 1.0 / np.abs(dobs).max())
+    from graada3d_amd.operators import depth_weights_offset, weighting_operators
+    W = weighting_operators(fp.prisms, 64, 64, 32, depth_weights_offset(fp.prisms, 50.0, 3.0),
+                            a_s=1e-3)
+    inv.set_regularization(*W)
+    nm = inv.Nm
+
+    def run(min_blocks):
+        inv.ctx.tune(fused_min_blocks=min_blocks)
+        inv.gn_setup(dobs, Wd, np.full(nm, -1500.0), np.full(nm, 1500.0), np.zeros(nm),
+                     np.zeros(nm), lp=1, eps2=1e-3, fused=False)
+        inv.ctx.timing(True)
+        inv.ctx.timing_reset()
+        out = [inv.gn_step(100.0 * 0.5 ** k, 0.0, 10) for k in range(3)]
+        passes = inv.ctx.timing_get(TIME_AFFINE)[1]
+        inv.ctx.timing(False)
+        return inv.gn_model(), out, passes
+
+    try:
+        m_a, out_a, passes_a = run(0)
+        m_b, out_b, passes_b = run(1000000)
+    finally:
+        inv.ctx.timing(False)
+        inv.ctx.tune(fused_min_blocks=0)
+    assert (passes_a, passes_b) == (3, 0)
+    assert [o[2] for o in out_a] == [o[2] for o in out_b] == [10, 10, 10]
+    for (mis_a, err_a, _), (mis_b, err_b, _) in zip(out_a, out_b):
+        assert abs(mis_a - mis_b) <= 1e-12 * mis_b
+        assert abs(err_a - err_b) <= 1e-9 * err_b
+    assert rel_norm(m_a, m_b) < 1e-11
 
 
 def test_large_tensor_reduced_storage_error_bounds(large):
