@@ -376,10 +376,16 @@ __device__ inline bool cg_done_block(const CgDev &d)
 }
 
 // Iteration it: end iteration it-1 (FIRST: it = 0, beta = 0 from the initial
-// scalars), then p = -r + beta p; Pp = P p   (GN.cpp:1558-1560)
+// scalars), then p = -r + beta p; Pp = P p   (GN.cpp:1558-1560).  With the
+// single-read iteration it also resets the fused product's nfill partials to
+// the sentinel (the previous product has finished reading them).
 template <bool FIRST>
-__global__ void __launch_bounds__(kB) k_p_update(CgDev d, int64_t nblk, int it, double tol)
+__global__ void __launch_bounds__(kB) k_p_update(CgDev d, int64_t nblk, int it, double tol,
+                                                 uint64_t *fill, int64_t nfill)
 {
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < nfill;
+         i += (int64_t)gridDim.x * kB)
+        fill[i] = kFusedSentinel;
     if (cg_done_block(d)) return;
     if (d.dev_it) it = d.sc->it_dev;
     double beta;
@@ -620,16 +626,18 @@ __global__ void k_irls(const int64_t *rp, const int32_t *ci, const double *v, in
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 
 // Iteration it's p update; from it = 1 on it also ends iteration it-1 from
-// the nblk_r partials of its residual.
+// the nblk_r partials of its residual, and fills `nfill` words at `fill` with
+// the fused product's sentinel.
 inline void launch_p_update(const CgDev &d, int64_t nm, int64_t nblk_r, int it, double tol,
-                            hipStream_t st)
+                            hipStream_t st, uint64_t *fill = nullptr, int64_t nfill = 0)
 {
+    const unsigned grid = nblocks(nm);
     if (it == 0)
-        hipLaunchKernelGGL(k_p_update<true>, dim3(nblocks(nm)), dim3(kB), 0, st, d, nblk_r, it,
-                           tol);
+        hipLaunchKernelGGL(k_p_update<true>, dim3(grid), dim3(kB), 0, st, d, nblk_r, it, tol, fill,
+                           nfill);
     else
-        hipLaunchKernelGGL(k_p_update<false>, dim3(nblocks(nm)), dim3(kB), 0, st, d, nblk_r, it,
-                           tol);
+        hipLaunchKernelGGL(k_p_update<false>, dim3(grid), dim3(kB), 0, st, d, nblk_r, it, tol, fill,
+                           nfill);
 }
 
 }  // namespace
@@ -967,12 +975,14 @@ gra_status cgls_solve(Ctx &c, const gra_cg_args &a, const double *d_mk, const do
         const int64_t nblk_reg = nblocks(nreg);
         for (int it = 0; it < a.maxit; ++it) {
             TimedRegion tr_it(c, kTimeCgIter, it);
-            launch_p_update(di, nm, nblk_r, it, tol_at(it), st);
+            launch_p_update(di, nm, nblk_r, it, tol_at(it), st,
+                            static_cast<uint64_t *>(c.fused_scratch.p), fused_partials(plan));
             {
                 TimedRegion tr(c, kTimeFused, it);
                 double *zp = nullptr, *t2p = nullptr;
                 e = launch_fused(plan, G_in, nd, ld, di.Pp, dg.Wd, di.t, c.fused_scratch.p,
-                                 fused_err, done, c.fused_fail_block, &zp, &t2p, st);
+                                 fused_err, done, c.fused_fail_block, &zp, &t2p, st, nullptr,
+                                 nullptr, true);
                 if (e != hipSuccess) return hip_fail(c, e, "fused product");
                 di.zpart = zp;
                 di.t2part = t2p;

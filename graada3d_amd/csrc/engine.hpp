@@ -344,13 +344,21 @@ hipError_t launch_expand_G(const void *G, gra_dtype dt, int64_t Nd, int64_t Nm, 
 bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs, int depth,
                 int math, int min_blocks, FusedPlan *out);
 size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld);
+// The fused product's cross-workgroup partials: the first fused_partials(p)
+// 64-bit words of its scratch, which must hold kFusedSentinel (a signalling
+// NaN) when a launch starts.
+constexpr uint64_t kFusedSentinel = 0x7FF4DEADBEEF0001ull;
+int64_t fused_partials(const FusedPlan &p);
 // fail_block >= 0 makes the wait on that row block fail (test hook).
 // dobs / yout: the affine pass (q = G^T (Wd^2 (G x - dobs)), y = G x to yout).
+// partials_filled: the caller has queued the sentinel fill itself (the CG
+// iteration does it inside its p update, one launch fewer per iteration);
+// otherwise launch_fused queues a fill kernel before the grid.
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
                         const double *x, const double *Wd, double *tdat, void *scratch, int *err,
                         const int *done, int64_t fail_block, double **zpart_out,
                         double **t2part_out, hipStream_t st, const double *dobs = nullptr,
-                        double *yout = nullptr);
+                        double *yout = nullptr, bool partials_filled = false);
 
 // CGLS (cgls.hip)
 struct CgDiag {

@@ -70,7 +70,7 @@ constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxPoll = 4;            // partials per lane of the polling wave
 // Sentinel of unpublished partials (a signalling NaN).
-constexpr uint64_t kSentinel = 0x7FF4DEADBEEF0001ull;
+constexpr uint64_t kSentinel = kFusedSentinel;
 
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 
@@ -543,10 +543,12 @@ bool fused_plan(gra_dtype dt, int64_t Nd, int64_t ld, int device, int target_wgs
     return true;
 }
 
+int64_t fused_partials(const FusedPlan &p) { return (int64_t)p.S * p.nrb_group * p.NT * p.RB; }
+
 size_t fused_scratch_bytes(const FusedPlan &p, int64_t ld)
 {
     int64_t nrb = (int64_t)p.S * p.nrb_group;
-    return sizeof(double) * (size_t)(nrb * p.NT * p.RB + nrb) +   // partial, t2part
+    return sizeof(double) * (size_t)(fused_partials(p) + nrb) +   // partial, t2part
            sizeof(double) * (size_t)p.S * (size_t)ld;               // zpart
 }
 
@@ -566,7 +568,8 @@ CoopOrder coop_order[kMaxDevices];
 hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t ld,
                         const double *x, const double *Wd, double *tdat, void *scratch, int *err,
                         const int *done, int64_t fail_block, double **zpart_out,
-                        double **t2part_out, hipStream_t st, const double *dobs, double *yout)
+                        double **t2part_out, hipStream_t st, const double *dobs, double *yout,
+                        bool partials_filled)
 {
     const int64_t nrb = (int64_t)p.S * p.nrb_group;
     char *b = static_cast<char *>(scratch);
@@ -594,9 +597,10 @@ hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t l
     a.done = done;
     a.wait_ticks = (int64_t)p.wait_ticks;
     a.fail_block = fail_block;
-    const int64_t npart = nrb * p.NT * p.RB;
-    hipLaunchKernelGGL(fill_u64, dim3((unsigned)((npart + 255) / 256)), dim3(256), 0, st,
-                       reinterpret_cast<uint64_t *>(a.partial), npart, kSentinel);
+    const int64_t npart = fused_partials(p);
+    if (!partials_filled)
+        hipLaunchKernelGGL(fill_u64, dim3((unsigned)((npart + 255) / 256)), dim3(256), 0, st,
+                           reinterpret_cast<uint64_t *>(a.partial), npart, kSentinel);
     if (zpart_out) *zpart_out = a.zpart;
     if (t2part_out) *t2part_out = a.t2part;
     // One cooperative grid at a time per device, in launch order, whatever
@@ -610,7 +614,11 @@ hipError_t launch_fused(const FusedPlan &p, const void *G, int64_t Nd, int64_t l
     std::lock_guard<std::mutex> lock(coop_mu);
     CoopOrder &o = coop_order[dev];
     if (!o.last) {
-        if ((e = hipEventCreateWithFlags(&o.last, hipEventDisableTiming)) != hipSuccess) return e;
+        // ordering only, waited on by streams of this device: no system-scope
+        // release needed at the record
+        if ((e = hipEventCreateWithFlags(&o.last, hipEventDisableTiming | hipEventDisableSystemFence)) !=
+            hipSuccess)
+            return e;
     } else if (o.stream != st) {
         if ((e = hipStreamWaitEvent(st, o.last, 0)) != hipSuccess) return e;
     }
