@@ -96,6 +96,24 @@ __device__ inline double block_sum(double v)
     return s;
 }
 
+// p[0] + p[ld] + ... + p[(n-1) ld], added in that order; the loads are issued
+// eight at a time (one lane may sum a hundred row-group partials, and one
+// load in flight at a time left the sum latency-bound).
+__device__ inline double sum_strided(const double *p, int n, int64_t ld)
+{
+    double v = p[0];
+    int g = 1;
+    for (; g + 7 < n; g += 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = p[(int64_t)(g + u) * ld];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v += t[u];
+    }
+    for (; g < n; ++g) v += p[(int64_t)g * ld];
+    return v;
+}
+
 // Sum of n partials by one block, fixed order.
 __device__ inline double sum_partials(const double *red, int64_t n)
 {
@@ -229,8 +247,7 @@ __global__ void __launch_bounds__(kB) k_r_update(CgDev d)
     int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
     double r2 = 0.0, atb2 = 0.0;
     if (j < d.nm) {
-        double gsum = d.part_t[j];
-        for (int c = 1; c < d.chunks; ++c) gsum += d.part_t[(int64_t)c * d.ld + j];
+        double gsum = sum_strided(d.part_t + j, d.chunks, d.ld);
         double Pj = d.P[j];
         double r = Pj * gsum;
         double rl = r;
@@ -284,13 +301,8 @@ __global__ void __launch_bounds__(kB) k_r_loop(CgDev d)
     double v = 0.0;
     if (ok) {
         if (sub == 0) {
-            if (FUSED) {
-                v = d.zpart[j];
-                for (int g = 1; g < d.zgroups; ++g) v += d.zpart[(int64_t)g * d.ld + j];
-            } else {
-                v = d.part_t[j];
-                for (int c = 1; c < d.chunks; ++c) v += d.part_t[(int64_t)c * d.ld + j];
-            }
+            if (FUSED) v = sum_strided(d.zpart + j, d.zgroups, d.ld);
+            else v = sum_strided(d.part_t + j, d.chunks, d.ld);
         } else if (sub <= d.nseg) {
             const int q = sub - 1;
             const double *seg = (FUSED ? d.t : d.s) + d.nd;
